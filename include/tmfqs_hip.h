@@ -1,0 +1,193 @@
+/*
+ * tmfqs_hip.h -- C-ABI of the MI355X (gfx950) dense state-vector engine.
+ *
+ * One opaque handle owns one state vector in HBM (complex128 or complex64,
+ * interleaved re/im, basis index bit n-1-q holds qubit q), one HIP stream,
+ * pinned staging buffers and device scalars.  The entry points below are the
+ * backend operations of the reference's storage plugin interface
+ * (include/tmfqs/storage/i_state_backend.h in the reference) plus the few
+ * register-level operations the device backend accelerates; each declaration
+ * cites the reference interface it replaces.
+ *
+ * Conventions
+ *   - Every function returns a tmfqs_hip_status; 0 is success.  Negative codes
+ *     map 1:1 onto the exceptions the reference throws (std::out_of_range,
+ *     std::invalid_argument, std::logic_error, std::runtime_error,
+ *     std::bad_alloc).  tmfqs_hip_last_error() returns the message of the
+ *     calling thread's last failure.
+ *   - Qubit arguments use the reference's qubit numbering (qubit 0 is the most
+ *     significant bit of the basis index).  State indices are 64-bit.
+ *   - Mutating calls are queued on the handle's stream and may be fused with
+ *     neighbouring calls; calls that return values (amplitude, probability,
+ *     sample, export, sums) flush the queue and synchronise first.
+ *   - A handle must not be used by two threads at once; distinct handles may.
+ *   - Host pointers are borrowed for the duration of the call only.
+ *   - No exception crosses this boundary.
+ */
+#ifndef TMFQS_HIP_H
+#define TMFQS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TMFQS_HIP_ABI_VERSION 1
+
+typedef struct tmfqs_hip_state tmfqs_hip_state;
+
+typedef enum tmfqs_hip_status {
+	TMFQS_HIP_OK = 0,
+	TMFQS_HIP_ERR_OUT_OF_RANGE = -1,      /* std::out_of_range */
+	TMFQS_HIP_ERR_INVALID_ARGUMENT = -2,  /* std::invalid_argument */
+	TMFQS_HIP_ERR_LOGIC = -3,             /* std::logic_error ("not initialized") */
+	TMFQS_HIP_ERR_RUNTIME = -4,           /* std::runtime_error */
+	TMFQS_HIP_ERR_OUT_OF_MEMORY = -5,     /* std::bad_alloc */
+	TMFQS_HIP_ERR_DEVICE = -6             /* HIP runtime failure (std::runtime_error) */
+} tmfqs_hip_status;
+
+typedef enum tmfqs_hip_precision {
+	TMFQS_HIP_COMPLEX128 = 0, /* FP64 amplitudes; matches the reference dense backend */
+	TMFQS_HIP_COMPLEX64 = 1   /* FP32 amplitudes; FP64 reductions, means and phase factors */
+} tmfqs_hip_precision;
+
+/* Backend configuration: the device analogue of the reference's per-backend
+ * config structs (include/tmfqs/config/register_config.h:41-91). */
+typedef struct tmfqs_hip_config {
+	int precision;       /* tmfqs_hip_precision */
+	int device;          /* HIP device ordinal */
+	int fusion;          /* 1: queue and fuse operations into tiled passes; 0: one sweep per call */
+	int tile_qubits;     /* tiled-pass tile size in qubits (0: automatic) */
+	int coalesce_qubits; /* low qubits always inside a tile (0: automatic) */
+	int grover_block;    /* most elementwise steps folded into one sweep (0: automatic) */
+	int exact_phases;    /* 1: apply every controlled phase as its own complex product
+	                        (bit-identical to the reference); 0: fuse QFT phase ladders */
+} tmfqs_hip_config;
+
+/* One circuit operation (the reference's circuit::Operation variant,
+ * include/tmfqs/circuit/circuit.h:16-62, as a flat record). */
+typedef enum tmfqs_hip_op_kind {
+	TMFQS_HIP_OP_H = 0,
+	TMFQS_HIP_OP_X = 1,
+	TMFQS_HIP_OP_CX = 2,
+	TMFQS_HIP_OP_CPHASE = 3,
+	TMFQS_HIP_OP_SWAP = 4,
+	TMFQS_HIP_OP_PHASE_FLIP = 5,
+	TMFQS_HIP_OP_DIFFUSE_LAZY = 6,
+	TMFQS_HIP_OP_DIFFUSE_MATERIALIZED = 7,
+	TMFQS_HIP_OP_MATRIX = 8
+} tmfqs_hip_op_kind;
+
+typedef struct tmfqs_hip_op {
+	int32_t kind;     /* tmfqs_hip_op_kind */
+	uint32_t q0;      /* target (H, X); control (CX, CPHASE); first (SWAP); MATRIX: offset into qubit pool */
+	uint32_t q1;      /* target (CX, CPHASE); second (SWAP) */
+	uint32_t k;       /* MATRIX: number of target qubits */
+	double theta;     /* CPHASE angle (radians) */
+	uint64_t state;   /* PHASE_FLIP basis state; MATRIX: offset (in doubles) into matrix pool */
+} tmfqs_hip_op;
+
+/* ---- library ---------------------------------------------------------- */
+const char *tmfqs_hip_last_error(void);
+int tmfqs_hip_abi_version(void);
+int tmfqs_hip_device_count(int *count);
+void tmfqs_hip_config_default(tmfqs_hip_config *cfg);
+
+/* ---- lifecycle -------------------------------------------------------- */
+/* createDenseStateBackend / createBloscStateBackend factory pattern
+ * (src/tmfqs/storage/blosc/blosc_state_backend.cpp:14-21).  The handle holds
+ * no state until one of the init/load calls below. */
+int tmfqs_hip_create(const tmfqs_hip_config *cfg, tmfqs_hip_state **out);
+int tmfqs_hip_destroy(tmfqs_hip_state *h);
+/* IStateBackend::clone (i_state_backend.h:31). */
+int tmfqs_hip_clone(tmfqs_hip_state *h, tmfqs_hip_state **out);
+int tmfqs_hip_qubit_count(tmfqs_hip_state *h, unsigned *out);
+int tmfqs_hip_get_config(tmfqs_hip_state *h, tmfqs_hip_config *out);
+
+/* ---- initialisation and I/O ------------------------------------------- */
+/* IStateBackend::initZero (i_state_backend.h:37). */
+int tmfqs_hip_init_zero(tmfqs_hip_state *h, unsigned num_qubits);
+/* IStateBackend::initBasis (i_state_backend.h:44). */
+int tmfqs_hip_init_basis(tmfqs_hip_state *h, unsigned num_qubits, uint64_t state, double re, double im);
+/* IStateBackend::initUniformSuperposition (i_state_backend.h:50). */
+int tmfqs_hip_init_uniform(tmfqs_hip_state *h, unsigned num_qubits, const uint64_t *states, size_t count);
+/* IStateBackend::loadAmplitudes (i_state_backend.h:56): elem_count = 2 * 2^n doubles. */
+int tmfqs_hip_load(tmfqs_hip_state *h, unsigned num_qubits, const double *interleaved, uint64_t elem_count);
+/* IStateBackend::exportAmplitudes / forEachAmplitudeChunk (i_state_backend.h:61,66):
+ * copies states [first, first + count) as interleaved FP64 into out. */
+int tmfqs_hip_export(tmfqs_hip_state *h, uint64_t first, uint64_t count, double *out);
+/* IStateBackend::amplitude (i_state_backend.h:73). */
+int tmfqs_hip_amplitude(tmfqs_hip_state *h, uint64_t state, double *re, double *im);
+/* IStateBackend::setAmplitude (i_state_backend.h:79). */
+int tmfqs_hip_set_amplitude(tmfqs_hip_state *h, uint64_t state, double re, double im);
+/* IStateBackend::probability (i_state_backend.h:85). */
+int tmfqs_hip_probability(tmfqs_hip_state *h, uint64_t state, double *out);
+/* IStateBackend::totalProbability (i_state_backend.h:90). */
+int tmfqs_hip_total_probability(tmfqs_hip_state *h, double *out);
+/* IStateBackend::sampleMeasurement (i_state_backend.h:96): first state whose
+ * sequential FP64 cumulative probability reaches rnd (bit-exact index). */
+int tmfqs_hip_sample(tmfqs_hip_state *h, double rnd, uint64_t *out);
+
+/* ---- register-level reads the device accelerates ----------------------- */
+/* QuantumRegister::logicalAmplitudeSum without an overlay
+ * (src/tmfqs/register/quantum_register.cpp:176-189). */
+int tmfqs_hip_amplitude_sum(tmfqs_hip_state *h, double *re, double *im);
+/* QuantumRegister::totalProbability with the lazy affine overlay active
+ * (quantum_register.cpp:269-281): sum |scale*a + bias|^2. */
+int tmfqs_hip_total_probability_affine(tmfqs_hip_state *h, double s_re, double s_im, double b_re, double b_im,
+                                       double *out);
+/* QuantumRegister::measure with the lazy affine overlay active
+ * (quantum_register.cpp:284-310): no range check on rnd, as there. */
+int tmfqs_hip_sample_affine(tmfqs_hip_state *h, double rnd, double s_re, double s_im, double b_re, double b_im,
+                            uint64_t *out);
+
+/* ---- operations ------------------------------------------------------- */
+/* IStateBackend::phaseFlipBasisState (i_state_backend.h:110). */
+int tmfqs_hip_phase_flip(tmfqs_hip_state *h, uint64_t state);
+/* IStateBackend::inversionAboutMean() (i_state_backend.h:112). */
+int tmfqs_hip_inversion_about_mean(tmfqs_hip_state *h);
+/* IStateBackend::inversionAboutMean(mean) (i_state_backend.h:117). */
+int tmfqs_hip_inversion_about_mean_value(tmfqs_hip_state *h, double mean_re, double mean_im);
+/* IStateBackend::applyAffineTransform (i_state_backend.h:123). */
+int tmfqs_hip_affine(tmfqs_hip_state *h, double s_re, double s_im, double b_re, double b_im);
+/* IStateBackend::applyHadamard (i_state_backend.h:128). */
+int tmfqs_hip_hadamard(tmfqs_hip_state *h, unsigned qubit);
+/* IStateBackend::applyPauliX (i_state_backend.h:133). */
+int tmfqs_hip_pauli_x(tmfqs_hip_state *h, unsigned qubit);
+/* IStateBackend::applyControlledPhaseShift (i_state_backend.h:140). */
+int tmfqs_hip_controlled_phase(tmfqs_hip_state *h, unsigned control, unsigned target, double theta);
+/* IStateBackend::applyControlledNot (i_state_backend.h:146). */
+int tmfqs_hip_controlled_not(tmfqs_hip_state *h, unsigned control, unsigned target);
+/* QuantumRegister::applySwap (quantum_register.cpp:518-527): bit-identical to
+ * its three controlled-NOTs, executed as one bit permutation. */
+int tmfqs_hip_swap(tmfqs_hip_state *h, unsigned q1, unsigned q2);
+/* IStateBackend::applyGate (i_state_backend.h:153): mat is a row-major
+ * interleaved 2^k x 2^k complex matrix, qubits[0] the MSB of its index. */
+int tmfqs_hip_apply_gate(tmfqs_hip_state *h, const double *mat, unsigned k, const unsigned *qubits);
+/* IStateBackend::beginOperationBatch / endOperationBatch (i_state_backend.h:159-163):
+ * end flushes every queued operation. */
+int tmfqs_hip_begin_batch(tmfqs_hip_state *h);
+int tmfqs_hip_end_batch(tmfqs_hip_state *h);
+/* Flushes the queue and waits for the device. */
+int tmfqs_hip_synchronize(tmfqs_hip_state *h);
+
+/* ---- observability ---------------------------------------------------- */
+/* Per-kernel device timing with HIP events on the handle's stream (the device
+ * analogue of ObservabilityConfig.traceOperations, simulator.h:16-19). */
+int tmfqs_hip_profile_enable(tmfqs_hip_state *h, int on);
+int tmfqs_hip_profile_reset(tmfqs_hip_state *h);
+/* Kernel-name table: for i < *count writes names[i] (NUL-terminated, up to
+ * 63 chars), launches[i], milliseconds[i] and algorithmic bytes[i].  On entry
+ * *count is the capacity; on return the number of entries. */
+int tmfqs_hip_profile_read(tmfqs_hip_state *h, char (*names)[64], uint64_t *launches, double *milliseconds,
+                           double *bytes, size_t *count);
+/* Raw stream handle (hipStream_t) for callers that time with their own events. */
+int tmfqs_hip_stream(tmfqs_hip_state *h, void **stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TMFQS_HIP_H */

@@ -1,0 +1,271 @@
+"""GPU parity of the C-ABI backend against the CPU oracle.
+
+Tolerances: complex128 max-abs amplitude error <= 1e-12
+against the reference dense backend; per-operation sweeps and every fused
+operation except the QFT phase-ladder fusion reproduce it bit for bit
+(np.array_equal); measured indices are bit-exact.  complex64: fidelity error
+<= 1e-6 against the FP64 oracle.
+"""
+
+import numpy as np
+import pytest
+
+import oracle as O
+import tmfqsfullstate_amd as T
+from tests import numpy_ref as R
+from tests.register_driver import Register
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def st(precision=T.C128, **cfg):
+    return T.HipState(precision=precision, **cfg)
+
+
+def cplx(inter):
+    return inter[0::2] + 1j * inter[1::2]
+
+
+def rand_state(n, seed):
+    return O.random_phase(n, seed)
+
+
+MODES = [dict(fusion=0), dict(fusion=1, exact_phases=1), dict(fusion=1)]
+
+
+@pytest.mark.parametrize("mode", MODES, ids=["unfused", "fused-exact", "fused"])
+def test_single_ops_bitexact(mode):
+    n = 10
+    init = rand_state(n, 1)
+    ops = []
+    for q in range(n):
+        ops.append([(O.OP_H, q)])
+        ops.append([(O.OP_X, q)])
+    for c, t in [(0, 9), (9, 0), (3, 4), (5, 2), (8, 7)]:
+        ops.append([(O.OP_CX, c, t)])
+        ops.append([(O.OP_CPHASE, c, t, 0.37 + c)])
+        ops.append([(O.OP_SWAP, c, t)])
+    for seq in ops:
+        s = st(**mode).load(n, init)
+        s.run_gates(seq)
+        got = s.export()
+        want = O.run_circuit(n, seq, init=init).amplitudes()
+        assert np.array_equal(got, want), seq
+
+
+@pytest.mark.parametrize("mode", MODES, ids=["unfused", "fused-exact", "fused"])
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 5])
+def test_matrix_gates_bitexact(mode, k):
+    n = 9
+    rng = np.random.default_rng(k)
+    init = rand_state(n, k)
+    u = R.random_unitary(rng, 1 << k)
+    qubits = [int(q) for q in rng.choice(n, size=k, replace=False)]
+    s = st(**mode).load(n, init)
+    s.gate(u, qubits)
+    want = O.apply_gate(init, n, R.interleave(u), qubits)
+    assert np.array_equal(s.export(), want)
+
+
+@pytest.mark.parametrize("mode", MODES, ids=["unfused", "fused-exact", "fused"])
+def test_random_circuits(mode):
+    rng = np.random.default_rng(7)
+    for n in (6, 13):
+        init = rand_state(n, n)
+        seq = []
+        for _ in range(400):
+            kind = int(rng.integers(0, 5))
+            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+            seq.append((kind, a, b, float(rng.uniform(0, 2 * np.pi))))
+        s = st(**mode).load(n, init)
+        s.run_gates(seq)
+        got = s.export()
+        want = O.run_circuit(n, seq, init=init).amplitudes()
+        assert np.array_equal(got, want), (mode, n)
+
+
+@pytest.mark.parametrize("mode", MODES, ids=["unfused", "fused-exact", "fused"])
+@pytest.mark.parametrize("n", [1, 2, 3, 9, 12, 16])
+def test_qft(mode, n):
+    init = rand_state(n, 7)
+    ops, cnt = O.qft_ops(n)
+    s = st(**mode).load(n, init)
+    reg = Register(s, n)
+    reg.run(ops, cnt)
+    got = s.export()
+    want = O.run_circuit(n, (ops, cnt), init=init).amplitudes()
+    if mode.get("fusion", 1) == 0 or mode.get("exact_phases", 0):
+        assert np.array_equal(got, want)
+    else:
+        assert np.abs(cplx(got) - cplx(want)).max() <= TOL
+    dft = np.fft.ifft(cplx(init), norm="ortho")
+    assert np.abs(cplx(got) - dft).max() <= TOL
+
+
+def test_qft_known_answers():
+    # test_circuit.cpp:76-106
+    for src in (0, 5):
+        s = st().init_basis(3, src)
+        reg = Register(s, 3)
+        ops, cnt = O.qft_ops(3)
+        reg.run(ops, cnt)
+        y = np.arange(8)
+        assert np.allclose(s.amplitudes(), np.exp(2j * np.pi * src * y / 8) / np.sqrt(8), atol=1e-6)
+
+
+@pytest.mark.parametrize("materialized", [True, False])
+@pytest.mark.parametrize("mode", MODES, ids=["unfused", "fused-exact", "fused"])
+def test_grover_vs_oracle(materialized, mode):
+    for n, marked in ((10, [511]), (12, [3, 2047, 4000])):
+        ops, cnt = O.grover_ops(n, marked, materialized)
+        s = st(**mode).init_basis(n, 0)
+        reg = Register(s, n)
+        reg.run(ops, cnt)
+        got = reg.amplitudes()
+        oreg = O.run_circuit(n, (ops, cnt))
+        want = cplx(oreg.amplitudes())
+        assert np.abs(got - want).max() <= TOL
+        for seed in (123456, 1, 2, 3):
+            rnd = O.mt_unit_draws(seed, 1)[0]
+            assert reg.measure(rnd) == oreg.measure(rnd)
+        k = O.grover_iterations(n, len(set(marked)))
+        p = sum(abs(got[m]) ** 2 for m in set(marked))
+        assert abs(p - R.grover_p_marked(n, len(set(marked)), k)) < 1e-10
+
+
+@pytest.mark.parametrize("n", [4, 11, 14, 17])
+def test_sample_and_total_probability_bitexact(n):
+    init = rand_state(n, 3)
+    s = st().load(n, init)
+    ops, cnt = O.qft_ops(n)
+    Register(s, n).run(ops, cnt)
+    amps = s.export()
+    assert s.total_probability() == O.total_probability(amps, n)
+    rng = np.random.default_rng(n)
+    for rnd in list(rng.random(40)) + [0.0, 1e-300, 0.5, float(np.nextafter(1.0, 0.0))]:
+        try:
+            want = O.sample(amps, n, rnd)
+        except O.OracleError as e:
+            with pytest.raises(T.HipError) as got:
+                s.sample(rnd)
+            assert got.value.code == T.ERR_RUNTIME and e.code == O.RUNTIME_ERROR
+            continue
+        assert s.sample(rnd) == want, rnd
+
+
+def test_sample_boundaries_exact():
+    # rnd exactly at cumulative boundaries of the sequential chain
+    n = 12
+    init = rand_state(n, 9)
+    s = st().load(n, init)
+    p = init[0::2] ** 2 + init[1::2] ** 2
+    cum = 0.0
+    bounds = []
+    for i, pi in enumerate(p):
+        cum += pi
+        if i % 97 == 0:
+            bounds.append(cum)
+    for b in bounds:
+        for r in (b, np.nextafter(b, 0), np.nextafter(b, 2)):
+            if r >= 1.0:
+                continue
+            assert s.sample(float(r)) == O.sample(init, n, float(r))
+
+
+def test_affine_overlay_reads():
+    n = 8
+    init = rand_state(n, 4)
+    s = st().load(n, init)
+    scale, bias = complex(-0.7, 0.2), complex(0.01, -0.03)
+    oreg = O.Register(n, init)
+    oreg.inversion(False, complex(0.005, -0.015))  # overlay scale -1, bias 2m
+    got_tp = s.total_probability_affine(complex(-1.0, 0.0), complex(0.01, -0.03))
+    assert got_tp == oreg.total_probability()
+    for rnd in (0.1, 0.5, 0.93):
+        assert s.sample_affine(rnd, complex(-1, 0), complex(0.01, -0.03)) == oreg.measure(rnd)
+    s.affine(scale, bias)
+    want = init.copy()
+    O.lib().orc_affine(want.ctypes.data_as(O._dp), n, scale.real, scale.imag, bias.real, bias.imag)
+    assert np.array_equal(s.export(), want)
+
+
+def test_state_io_and_clone():
+    n = 7
+    s = st().init_uniform(n, [5, 3, 3, 100])
+    a = s.amplitudes()
+    assert np.allclose(a[[3, 5, 100]], 1 / np.sqrt(3)) and np.count_nonzero(a) == 3
+    c = s.clone()
+    s.h(0)
+    assert np.array_equal(c.amplitudes(), a)  # clone is independent
+    s.set_amplitude(9, 0.25 - 0.5j)
+    assert s.amplitude(9) == 0.25 - 0.5j
+    assert abs(s.probability(9) - 0.3125) < 1e-15
+    part = s.export(8, 4)
+    full = s.export()
+    assert np.array_equal(part, full[16:24])
+
+
+def test_errors():
+    s = st()
+    with pytest.raises(T.HipError) as e:
+        s.h(0)  # not initialised
+    assert e.value.code == T.ERR_LOGIC
+    s.init_basis(2, 0)
+    with pytest.raises(T.OutOfRange):
+        s.h(2)
+    with pytest.raises(T.OutOfRange):
+        s.amplitude(4)
+    with pytest.raises(T.InvalidArgument):
+        s.cx(1, 1)
+    with pytest.raises(T.InvalidArgument):
+        s.gate(R.CX, [1, 1])
+    with pytest.raises(T.OutOfRange):
+        s.gate(R.H, [2])
+    with pytest.raises(T.InvalidArgument):
+        s.sample(1.0)
+    with pytest.raises(T.InvalidArgument):
+        s.init_uniform(3, [])
+    with pytest.raises(T.OutOfRange):
+        s.init_uniform(3, [0, 8])
+    with pytest.raises(T.InvalidArgument):
+        s.load(2, np.zeros(6))
+
+
+def test_complex64_fidelity():
+    for n in (12, 18):
+        init = rand_state(n, 11)
+        s = st(T.C64).load(n, init)
+        ops, cnt = O.qft_ops(n)
+        Register(s, n).run(ops, cnt)
+        got = s.amplitudes()
+        want = np.fft.ifft(cplx(init), norm="ortho")
+        fid = abs(np.vdot(want, got)) ** 2
+        assert 1 - fid <= 1e-6, (n, 1 - fid)
+        assert np.abs(got - want).max() < 1e-5
+    n = 14
+    ops, cnt = O.grover_ops(n, [77, 9000], True)
+    s = st(T.C64).init_basis(n, 0)
+    reg = Register(s, n)
+    reg.run(ops, cnt)
+    got = reg.amplitudes()
+    want = cplx(O.run_circuit(n, (ops, cnt)).amplitudes())
+    fid = abs(np.vdot(want, got)) ** 2
+    assert 1 - fid <= 1e-6
+
+
+def test_qft_large_vs_fft():
+    # n = 24: a size-independent property (the DFT) rather than the oracle.
+    n = 24
+    rng = np.random.default_rng(0)
+    ph = rng.uniform(0, 2 * np.pi, 1 << n)
+    init = np.empty(2 << n)
+    init[0::2] = np.cos(ph) * 2.0 ** (-n / 2)
+    init[1::2] = np.sin(ph) * 2.0 ** (-n / 2)
+    s = st().load(n, init)
+    ops, cnt = O.qft_ops(n)
+    Register(s, n).run(ops, cnt)
+    got = s.amplitudes()
+    want = np.fft.ifft(cplx(init), norm="ortho")
+    assert np.abs(got - want).max() <= TOL

@@ -1,0 +1,351 @@
+"""MI355X (gfx950) dense state-vector engine: Python bindings.
+
+``HipState`` binds the C-ABI of ``lib/libtmfqs_hip.so`` (declared in
+``include/tmfqs_hip.h``) with ctypes; it is the handle a test or a Python
+caller uses in place of the reference's ``IStateBackend``.  ``Simulator``
+(module ``sim``) binds the C++ API's simulator entry points.
+
+Nothing here computes on the CPU: every operation runs on the GPU through the
+native library, and importing a binding raises if the library is missing.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(PKG, "lib")
+HIP_LIB = os.path.join(LIB_DIR, "libtmfqs_hip.so")
+
+C128, C64 = 0, 1
+
+OK = 0
+ERR_OUT_OF_RANGE = -1
+ERR_INVALID_ARGUMENT = -2
+ERR_LOGIC = -3
+ERR_RUNTIME = -4
+ERR_OUT_OF_MEMORY = -5
+ERR_DEVICE = -6
+
+OP_H, OP_X, OP_CX, OP_CPHASE, OP_SWAP, OP_PHASE_FLIP, OP_DIFFUSE_LAZY, OP_DIFFUSE_MAT, OP_MATRIX = range(9)
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("precision", ctypes.c_int),
+        ("device", ctypes.c_int),
+        ("fusion", ctypes.c_int),
+        ("tile_qubits", ctypes.c_int),
+        ("coalesce_qubits", ctypes.c_int),
+        ("grover_block", ctypes.c_int),
+        ("exact_phases", ctypes.c_int),
+    ]
+
+
+class Op(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("q0", ctypes.c_uint32),
+        ("q1", ctypes.c_uint32),
+        ("k", ctypes.c_uint32),
+        ("theta", ctypes.c_double),
+        ("state", ctypes.c_uint64),
+    ]
+
+
+class HipError(RuntimeError):
+    """Raised for a non-zero status; ``code`` is the tmfqs_hip_status."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class OutOfRange(HipError, IndexError):
+    pass
+
+
+class InvalidArgument(HipError, ValueError):
+    pass
+
+
+_EXC = {ERR_OUT_OF_RANGE: OutOfRange, ERR_INVALID_ARGUMENT: InvalidArgument}
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+def lib():
+    """Loads libtmfqs_hip.so; raises OSError when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(HIP_LIB):
+            raise OSError(f"{HIP_LIB} is missing: run python -m tmfqsfullstate_amd.build")
+        L = ctypes.CDLL(HIP_LIB)
+        L.tmfqs_hip_last_error.restype = ctypes.c_char_p
+        vp = ctypes.c_void_p
+        L.tmfqs_hip_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(vp)]
+        L.tmfqs_hip_clone.argtypes = [vp, ctypes.POINTER(vp)]
+        L.tmfqs_hip_destroy.argtypes = [vp]
+        L.tmfqs_hip_init_zero.argtypes = [vp, ctypes.c_uint]
+        L.tmfqs_hip_init_basis.argtypes = [vp, ctypes.c_uint, ctypes.c_uint64, ctypes.c_double, ctypes.c_double]
+        L.tmfqs_hip_init_uniform.argtypes = [vp, ctypes.c_uint, _u64p, ctypes.c_size_t]
+        L.tmfqs_hip_load.argtypes = [vp, ctypes.c_uint, _dp, ctypes.c_uint64]
+        L.tmfqs_hip_export.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, _dp]
+        L.tmfqs_hip_amplitude.argtypes = [vp, ctypes.c_uint64, _dp, _dp]
+        L.tmfqs_hip_set_amplitude.argtypes = [vp, ctypes.c_uint64, ctypes.c_double, ctypes.c_double]
+        L.tmfqs_hip_probability.argtypes = [vp, ctypes.c_uint64, _dp]
+        L.tmfqs_hip_total_probability.argtypes = [vp, _dp]
+        L.tmfqs_hip_sample.argtypes = [vp, ctypes.c_double, _u64p]
+        L.tmfqs_hip_amplitude_sum.argtypes = [vp, _dp, _dp]
+        L.tmfqs_hip_total_probability_affine.argtypes = [vp] + [ctypes.c_double] * 4 + [_dp]
+        L.tmfqs_hip_sample_affine.argtypes = [vp, ctypes.c_double] + [ctypes.c_double] * 4 + [_u64p]
+        L.tmfqs_hip_phase_flip.argtypes = [vp, ctypes.c_uint64]
+        L.tmfqs_hip_inversion_about_mean.argtypes = [vp]
+        L.tmfqs_hip_inversion_about_mean_value.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+        L.tmfqs_hip_affine.argtypes = [vp] + [ctypes.c_double] * 4
+        L.tmfqs_hip_hadamard.argtypes = [vp, ctypes.c_uint]
+        L.tmfqs_hip_pauli_x.argtypes = [vp, ctypes.c_uint]
+        L.tmfqs_hip_controlled_phase.argtypes = [vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_double]
+        L.tmfqs_hip_controlled_not.argtypes = [vp, ctypes.c_uint, ctypes.c_uint]
+        L.tmfqs_hip_swap.argtypes = [vp, ctypes.c_uint, ctypes.c_uint]
+        L.tmfqs_hip_apply_gate.argtypes = [vp, _dp, ctypes.c_uint, ctypes.POINTER(ctypes.c_uint)]
+        L.tmfqs_hip_begin_batch.argtypes = [vp]
+        L.tmfqs_hip_end_batch.argtypes = [vp]
+        L.tmfqs_hip_synchronize.argtypes = [vp]
+        L.tmfqs_hip_profile_enable.argtypes = [vp, ctypes.c_int]
+        L.tmfqs_hip_profile_reset.argtypes = [vp]
+        L.tmfqs_hip_profile_read.argtypes = [vp, ctypes.c_void_p, _u64p, _dp, _dp, ctypes.POINTER(ctypes.c_size_t)]
+        L.tmfqs_hip_stream.argtypes = [vp, ctypes.POINTER(vp)]
+        L.tmfqs_hip_qubit_count.argtypes = [vp, ctypes.POINTER(ctypes.c_uint)]
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().tmfqs_hip_last_error().decode(errors="replace")
+        raise _EXC.get(rc, HipError)(rc, msg)
+
+
+def device_count():
+    c = ctypes.c_int(0)
+    check(lib().tmfqs_hip_device_count(ctypes.byref(c)))
+    return c.value
+
+
+def default_config(**kw):
+    cfg = Config()
+    lib().tmfqs_hip_config_default(ctypes.byref(cfg))
+    for k, v in kw.items():
+        setattr(cfg, k, int(v))
+    return cfg
+
+
+class HipState:
+    """One device state vector behind the C-ABI."""
+
+    def __init__(self, precision=C128, device=0, _handle=None, **cfg):
+        self._h = ctypes.c_void_p()
+        if _handle is not None:
+            self._h = _handle
+        else:
+            c = default_config(precision=precision, device=device, **cfg)
+            check(lib().tmfqs_hip_create(ctypes.byref(c), ctypes.byref(self._h)))
+        self.n = None
+
+    def close(self):
+        if self._h:
+            lib().tmfqs_hip_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def clone(self):
+        out = ctypes.c_void_p()
+        check(lib().tmfqs_hip_clone(self._h, ctypes.byref(out)))
+        c = HipState(_handle=out)
+        c.n = self.n
+        return c
+
+    # ---- init / IO
+    def init_zero(self, n):
+        check(lib().tmfqs_hip_init_zero(self._h, n))
+        self.n = n
+        return self
+
+    def init_basis(self, n, state=0, amp=1.0 + 0j):
+        check(lib().tmfqs_hip_init_basis(self._h, n, state, amp.real, amp.imag))
+        self.n = n
+        return self
+
+    def init_uniform(self, n, states):
+        a = np.ascontiguousarray(states, dtype=np.uint64)
+        check(lib().tmfqs_hip_init_uniform(self._h, n, a.ctypes.data_as(_u64p), len(a)))
+        self.n = n
+        return self
+
+    def load(self, n, interleaved):
+        a = np.ascontiguousarray(interleaved, dtype=np.float64)
+        check(lib().tmfqs_hip_load(self._h, n, a.ctypes.data_as(_dp), a.size))
+        self.n = n
+        return self
+
+    def export(self, first=0, count=None):
+        if count is None:
+            count = (1 << self.n) - first
+        out = np.empty(2 * count, dtype=np.float64)
+        check(lib().tmfqs_hip_export(self._h, first, count, out.ctypes.data_as(_dp)))
+        return out
+
+    def amplitudes(self):
+        e = self.export()
+        return e[0::2] + 1j * e[1::2]
+
+    def amplitude(self, s):
+        re, im = ctypes.c_double(), ctypes.c_double()
+        check(lib().tmfqs_hip_amplitude(self._h, s, ctypes.byref(re), ctypes.byref(im)))
+        return complex(re.value, im.value)
+
+    def set_amplitude(self, s, v):
+        check(lib().tmfqs_hip_set_amplitude(self._h, s, v.real, v.imag))
+
+    def probability(self, s):
+        o = ctypes.c_double()
+        check(lib().tmfqs_hip_probability(self._h, s, ctypes.byref(o)))
+        return o.value
+
+    def total_probability(self):
+        o = ctypes.c_double()
+        check(lib().tmfqs_hip_total_probability(self._h, ctypes.byref(o)))
+        return o.value
+
+    def sample(self, rnd):
+        o = ctypes.c_uint64()
+        check(lib().tmfqs_hip_sample(self._h, rnd, ctypes.byref(o)))
+        return o.value
+
+    def amplitude_sum(self):
+        re, im = ctypes.c_double(), ctypes.c_double()
+        check(lib().tmfqs_hip_amplitude_sum(self._h, ctypes.byref(re), ctypes.byref(im)))
+        return complex(re.value, im.value)
+
+    def total_probability_affine(self, scale, bias):
+        o = ctypes.c_double()
+        check(lib().tmfqs_hip_total_probability_affine(self._h, scale.real, scale.imag, bias.real, bias.imag,
+                                                       ctypes.byref(o)))
+        return o.value
+
+    def sample_affine(self, rnd, scale, bias):
+        o = ctypes.c_uint64()
+        check(lib().tmfqs_hip_sample_affine(self._h, rnd, scale.real, scale.imag, bias.real, bias.imag,
+                                            ctypes.byref(o)))
+        return o.value
+
+    # ---- operations
+    def h(self, q):
+        check(lib().tmfqs_hip_hadamard(self._h, q))
+
+    def x(self, q):
+        check(lib().tmfqs_hip_pauli_x(self._h, q))
+
+    def cx(self, c, t):
+        check(lib().tmfqs_hip_controlled_not(self._h, c, t))
+
+    def cphase(self, c, t, theta):
+        check(lib().tmfqs_hip_controlled_phase(self._h, c, t, theta))
+
+    def swap(self, a, b):
+        check(lib().tmfqs_hip_swap(self._h, a, b))
+
+    def gate(self, mat, qubits):
+        m = np.ascontiguousarray(mat, dtype=np.complex128)
+        inter = np.empty(2 * m.size)
+        inter[0::2] = m.real.ravel()
+        inter[1::2] = m.imag.ravel()
+        q = (ctypes.c_uint * max(1, len(qubits)))(*qubits)
+        check(lib().tmfqs_hip_apply_gate(self._h, inter.ctypes.data_as(_dp), len(qubits), q))
+
+    def phase_flip(self, s):
+        check(lib().tmfqs_hip_phase_flip(self._h, s))
+
+    def inversion_about_mean(self, mean=None):
+        if mean is None:
+            check(lib().tmfqs_hip_inversion_about_mean(self._h))
+        else:
+            check(lib().tmfqs_hip_inversion_about_mean_value(self._h, mean.real, mean.imag))
+
+    def affine(self, scale, bias):
+        check(lib().tmfqs_hip_affine(self._h, scale.real, scale.imag, bias.real, bias.imag))
+
+    def begin_batch(self):
+        check(lib().tmfqs_hip_begin_batch(self._h))
+
+    def end_batch(self):
+        check(lib().tmfqs_hip_end_batch(self._h))
+
+    def synchronize(self):
+        check(lib().tmfqs_hip_synchronize(self._h))
+
+    # ---- observability
+    def profile(self, on=True):
+        check(lib().tmfqs_hip_profile_enable(self._h, int(on)))
+
+    def profile_reset(self):
+        check(lib().tmfqs_hip_profile_reset(self._h))
+
+    def profile_read(self):
+        cap = 64
+        names = (ctypes.c_char * 64 * cap)()
+        launches = (ctypes.c_uint64 * cap)()
+        ms = (ctypes.c_double * cap)()
+        byts = (ctypes.c_double * cap)()
+        cnt = ctypes.c_size_t(cap)
+        check(lib().tmfqs_hip_profile_read(self._h, ctypes.cast(names, ctypes.c_void_p), launches, ms, byts,
+                                           ctypes.byref(cnt)))
+        out = {}
+        for i in range(cnt.value):
+            nm = bytes(names[i]).split(b"\0", 1)[0].decode()
+            out[nm] = {"launches": launches[i], "ms": ms[i], "bytes": byts[i]}
+        return out
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        check(lib().tmfqs_hip_stream(self._h, ctypes.byref(s)))
+        return s.value
+
+    # ---- convenience: run a list of (kind, q0, q1, theta, state) ops with the
+    # backend semantics (swap as one permutation; diffusion needs the mean)
+    def run_gates(self, seq):
+        for t in seq:
+            k = t[0]
+            if k == OP_H:
+                self.h(t[1])
+            elif k == OP_X:
+                self.x(t[1])
+            elif k == OP_CX:
+                self.cx(t[1], t[2])
+            elif k == OP_CPHASE:
+                self.cphase(t[1], t[2], t[3])
+            elif k == OP_SWAP:
+                self.swap(t[1], t[2])
+            elif k == OP_PHASE_FLIP:
+                self.phase_flip(t[4])
+            else:
+                raise ValueError(f"op kind {k} needs register semantics")

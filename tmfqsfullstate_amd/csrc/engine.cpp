@@ -1,0 +1,1530 @@
+// Host side of the gfx950 dense state-vector engine: handle, operation queue,
+// pass planner, exact sequential-chain sampling, and the extern "C" ABI
+// declared in include/tmfqs_hip.h.
+//
+// Compiled with -ffp-contract=off: the host replays elementwise steps and
+// sequential probability chains with exactly the reference's operation order.
+#include "tmfqs_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+#include "engine.h"
+
+using namespace tmfqs_hip;
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;  // include/tmfqs/core/constants.h:6 in the reference
+
+thread_local std::string g_last_error;
+
+struct DeviceError : std::runtime_error {
+	using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char *what) {
+	if(e != hipSuccess) {
+		if(e == hipErrorOutOfMemory) throw std::bad_alloc();
+		throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+	}
+}
+
+#define HIPC(expr) hip_check((expr), #expr)
+
+int map_exception() {
+	try {
+		throw;
+	} catch(const std::out_of_range &e) {
+		g_last_error = e.what();
+		return TMFQS_HIP_ERR_OUT_OF_RANGE;
+	} catch(const std::invalid_argument &e) {
+		g_last_error = e.what();
+		return TMFQS_HIP_ERR_INVALID_ARGUMENT;
+	} catch(const std::logic_error &e) {
+		g_last_error = e.what();
+		return TMFQS_HIP_ERR_LOGIC;
+	} catch(const DeviceError &e) {
+		g_last_error = e.what();
+		return TMFQS_HIP_ERR_DEVICE;
+	} catch(const std::bad_alloc &) {
+		g_last_error = "out of memory";
+		return TMFQS_HIP_ERR_OUT_OF_MEMORY;
+	} catch(const std::runtime_error &e) {
+		g_last_error = e.what();
+		return TMFQS_HIP_ERR_RUNTIME;
+	} catch(...) {
+		g_last_error = "unknown error";
+		return TMFQS_HIP_ERR_RUNTIME;
+	}
+}
+
+// Queued operation; bit arguments are physical bit positions (p = n-1-q).
+enum QKind { Q_H, Q_X, Q_CX, Q_CP, Q_SWAP, Q_U1, Q_UK, Q_FLIP, Q_SET, Q_DIFF, Q_AFFINE, Q_SCATTER };
+
+struct QOp {
+	int kind;
+	unsigned b0 = 0, b1 = 0;
+	unsigned k = 0;
+	unsigned bits[5] = {0, 0, 0, 0, 0};
+	size_t mat = 0;  // offset into qmats
+	double cr = 0, ci = 0, theta = 0;
+	u64 state = 0;
+	double ar = 0, ai = 0, br = 0, bi = 0;
+	size_t sc0 = 0, scn = 0;  // Q_SCATTER: range in qscatter
+};
+
+struct Cplx {
+	double re, im;
+};
+
+// Upload arena: pinned host and device buffers used as a bump allocator; a
+// wrap-around synchronises the stream before reusing the space.
+struct Arena {
+	char *host = nullptr;
+	char *dev = nullptr;
+	size_t cap = 0, off = 0;
+};
+
+struct ProfAgg {
+	u64 launches = 0;
+	double ms = 0;
+	double bytes = 0;
+};
+
+struct ProfRec {
+	std::string name;
+	double bytes;
+	hipEvent_t start, stop;
+};
+
+}  // namespace
+
+struct tmfqs_hip_state {
+	tmfqs_hip_config cfg{};
+	int prec = 0;
+	size_t esz = 16;
+	unsigned n = 0;
+	u64 N = 0;
+	bool init = false;
+	void *amp = nullptr;
+	size_t amp_bytes = 0;
+	void *scratch = nullptr;  // lazily allocated second state buffer
+	size_t scratch_bytes = 0;
+	hipStream_t stream = nullptr;
+	double *d_red = nullptr;    // reduction partials + outputs
+	double *d_phase = nullptr;  // 5 x 256 complex phase tables
+	double *d_sums = nullptr;   // sampling chunk sums
+	long long *d_ticks = nullptr;
+	int32_t *d_ulog = nullptr, *d_ties = nullptr;
+	size_t chunk_cap = 0;
+	Arena arena;
+	char *h_stage = nullptr;  // pinned staging for bulk copies
+	size_t stage_bytes = 0;
+	std::vector<QOp> queue;
+	std::vector<double> qmats;
+	std::vector<Scatter> qscatter;
+	size_t ew_steps_pending = 0;
+	std::unordered_map<u64, Cplx> tracked;
+	std::vector<u64> dirty;  // states whose tracked value must be scattered at the end of the run
+	std::unordered_set<u64> dirty_set;
+	bool profiling = false;
+	std::vector<ProfRec> recs;
+	std::vector<hipEvent_t> event_pool;
+	std::map<std::string, ProfAgg> prof;
+	Launch L() const { return Launch{stream, prec}; }
+};
+
+namespace {
+
+// ------------------------------------------------------------ utilities
+void require_init(const tmfqs_hip_state *h, const char *op) {
+	if(!h->init) throw std::logic_error(std::string("HipStateBackend is not initialized for ") + op);
+}
+
+void validate_state(const tmfqs_hip_state *h, u64 s, const char *scope) {
+	if(s >= h->N) throw std::out_of_range(std::string(scope) + " state index out of range");
+}
+
+void validate_qubit(const tmfqs_hip_state *h, unsigned q, const char *scope) {
+	if(q >= h->n) throw std::out_of_range(std::string(scope) + " qubit index out of range");
+}
+
+void validate_two(const tmfqs_hip_state *h, unsigned a, unsigned b, const char *scope) {
+	validate_qubit(h, a, scope);
+	validate_qubit(h, b, scope);
+	if(a == b) throw std::invalid_argument(std::string(scope) + " requires distinct qubits");
+}
+
+unsigned pbit(const tmfqs_hip_state *h, unsigned q) { return h->n - 1u - q; }
+
+hipEvent_t take_event(tmfqs_hip_state *h) {
+	if(!h->event_pool.empty()) {
+		hipEvent_t e = h->event_pool.back();
+		h->event_pool.pop_back();
+		return e;
+	}
+	hipEvent_t e;
+	HIPC(hipEventCreate(&e));
+	return e;
+}
+
+// Brackets one kernel launch with events when profiling is on.
+struct ProfScope {
+	tmfqs_hip_state *h;
+	ProfRec rec;
+	bool on;
+	ProfScope(tmfqs_hip_state *hh, const char *name, double bytes) : h(hh), on(hh->profiling) {
+		if(on) {
+			rec.name = name;
+			rec.bytes = bytes;
+			rec.start = take_event(h);
+			rec.stop = take_event(h);
+			HIPC(hipEventRecord(rec.start, h->stream));
+		}
+	}
+	~ProfScope() {
+		if(on) {
+			if(hipEventRecord(rec.stop, h->stream) == hipSuccess) h->recs.push_back(rec);
+		}
+	}
+};
+
+void collect_profile(tmfqs_hip_state *h) {
+	if(h->recs.empty()) return;
+	HIPC(hipStreamSynchronize(h->stream));
+	for(auto &r : h->recs) {
+		float ms = 0;
+		HIPC(hipEventElapsedTime(&ms, r.start, r.stop));
+		auto &a = h->prof[r.name];
+		a.launches += 1;
+		a.ms += ms;
+		a.bytes += r.bytes;
+		h->event_pool.push_back(r.start);
+		h->event_pool.push_back(r.stop);
+	}
+	h->recs.clear();
+}
+
+void arena_ensure(tmfqs_hip_state *h, size_t need) {
+	Arena &A = h->arena;
+	if(need > A.cap) {
+		HIPC(hipStreamSynchronize(h->stream));
+		if(A.host) hipHostFree(A.host);
+		if(A.dev) hipFree(A.dev);
+		A.host = nullptr;
+		A.dev = nullptr;
+		size_t cap = std::max<size_t>(need * 2, 8u << 20);
+		HIPC(hipHostMalloc((void **)&A.host, cap, hipHostMallocDefault));
+		HIPC(hipMalloc((void **)&A.dev, cap));
+		A.cap = cap;
+		A.off = 0;
+	}
+}
+
+// Copies bytes into the upload arena and enqueues the H2D copy; returns the
+// device pointer valid for kernels enqueued after this call.
+void *arena_upload(tmfqs_hip_state *h, const void *src, size_t bytes) {
+	if(bytes == 0) return nullptr;
+	const size_t need = (bytes + 255) & ~size_t(255);
+	arena_ensure(h, need);
+	Arena &A = h->arena;
+	if(A.off + need > A.cap) {
+		HIPC(hipStreamSynchronize(h->stream));
+		A.off = 0;
+	}
+	std::memcpy(A.host + A.off, src, bytes);
+	void *d = A.dev + A.off;
+	HIPC(hipMemcpyAsync(d, A.host + A.off, bytes, hipMemcpyHostToDevice, h->stream));
+	A.off += need;
+	return d;
+}
+
+void ensure_stage(tmfqs_hip_state *h, size_t bytes) {
+	if(h->stage_bytes >= bytes) return;
+	if(h->h_stage) {
+		HIPC(hipStreamSynchronize(h->stream));
+		hipHostFree(h->h_stage);
+		h->h_stage = nullptr;
+	}
+	HIPC(hipHostMalloc((void **)&h->h_stage, bytes, hipHostMallocDefault));
+	h->stage_bytes = bytes;
+}
+
+void ensure_scratch(tmfqs_hip_state *h) {
+	if(h->scratch_bytes >= h->amp_bytes) return;
+	if(h->scratch) hipFree(h->scratch);
+	h->scratch = nullptr;
+	h->scratch_bytes = 0;
+	HIPC(hipMalloc(&h->scratch, h->amp_bytes));
+	h->scratch_bytes = h->amp_bytes;
+}
+
+void ensure_chunk_buffers(tmfqs_hip_state *h, size_t nch) {
+	if(h->chunk_cap >= nch) return;
+	hipFree(h->d_sums);
+	hipFree(h->d_ticks);
+	hipFree(h->d_ulog);
+	hipFree(h->d_ties);
+	HIPC(hipMalloc((void **)&h->d_sums, nch * sizeof(double)));
+	HIPC(hipMalloc((void **)&h->d_ticks, nch * sizeof(long long)));
+	HIPC(hipMalloc((void **)&h->d_ulog, nch * sizeof(int32_t)));
+	HIPC(hipMalloc((void **)&h->d_ties, nch * sizeof(int32_t)));
+	h->chunk_cap = nch;
+}
+
+void build_phase_tables(tmfqs_hip_state *h) {
+	std::vector<double> t(5 * 256 * 2);
+	for(int j = 0; j < 5; ++j) {
+		const double denom = std::ldexp(1.0, 8 * (j + 1));
+		for(int v = 0; v < 256; ++v) {
+			const double ang = 2.0 * kPi * (double)v / denom;
+			t[2 * (j * 256 + v)] = std::cos(ang);
+			t[2 * (j * 256 + v) + 1] = std::sin(ang);
+		}
+	}
+	HIPC(hipMalloc((void **)&h->d_phase, t.size() * sizeof(double)));
+	HIPC(hipMemcpy(h->d_phase, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+}
+
+double pass_bytes(const tmfqs_hip_state *h) { return 2.0 * (double)h->N * (double)h->esz; }
+
+// ------------------------------------------------------------ allocation
+void allocate_state(tmfqs_hip_state *h, unsigned n) {
+	if(n > 63u) throw std::invalid_argument("Number of qubits exceeds supported range for state indexing");
+	const u64 N = 1ull << n;
+	const size_t bytes = (size_t)N * h->esz;
+	h->queue.clear();
+	h->qmats.clear();
+	h->qscatter.clear();
+	h->ew_steps_pending = 0;
+	h->tracked.clear();
+	h->dirty.clear();
+	h->dirty_set.clear();
+	if(h->amp_bytes != bytes) {
+		HIPC(hipStreamSynchronize(h->stream));
+		if(h->amp) hipFree(h->amp);
+		h->amp = nullptr;
+		h->amp_bytes = 0;
+		h->init = false;
+		HIPC(hipMalloc(&h->amp, bytes));
+		h->amp_bytes = bytes;
+		if(h->scratch) {
+			hipFree(h->scratch);
+			h->scratch = nullptr;
+			h->scratch_bytes = 0;
+		}
+	}
+	h->n = n;
+	h->N = N;
+	h->init = true;
+}
+
+// ------------------------------------------------------------ execution
+void exec_op_unfused(tmfqs_hip_state *h, const QOp &op);
+void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &ops, size_t lo, size_t hi);
+void exec_elementwise(tmfqs_hip_state *h, const std::vector<QOp> &ops, size_t lo, size_t hi);
+
+bool is_elementwise(int kind) {
+	return kind == Q_FLIP || kind == Q_SET || kind == Q_DIFF || kind == Q_AFFINE || kind == Q_SCATTER;
+}
+
+// Closes the current elementwise run: the exact values of every state flipped
+// or set in the run become a scatter applied after the run's sweeps.
+void close_ew_run(tmfqs_hip_state *h) {
+	if(h->dirty.empty()) return;
+	QOp sc;
+	sc.kind = Q_SCATTER;
+	sc.sc0 = h->qscatter.size();
+	for(u64 s : h->dirty) {
+		const Cplx v = h->tracked.at(s);
+		h->qscatter.push_back(Scatter{s, v.re, v.im});
+	}
+	sc.scn = h->dirty.size();
+	h->queue.push_back(sc);
+	h->dirty.clear();
+	h->dirty_set.clear();
+}
+
+void flush(tmfqs_hip_state *h) {
+	if(h->queue.empty()) return;
+	close_ew_run(h);
+	std::vector<QOp> ops;
+	ops.swap(h->queue);
+	size_t i = 0;
+	while(i < ops.size()) {
+		size_t j = i;
+		const bool ew = is_elementwise(ops[i].kind);
+		while(j < ops.size() && is_elementwise(ops[j].kind) == ew) ++j;
+		if(ew)
+			exec_elementwise(h, ops, i, j);
+		else
+			exec_gates(h, ops, i, j);
+		i = j;
+	}
+	h->qmats.clear();
+	h->qscatter.clear();
+	h->ew_steps_pending = 0;
+	if(h->prec == 1) {
+		// complex64 storage rounds every tracked value once it is flushed.
+		for(auto &kv : h->tracked) {
+			kv.second.re = (double)(float)kv.second.re;
+			kv.second.im = (double)(float)kv.second.im;
+		}
+	}
+}
+
+void sync_all(tmfqs_hip_state *h) {
+	flush(h);
+	HIPC(hipStreamSynchronize(h->stream));
+}
+
+void exec_elementwise(tmfqs_hip_state *h, const std::vector<QOp> &ops, size_t lo, size_t hi) {
+	std::vector<EwStep> steps;
+	const unsigned block = h->cfg.grover_block > 0 ? (unsigned)h->cfg.grover_block : (h->prec == 0 ? 48u : 32u);
+	auto run_steps = [&]() {
+		size_t s0 = 0;
+		while(s0 < steps.size()) {
+			const size_t cnt = std::min<size_t>(block, steps.size() - s0);
+			const EwStep *d = (const EwStep *)arena_upload(h, steps.data() + s0, cnt * sizeof(EwStep));
+			ProfScope ps(h, "k_ew_chain", pass_bytes(h));
+			HIPC(launch_ew_chain(h->L(), h->amp, h->n, d, (unsigned)cnt));
+			s0 += cnt;
+		}
+		steps.clear();
+	};
+	for(size_t i = lo; i < hi; ++i) {
+		const QOp &op = ops[i];
+		if(op.kind == Q_DIFF) {
+			EwStep s{EW_DIFFUSE, 0, op.ar, op.ai, 0, 0};
+			steps.push_back(s);
+		} else if(op.kind == Q_AFFINE) {
+			EwStep s{EW_AFFINE, 0, op.ar, op.ai, op.br, op.bi};
+			steps.push_back(s);
+		} else if(op.kind == Q_SCATTER) {
+			run_steps();
+			const Scatter *d = (const Scatter *)arena_upload(h, h->qscatter.data() + op.sc0, op.scn * sizeof(Scatter));
+			HIPC(launch_scatter(h->L(), h->amp, d, (unsigned)op.scn));
+		}
+		// Q_FLIP / Q_SET carry no sweep: their exact values arrive via Q_SCATTER.
+	}
+	run_steps();
+}
+
+// Host replay of one elementwise step on a tracked value (same arithmetic as
+// k_ew_chain, i.e. the reference's dense_state_backend.cpp:230-252).
+void replay(Cplx &v, const QOp &op) {
+	if(op.kind == Q_DIFF) {
+		v.re = 2.0 * op.ar - v.re;
+		v.im = 2.0 * op.ai - v.im;
+	} else if(op.kind == Q_AFFINE) {
+		const double nr = op.ar * v.re - op.ai * v.im + op.br;
+		const double ni = op.ar * v.im + op.ai * v.re + op.bi;
+		v.re = nr;
+		v.im = ni;
+	}
+}
+
+Cplx read_device(tmfqs_hip_state *h, u64 s) {
+	sync_all(h);
+	Cplx v;
+	if(h->prec == 0) {
+		double buf[2];
+		HIPC(hipMemcpy(buf, (char *)h->amp + s * 16, 16, hipMemcpyDeviceToHost));
+		v.re = buf[0];
+		v.im = buf[1];
+	} else {
+		float buf[2];
+		HIPC(hipMemcpy(buf, (char *)h->amp + s * 8, 8, hipMemcpyDeviceToHost));
+		v.re = buf[0];
+		v.im = buf[1];
+	}
+	return v;
+}
+
+Cplx current_value(tmfqs_hip_state *h, u64 s) {
+	auto it = h->tracked.find(s);
+	if(it != h->tracked.end()) return it->second;
+	const Cplx v = read_device(h, s);
+	h->tracked[s] = v;
+	return v;
+}
+
+void enqueue_gate(tmfqs_hip_state *h, const QOp &op) {
+	if(!h->cfg.fusion) {
+		flush(h);
+		h->tracked.clear();
+		exec_op_unfused(h, op);
+		h->qmats.clear();
+		return;
+	}
+	close_ew_run(h);
+	h->tracked.clear();
+	h->queue.push_back(op);
+	if(h->queue.size() >= 8192) flush(h);
+}
+
+void enqueue_ew(tmfqs_hip_state *h, const QOp &op) {
+	for(auto &kv : h->tracked) replay(kv.second, op);
+	h->queue.push_back(op);
+	if(op.kind == Q_DIFF || op.kind == Q_AFFINE) {
+		++h->ew_steps_pending;
+		const size_t block = h->cfg.grover_block > 0 ? (size_t)h->cfg.grover_block : (h->prec == 0 ? 48u : 32u);
+		if(!h->cfg.fusion || h->ew_steps_pending >= block) flush(h);
+	}
+}
+
+void mark_dirty(tmfqs_hip_state *h, u64 s) {
+	if(h->dirty_set.insert(s).second) h->dirty.push_back(s);
+}
+
+// ----------------------------------------------------- unfused execution
+void exec_op_unfused(tmfqs_hip_state *h, const QOp &op) {
+	const Launch L = h->L();
+	const double pb = pass_bytes(h);
+	switch(op.kind) {
+		case Q_H: {
+			ProfScope ps(h, "k_hadamard", pb);
+			HIPC(launch_hadamard(L, h->amp, h->n, op.b0));
+			break;
+		}
+		case Q_X: {
+			ProfScope ps(h, "k_pauli_x", pb);
+			HIPC(launch_pauli_x(L, h->amp, h->n, op.b0, 0));
+			break;
+		}
+		case Q_CX: {
+			ProfScope ps(h, "k_pauli_x", pb / 2);
+			HIPC(launch_pauli_x(L, h->amp, h->n, op.b1, 1ull << op.b0));
+			break;
+		}
+		case Q_CP: {
+			ProfScope ps(h, "k_cphase", pb / 4);
+			HIPC(launch_cphase(L, h->amp, h->n, op.b1, 1ull << op.b0, op.cr, op.ci));
+			break;
+		}
+		case Q_SWAP: {
+			ProfScope ps(h, "k_bitswap", pb / 2);
+			HIPC(launch_bitswap(L, h->amp, h->n, op.b0, op.b1));
+			break;
+		}
+		case Q_U1: {
+			ProfScope ps(h, "k_u1", pb);
+			HIPC(launch_u1(L, h->amp, h->n, op.b0, h->qmats.data() + op.mat));
+			break;
+		}
+		case Q_UK: {
+			const size_t dim = 1ull << op.k;
+			const double *d = (const double *)arena_upload(h, h->qmats.data() + op.mat, 2 * dim * dim * sizeof(double));
+			ProfScope ps(h, "k_uk", pb);
+			HIPC(launch_uk(L, h->amp, h->n, op.k, op.bits, d));
+			break;
+		}
+		default: break;
+	}
+}
+
+// ------------------------------------------------------------ planner
+struct PassPlan {
+	std::vector<size_t> ops;  // indices into the mapped op list
+	u64 active = 0;           // bits needing locality
+};
+
+unsigned tile_bits(const tmfqs_hip_state *h) {
+	unsigned T = h->cfg.tile_qubits > 0 ? (unsigned)h->cfg.tile_qubits : (h->prec == 0 ? 12u : 13u);
+	if(T > h->n) T = h->n;
+	return T;
+}
+
+unsigned coalesce_bits(const tmfqs_hip_state *h, unsigned T) {
+	unsigned C = h->cfg.coalesce_qubits > 0 ? (unsigned)h->cfg.coalesce_qubits : (h->prec == 0 ? 4u : 5u);
+	if(C > T) C = T;
+	if(T > (unsigned)kMaxHighBits && C < T - (unsigned)kMaxHighBits) C = T - (unsigned)kMaxHighBits;
+	return C;
+}
+
+u64 nondiag_bits(const QOp &op) {
+	switch(op.kind) {
+		case Q_H:
+		case Q_X:
+		case Q_U1: return 1ull << op.b0;
+		case Q_CX: return 1ull << op.b1;
+		case Q_UK: {
+			u64 m = 0;
+			for(unsigned i = 0; i < op.k; ++i) m |= 1ull << op.bits[i];
+			return m;
+		}
+		default: return 0;
+	}
+}
+
+bool is_qft_ladder_cp(const QOp &op, unsigned &hi, unsigned &lo) {
+	if(op.kind != Q_CP) return false;
+	hi = std::max(op.b0, op.b1);
+	lo = std::min(op.b0, op.b1);
+	if(hi > 39) return false;
+	const double want = kPi / (double)(1ull << (hi - lo));
+	return op.theta == want;
+}
+
+void run_permutation(tmfqs_hip_state *h, std::vector<unsigned> map) {
+	// Data of nominal bit b sits at physical map[b]; move it back to b.
+	const unsigned n = h->n;
+	std::vector<unsigned> pi(n);
+	bool ident = true;
+	for(unsigned b = 0; b < n; ++b) {
+		pi[map[b]] = b;
+		if(map[b] != b) ident = false;
+	}
+	if(ident) return;
+	bool invol = true;
+	for(unsigned b = 0; b < n; ++b)
+		if(pi[pi[b]] != b) invol = false;
+	const unsigned C = h->prec == 0 ? 4u : 5u;
+	if(invol && n >= 2 * C + 1) {
+		ProfScope ps(h, "k_involution", pass_bytes(h));
+		HIPC(launch_involution(h->L(), h->amp, n, pi.data(), C));
+		return;
+	}
+	// General permutation: transpositions, one in-place pass each.
+	std::vector<unsigned> cur = map;  // cur[b] = physical position of nominal b
+	std::vector<unsigned> who(n);     // who[p] = nominal bit at physical p
+	for(unsigned b = 0; b < n; ++b) who[cur[b]] = b;
+	for(unsigned b = 0; b < n; ++b) {
+		if(cur[b] == b) continue;
+		const unsigned p = cur[b];
+		const unsigned other = who[b];
+		ProfScope ps(h, "k_bitswap", pass_bytes(h) / 2);
+		HIPC(launch_bitswap(h->L(), h->amp, n, p, b));
+		who[p] = other;
+		cur[other] = p;
+		who[b] = b;
+		cur[b] = b;
+	}
+}
+
+void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_t hi) {
+	const unsigned n = h->n;
+	// 1. relabel swaps into a bit map; op bits become physical positions.
+	std::vector<unsigned> map(n);
+	for(unsigned b = 0; b < n; ++b) map[b] = b;
+	std::vector<QOp> ops;
+	ops.reserve(hi - lo);
+	for(size_t i = lo; i < hi; ++i) {
+		QOp op = in[i];
+		if(op.kind == Q_SWAP) {
+			std::swap(map[op.b0], map[op.b1]);
+			continue;
+		}
+		op.b0 = map[op.b0];
+		op.b1 = map[op.b1];
+		for(unsigned j = 0; j < op.k; ++j) op.bits[j] = map[op.bits[j]];
+		ops.push_back(op);
+	}
+	// 2. greedy grouping into passes.
+	const unsigned T = tile_bits(h);
+	const unsigned C = coalesce_bits(h, T);
+	const u64 lowC = (C >= 64) ? ~0ull : ((1ull << C) - 1ull);
+	std::vector<PassPlan> passes;
+	PassPlan cur;
+	for(size_t i = 0; i < ops.size(); ++i) {
+		const u64 need = nondiag_bits(ops[i]);
+		const u64 na = cur.active | need;
+		if(!cur.ops.empty() && (unsigned)__builtin_popcountll(na | lowC) > T) {
+			passes.push_back(cur);
+			cur = PassPlan();
+		}
+		cur.active |= need;
+		cur.ops.push_back(i);
+	}
+	if(!cur.ops.empty()) passes.push_back(cur);
+
+	// 3. emit each pass as a tile kernel.
+	for(const PassPlan &P : passes) {
+		unsigned Cp = T;
+		std::vector<unsigned> high;
+		for(;;) {
+			high.clear();
+			for(unsigned b = Cp; b < n; ++b)
+				if((P.active >> b) & 1ull) high.push_back(b);
+			if(Cp + high.size() <= T) break;
+			--Cp;
+		}
+		const unsigned Tp = Cp + (unsigned)high.size();
+		auto local = [&](unsigned b) -> int {
+			if(b < Cp) return (int)b;
+			for(size_t i = 0; i < high.size(); ++i)
+				if(high[i] == b) return (int)(Cp + i);
+			return -1;
+		};
+		std::vector<TileOp> tops;
+		std::vector<TwiddleTerm> terms;
+		std::vector<double> mats;
+		// bits touched non-diagonally by ops at or after position k in this pass
+		std::vector<u64> later(P.ops.size() + 1, 0);
+		for(size_t k = P.ops.size(); k-- > 0;) later[k] = later[k + 1] | nondiag_bits(ops[P.ops[k]]);
+		auto add_cond = [&](TileOp &t, unsigned b) {
+			const int j = local(b);
+			if(j >= 0)
+				t.lcond |= 1ull << j;
+			else
+				t.gcond |= 1ull << b;
+		};
+		size_t k = 0;
+		while(k < P.ops.size()) {
+			const QOp &op = ops[P.ops[k]];
+			TileOp t{};
+			switch(op.kind) {
+				case Q_H:
+					t.kind = TOP_H;
+					t.j0 = (unsigned)local(op.b0);
+					tops.push_back(t);
+					++k;
+					break;
+				case Q_X:
+					t.kind = TOP_X;
+					t.j0 = (unsigned)local(op.b0);
+					tops.push_back(t);
+					++k;
+					break;
+				case Q_CX:
+					t.kind = TOP_X;
+					t.j0 = (unsigned)local(op.b1);
+					add_cond(t, op.b0);
+					tops.push_back(t);
+					++k;
+					break;
+				case Q_U1:
+					t.kind = TOP_U1;
+					t.j0 = (unsigned)local(op.b0);
+					t.mat = (unsigned)mats.size();
+					mats.insert(mats.end(), h->qmats.begin() + op.mat, h->qmats.begin() + op.mat + 8);
+					tops.push_back(t);
+					++k;
+					break;
+				case Q_UK: {
+					t.kind = TOP_UK;
+					t.k = op.k;
+					for(unsigned j = 0; j < op.k; ++j) t.jl[j] = (unsigned)local(op.bits[j]);
+					t.mat = (unsigned)mats.size();
+					const size_t dim = 1ull << op.k;
+					mats.insert(mats.end(), h->qmats.begin() + op.mat, h->qmats.begin() + op.mat + 2 * dim * dim);
+					tops.push_back(t);
+					++k;
+					break;
+				}
+				case Q_CP: {
+					unsigned hb, lb;
+					if(!h->cfg.exact_phases && is_qft_ladder_cp(op, hb, lb)) {
+						// maximal run of ladder CPs sharing the high bit
+						u64 amask = 0;
+						size_t e = k;
+						while(e < P.ops.size()) {
+							unsigned h2, l2;
+							if(!is_qft_ladder_cp(ops[P.ops[e]], h2, l2) || h2 != hb) break;
+							amask |= 1ull << l2;
+							++e;
+						}
+						const u64 after = later[e];
+						const bool hb_free = !((after >> hb) & 1ull);
+						const u64 defer = hb_free ? (amask & ~after) : 0;
+						const u64 keep = amask & ~defer;
+						if(keep) {
+							t.kind = TOP_LADDER;
+							add_cond(t, hb);
+							t.amask = keep;
+							t.shift = 63u - hb;
+							tops.push_back(t);
+						}
+						if(defer) terms.push_back(TwiddleTerm{1ull << hb, defer, 63u - hb, 0});
+						k = e;
+					} else {
+						t.kind = TOP_CP;
+						add_cond(t, op.b0);
+						add_cond(t, op.b1);
+						t.cr = op.cr;
+						t.ci = op.ci;
+						tops.push_back(t);
+						++k;
+					}
+					break;
+				}
+				default: ++k; break;
+			}
+		}
+		// deferred ladder factors: one twiddle per kMaxTwiddleTerms terms, at the end
+		std::vector<TwiddleTerm> allterms;
+		for(size_t s = 0; s < terms.size(); s += kMaxTwiddleTerms) {
+			TileOp t{};
+			t.kind = TOP_TWIDDLE;
+			t.twid = (unsigned)allterms.size();
+			t.nterms = (unsigned)std::min<size_t>(kMaxTwiddleTerms, terms.size() - s);
+			for(unsigned q = 0; q < t.nterms; ++q) allterms.push_back(terms[s + q]);
+			tops.push_back(t);
+		}
+		if(tops.empty()) continue;
+		TilePass tp{};
+		tp.n = n;
+		tp.T = Tp;
+		tp.C = Cp;
+		tp.nh = (unsigned)high.size();
+		for(size_t i = 0; i < high.size(); ++i) tp.hbits[i] = high[i];
+		tp.ntiles = 1ull << (n - Tp);
+		// op lists longer than kMaxTileOps are split across launches.
+		for(size_t s = 0; s < tops.size(); s += kMaxTileOps) {
+			const size_t cnt = std::min<size_t>(kMaxTileOps, tops.size() - s);
+			const TileOp *dops = (const TileOp *)arena_upload(h, tops.data() + s, cnt * sizeof(TileOp));
+			const double *dm = mats.empty() ? nullptr : (const double *)arena_upload(h, mats.data(), mats.size() * sizeof(double));
+			const TwiddleTerm *dt =
+			    allterms.empty() ? nullptr : (const TwiddleTerm *)arena_upload(h, allterms.data(), allterms.size() * sizeof(TwiddleTerm));
+			tp.nops = (unsigned)cnt;
+			tp.op0 = 0;
+			ProfScope ps(h, "k_tile", pass_bytes(h));
+			HIPC(launch_tile_pass(h->L(), h->amp, tp, dops, dm, dt, h->d_phase));
+		}
+	}
+	// 4. restore the identity bit layout.
+	run_permutation(h, map);
+}
+
+// ------------------------------------------------------------ sampling
+// Exact replay of the reference's sequential FP64 chain S = fl(S + p_i)
+// (dense_state_backend.cpp:173-187, quantum_register.cpp:289-309).  Chunk
+// sums locate each chunk's binade; inside one binade every step adds
+// rint(p_i / ulp) ulps exactly, so whole chunks advance by an integer count
+// computed on the device.  Chunks that cross a binade, contain a tie, or start
+// at S == 0 are replayed element by element on the host.
+struct ChainResult {
+	bool found = false;
+	u64 index = 0;
+	double total = 0;
+};
+
+inline double host_prob(double re, double im, const double *af) {
+	if(af) {
+		const double mr = af[0] * re - af[1] * im;
+		const double mi = af[0] * im + af[1] * re;
+		re = mr + af[2];
+		im = mi + af[3];
+	}
+	return re * re + im * im;
+}
+
+void two_sum(double a, double b, double &s, double &e) {
+	s = a + b;
+	const double bb = s - a;
+	e = (a - (s - bb)) + (b - bb);
+}
+
+ChainResult chain_walk(tmfqs_hip_state *h, bool find, double rnd, const double *af) {
+	sync_all(h);
+	const unsigned cl = h->n < kSampleChunkLog ? h->n : kSampleChunkLog;
+	const u64 chunk = 1ull << cl;
+	const u64 nch = 1ull << (h->n - cl);
+	ensure_chunk_buffers(h, nch);
+	double *daf = nullptr;
+	if(af) {
+		daf = h->d_red + 2 * kReduceBlocks + 8;
+		HIPC(hipMemcpyAsync(daf, af, 4 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+	}
+	std::vector<double> sums(nch);
+	{
+		ProfScope ps(h, "k_chunk_prob", (double)h->N * h->esz);
+		HIPC(launch_chunk_prob(h->L(), h->amp, h->n, daf, h->d_sums));
+	}
+	HIPC(hipMemcpyAsync(sums.data(), h->d_sums, nch * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+	HIPC(hipStreamSynchronize(h->stream));
+	// approximate exact prefix (double-double) -> predicted binade per chunk
+	std::vector<int32_t> ulog(nch, INT32_MIN);
+	double eh = 0, el = 0;
+	for(u64 c = 0; c < nch; ++c) {
+		const double start = eh + el;
+		double s, e;
+		two_sum(eh, sums[c], s, e);
+		el += e;
+		two_sum(s, el, eh, el);
+		const double end = eh + el;
+		if(start > 0x1p-1000 && sums[c] > 0) {
+			const int e0 = std::ilogb(start * (1.0 - 1e-6));
+			const int e1 = std::ilogb(end * (1.0 + 1e-6));
+			if(e0 == e1) ulog[c] = e0 - 52;
+		}
+	}
+	std::vector<long long> ticks(nch);
+	std::vector<int32_t> ties(nch);
+	HIPC(hipMemcpyAsync(h->d_ulog, ulog.data(), nch * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
+	{
+		ProfScope ps(h, "k_chunk_ticks", (double)h->N * h->esz);
+		HIPC(launch_chunk_ticks(h->L(), h->amp, h->n, daf, h->d_ulog, h->d_ticks, h->d_ties));
+	}
+	HIPC(hipMemcpyAsync(ticks.data(), h->d_ticks, nch * sizeof(long long), hipMemcpyDeviceToHost, h->stream));
+	HIPC(hipMemcpyAsync(ties.data(), h->d_ties, nch * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+	HIPC(hipStreamSynchronize(h->stream));
+
+	std::vector<double> buf(2 * chunk);
+	std::vector<float> fbuf(h->prec == 1 ? 2 * chunk : 0);
+	auto load_chunk = [&](u64 c) {
+		if(h->prec == 0) {
+			HIPC(hipMemcpy(buf.data(), (char *)h->amp + c * chunk * 16, chunk * 16, hipMemcpyDeviceToHost));
+		} else {
+			HIPC(hipMemcpy(fbuf.data(), (char *)h->amp + c * chunk * 8, chunk * 8, hipMemcpyDeviceToHost));
+			for(u64 i = 0; i < 2 * chunk; ++i) buf[i] = fbuf[i];
+		}
+	};
+	ChainResult r;
+	double S = 0.0;
+	for(u64 c = 0; c < nch; ++c) {
+		if(sums[c] == 0.0) continue;  // every p_i is +0: S unchanged
+		bool fast = false;
+		if(ulog[c] != INT32_MIN && !ties[c] && S > 0x1p-1000 && std::ilogb(S) - 52 == ulog[c]) {
+			const double K = std::ldexp(S, -ulog[c]);  // exact integer
+			const double Kend = K + (double)ticks[c];
+			if(Kend < 0x1p53) {
+				const double Send = std::ldexp(Kend, ulog[c]);
+				if(!(find && rnd <= Send)) {
+					S = Send;
+					fast = true;
+				}
+			}
+		}
+		if(fast) continue;
+		load_chunk(c);
+		for(u64 i = 0; i < chunk; ++i) {
+			S += host_prob(buf[2 * i], buf[2 * i + 1], af);
+			if(find && rnd <= S) {
+				r.found = true;
+				r.index = c * chunk + i;
+				r.total = S;
+				return r;
+			}
+		}
+	}
+	r.total = S;
+	return r;
+}
+
+// ------------------------------------------------------------ bulk copies
+void load_host(tmfqs_hip_state *h, const double *src) {
+	const size_t stage = 64u << 20;
+	ensure_stage(h, 2 * stage);
+	const u64 total_states = h->N;
+	const u64 per = stage / h->esz;
+	hipEvent_t ev[2];
+	HIPC(hipEventCreate(&ev[0]));
+	HIPC(hipEventCreate(&ev[1]));
+	bool used[2] = {false, false};
+	u64 s0 = 0;
+	int slot = 0;
+	while(s0 < total_states) {
+		const u64 cnt = std::min<u64>(per, total_states - s0);
+		if(used[slot]) HIPC(hipEventSynchronize(ev[slot]));
+		char *dst = h->h_stage + slot * stage;
+		if(h->prec == 0) {
+			std::memcpy(dst, src + 2 * s0, cnt * 16);
+		} else {
+			float *f = (float *)dst;
+			for(u64 i = 0; i < 2 * cnt; ++i) f[i] = (float)src[2 * s0 + i];
+		}
+		HIPC(hipMemcpyAsync((char *)h->amp + s0 * h->esz, dst, cnt * h->esz, hipMemcpyHostToDevice, h->stream));
+		HIPC(hipEventRecord(ev[slot], h->stream));
+		used[slot] = true;
+		s0 += cnt;
+		slot ^= 1;
+	}
+	HIPC(hipStreamSynchronize(h->stream));
+	hipEventDestroy(ev[0]);
+	hipEventDestroy(ev[1]);
+}
+
+void export_host(tmfqs_hip_state *h, u64 first, u64 count, double *out) {
+	sync_all(h);
+	const size_t stage = 64u << 20;
+	ensure_stage(h, stage);
+	const u64 per = stage / h->esz;
+	u64 s0 = 0;
+	while(s0 < count) {
+		const u64 cnt = std::min<u64>(per, count - s0);
+		HIPC(hipMemcpy(h->h_stage, (char *)h->amp + (first + s0) * h->esz, cnt * h->esz, hipMemcpyDeviceToHost));
+		if(h->prec == 0) {
+			std::memcpy(out + 2 * s0, h->h_stage, cnt * 16);
+		} else {
+			const float *f = (const float *)h->h_stage;
+			for(u64 i = 0; i < 2 * cnt; ++i) out[2 * s0 + i] = f[i];
+		}
+		s0 += cnt;
+	}
+	// exported values reflect queued exact scatters already (sync_all flushed)
+}
+
+void device_zero(tmfqs_hip_state *h) { HIPC(hipMemsetAsync(h->amp, 0, h->amp_bytes, h->stream)); }
+
+void device_set(tmfqs_hip_state *h, u64 s, double re, double im) {
+	Scatter sc{s, re, im};
+	const Scatter *d = (const Scatter *)arena_upload(h, &sc, sizeof(sc));
+	HIPC(launch_scatter(h->L(), h->amp, d, 1));
+}
+
+tmfqs_hip_state *checked(tmfqs_hip_state *h) {
+	if(!h) throw std::invalid_argument("null tmfqs_hip_state handle");
+	return h;
+}
+
+void destroy_handle(tmfqs_hip_state *h) {
+	if(!h) return;
+	hipSetDevice(h->cfg.device);
+	if(h->stream) hipStreamSynchronize(h->stream);
+	for(auto &r : h->recs) {
+		hipEventDestroy(r.start);
+		hipEventDestroy(r.stop);
+	}
+	for(auto e : h->event_pool) hipEventDestroy(e);
+	hipFree(h->amp);
+	hipFree(h->scratch);
+	hipFree(h->d_red);
+	hipFree(h->d_phase);
+	hipFree(h->d_sums);
+	hipFree(h->d_ticks);
+	hipFree(h->d_ulog);
+	hipFree(h->d_ties);
+	if(h->arena.host) hipHostFree(h->arena.host);
+	if(h->arena.dev) hipFree(h->arena.dev);
+	if(h->h_stage) hipHostFree(h->h_stage);
+	if(h->stream) hipStreamDestroy(h->stream);
+	delete h;
+}
+
+}  // namespace
+
+// =========================================================================
+// extern "C" ABI
+#define ABI_BEGIN try {
+#define ABI_END                 \
+	return TMFQS_HIP_OK;        \
+	}                           \
+	catch(...) {                \
+		return map_exception(); \
+	}
+
+extern "C" {
+
+const char *tmfqs_hip_last_error(void) { return g_last_error.c_str(); }
+
+int tmfqs_hip_abi_version(void) { return TMFQS_HIP_ABI_VERSION; }
+
+int tmfqs_hip_device_count(int *count) {
+	ABI_BEGIN
+	int c = 0;
+	if(hipGetDeviceCount(&c) != hipSuccess) c = 0;
+	*count = c;
+	ABI_END
+}
+
+void tmfqs_hip_config_default(tmfqs_hip_config *cfg) {
+	if(!cfg) return;
+	std::memset(cfg, 0, sizeof(*cfg));
+	cfg->precision = TMFQS_HIP_COMPLEX128;
+	cfg->device = 0;
+	cfg->fusion = 1;
+	cfg->exact_phases = 0;
+}
+
+int tmfqs_hip_create(const tmfqs_hip_config *cfg, tmfqs_hip_state **out) {
+	tmfqs_hip_state *h = nullptr;
+	try {
+		if(!out) throw std::invalid_argument("tmfqs_hip_create: null output pointer");
+		*out = nullptr;
+		h = new tmfqs_hip_state();
+		if(cfg)
+			h->cfg = *cfg;
+		else
+			tmfqs_hip_config_default(&h->cfg);
+		if(h->cfg.precision != TMFQS_HIP_COMPLEX128 && h->cfg.precision != TMFQS_HIP_COMPLEX64)
+			throw std::invalid_argument("tmfqs_hip_create: unknown precision");
+		// a tile of 2^T amplitudes plus its index table must fit the 160 KiB LDS of a gfx950 CU
+		const int maxT = h->cfg.precision == TMFQS_HIP_COMPLEX128 ? 13 : 14;
+		if(h->cfg.tile_qubits < 0 || h->cfg.tile_qubits > maxT) throw std::invalid_argument("tmfqs_hip_create: tile_qubits out of range");
+		if(h->cfg.coalesce_qubits < 0 || h->cfg.coalesce_qubits > 14)
+			throw std::invalid_argument("tmfqs_hip_create: coalesce_qubits out of range");
+		h->prec = h->cfg.precision;
+		h->esz = h->prec == 0 ? 16 : 8;
+		int count = 0;
+		if(hipGetDeviceCount(&count) != hipSuccess || count == 0) throw DeviceError("tmfqs_hip_create: no HIP device available");
+		if(h->cfg.device < 0 || h->cfg.device >= count) throw std::invalid_argument("tmfqs_hip_create: device ordinal out of range");
+		HIPC(hipSetDevice(h->cfg.device));
+		HIPC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+		HIPC(hipMalloc((void **)&h->d_red, (2 * kReduceBlocks + 16) * sizeof(double)));
+		build_phase_tables(h);
+		arena_ensure(h, 8u << 20);
+		*out = h;
+		return TMFQS_HIP_OK;
+	} catch(...) {
+		const int rc = map_exception();
+		destroy_handle(h);
+		return rc;
+	}
+}
+
+int tmfqs_hip_destroy(tmfqs_hip_state *h) {
+	ABI_BEGIN
+	destroy_handle(h);
+	ABI_END
+}
+
+int tmfqs_hip_clone(tmfqs_hip_state *src, tmfqs_hip_state **out) {
+	tmfqs_hip_state *h = nullptr;
+	try {
+		checked(src);
+		HIPC(hipSetDevice(src->cfg.device));
+		sync_all(src);
+		const int rc = tmfqs_hip_create(&src->cfg, &h);
+		if(rc) return rc;
+		if(src->init) {
+			allocate_state(h, src->n);
+			HIPC(hipMemcpyAsync(h->amp, src->amp, src->amp_bytes, hipMemcpyDeviceToDevice, h->stream));
+			HIPC(hipStreamSynchronize(h->stream));
+			h->tracked = src->tracked;
+		}
+		*out = h;
+		return TMFQS_HIP_OK;
+	} catch(...) {
+		const int rc = map_exception();
+		destroy_handle(h);
+		return rc;
+	}
+}
+
+int tmfqs_hip_qubit_count(tmfqs_hip_state *h, unsigned *out) {
+	ABI_BEGIN
+	*out = checked(h)->n;
+	ABI_END
+}
+
+int tmfqs_hip_get_config(tmfqs_hip_state *h, tmfqs_hip_config *out) {
+	ABI_BEGIN
+	*out = checked(h)->cfg;
+	ABI_END
+}
+
+int tmfqs_hip_init_zero(tmfqs_hip_state *h, unsigned n) {
+	ABI_BEGIN
+	checked(h);
+	HIPC(hipSetDevice(h->cfg.device));
+	allocate_state(h, n);
+	device_zero(h);
+	ABI_END
+}
+
+int tmfqs_hip_init_basis(tmfqs_hip_state *h, unsigned n, uint64_t state, double re, double im) {
+	ABI_BEGIN
+	checked(h);
+	HIPC(hipSetDevice(h->cfg.device));
+	allocate_state(h, n);
+	device_zero(h);
+	validate_state(h, state, "HipStateBackend::initBasis");
+	device_set(h, state, re, im);
+	ABI_END
+}
+
+int tmfqs_hip_init_uniform(tmfqs_hip_state *h, unsigned n, const uint64_t *states, size_t count) {
+	ABI_BEGIN
+	checked(h);
+	HIPC(hipSetDevice(h->cfg.device));
+	allocate_state(h, n);
+	device_zero(h);
+	std::vector<u64> sel;
+	for(size_t i = 0; i < count; ++i) {
+		validate_state(h, states[i], "HipStateBackend::initUniformSuperposition");
+		sel.push_back(states[i]);
+	}
+	std::sort(sel.begin(), sel.end());
+	sel.erase(std::unique(sel.begin(), sel.end()), sel.end());
+	if(sel.empty()) throw std::invalid_argument("HipStateBackend::initUniformSuperposition requires at least one state");
+	const double amp = 1.0 / std::sqrt((double)sel.size());
+	std::vector<Scatter> sc;
+	sc.reserve(sel.size());
+	for(u64 s : sel) sc.push_back(Scatter{s, amp, 0.0});
+	for(size_t s0 = 0; s0 < sc.size(); s0 += 1u << 20) {
+		const size_t cnt = std::min<size_t>(1u << 20, sc.size() - s0);
+		const Scatter *d = (const Scatter *)arena_upload(h, sc.data() + s0, cnt * sizeof(Scatter));
+		HIPC(launch_scatter(h->L(), h->amp, d, (unsigned)cnt));
+	}
+	ABI_END
+}
+
+int tmfqs_hip_load(tmfqs_hip_state *h, unsigned n, const double *amps, uint64_t elem_count) {
+	ABI_BEGIN
+	checked(h);
+	HIPC(hipSetDevice(h->cfg.device));
+	if(n > 63u) throw std::invalid_argument("Number of qubits exceeds supported range for state indexing");
+	if(elem_count != 2ull * (1ull << n)) throw std::invalid_argument("HipStateBackend: amplitudes size mismatch");
+	allocate_state(h, n);
+	load_host(h, amps);
+	ABI_END
+}
+
+int tmfqs_hip_export(tmfqs_hip_state *h, uint64_t first, uint64_t count, double *out) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "amplitude export");
+	HIPC(hipSetDevice(h->cfg.device));
+	if(first > h->N || count > h->N - first) throw std::out_of_range("HipStateBackend::export range out of bounds");
+	export_host(h, first, count, out);
+	ABI_END
+}
+
+int tmfqs_hip_amplitude(tmfqs_hip_state *h, uint64_t state, double *re, double *im) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "amplitude query");
+	validate_state(h, state, "HipStateBackend::amplitude");
+	HIPC(hipSetDevice(h->cfg.device));
+	const Cplx v = current_value(h, state);
+	*re = v.re;
+	*im = v.im;
+	ABI_END
+}
+
+int tmfqs_hip_set_amplitude(tmfqs_hip_state *h, uint64_t state, double re, double im) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "state update");
+	validate_state(h, state, "HipStateBackend::setAmplitude");
+	HIPC(hipSetDevice(h->cfg.device));
+	if(!h->cfg.fusion) {
+		sync_all(h);
+		device_set(h, state, re, im);
+		h->tracked[state] = Cplx{re, im};
+		return TMFQS_HIP_OK;
+	}
+	h->tracked[state] = Cplx{re, im};
+	mark_dirty(h, state);
+	QOp op;
+	op.kind = Q_SET;
+	op.state = state;
+	h->queue.push_back(op);
+	ABI_END
+}
+
+int tmfqs_hip_probability(tmfqs_hip_state *h, uint64_t state, double *out) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "amplitude query");
+	validate_state(h, state, "HipStateBackend::amplitude");
+	HIPC(hipSetDevice(h->cfg.device));
+	const Cplx v = current_value(h, state);
+	*out = v.re * v.re + v.im * v.im;
+	ABI_END
+}
+
+int tmfqs_hip_total_probability(tmfqs_hip_state *h, double *out) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "total probability query");
+	HIPC(hipSetDevice(h->cfg.device));
+	*out = chain_walk(h, false, 0.0, nullptr).total;
+	ABI_END
+}
+
+int tmfqs_hip_sample(tmfqs_hip_state *h, double rnd, uint64_t *out) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "measurement");
+	if(rnd < 0.0 || rnd >= 1.0) throw std::invalid_argument("HipStateBackend::sampleMeasurement requires rnd in [0,1)");
+	HIPC(hipSetDevice(h->cfg.device));
+	const ChainResult r = chain_walk(h, true, rnd, nullptr);
+	if(!r.found) throw std::runtime_error("HipStateBackend::sampleMeasurement cumulative probability did not reach sample");
+	*out = r.index;
+	ABI_END
+}
+
+int tmfqs_hip_amplitude_sum(tmfqs_hip_state *h, double *re, double *im) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "amplitude chunk iteration");
+	HIPC(hipSetDevice(h->cfg.device));
+	flush(h);
+	{
+		ProfScope ps(h, "k_sum", (double)h->N * h->esz);
+		HIPC(launch_sum(h->L(), h->amp, h->n, h->d_red, h->d_red + 2 * kReduceBlocks));
+	}
+	double r[2];
+	HIPC(hipMemcpyAsync(r, h->d_red + 2 * kReduceBlocks, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+	HIPC(hipStreamSynchronize(h->stream));
+	*re = r[0];
+	*im = r[1];
+	ABI_END
+}
+
+int tmfqs_hip_total_probability_affine(tmfqs_hip_state *h, double s_re, double s_im, double b_re, double b_im, double *out) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "probability summation");
+	HIPC(hipSetDevice(h->cfg.device));
+	const double af[4] = {s_re, s_im, b_re, b_im};
+	*out = chain_walk(h, false, 0.0, af).total;
+	ABI_END
+}
+
+int tmfqs_hip_sample_affine(tmfqs_hip_state *h, double rnd, double s_re, double s_im, double b_re, double b_im, uint64_t *out) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "measurement");
+	HIPC(hipSetDevice(h->cfg.device));
+	const double af[4] = {s_re, s_im, b_re, b_im};
+	const ChainResult r = chain_walk(h, true, rnd, af);
+	if(!r.found) throw std::runtime_error("QuantumRegister::measure cumulative probability did not reach sample");
+	*out = r.index;
+	ABI_END
+}
+
+int tmfqs_hip_phase_flip(tmfqs_hip_state *h, uint64_t state) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "basis-state phase flip");
+	validate_state(h, state, "HipStateBackend::phaseFlipBasisState");
+	HIPC(hipSetDevice(h->cfg.device));
+	Cplx v = current_value(h, state);
+	v.re = -v.re;
+	v.im = -v.im;
+	h->tracked[state] = v;
+	if(!h->cfg.fusion) {
+		device_set(h, state, v.re, v.im);
+		return TMFQS_HIP_OK;
+	}
+	mark_dirty(h, state);
+	QOp op;
+	op.kind = Q_FLIP;
+	op.state = state;
+	h->queue.push_back(op);
+	ABI_END
+}
+
+int tmfqs_hip_inversion_about_mean(tmfqs_hip_state *h) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "inversion about mean");
+	double re, im;
+	const int rc = tmfqs_hip_amplitude_sum(h, &re, &im);
+	if(rc) return rc;
+	return tmfqs_hip_inversion_about_mean_value(h, re / (double)h->N, im / (double)h->N);
+	ABI_END
+}
+
+int tmfqs_hip_inversion_about_mean_value(tmfqs_hip_state *h, double mean_re, double mean_im) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "inversion about mean");
+	HIPC(hipSetDevice(h->cfg.device));
+	QOp op;
+	op.kind = Q_DIFF;
+	op.ar = mean_re;
+	op.ai = mean_im;
+	enqueue_ew(h, op);
+	ABI_END
+}
+
+int tmfqs_hip_affine(tmfqs_hip_state *h, double s_re, double s_im, double b_re, double b_im) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "affine transform");
+	HIPC(hipSetDevice(h->cfg.device));
+	QOp op;
+	op.kind = Q_AFFINE;
+	op.ar = s_re;
+	op.ai = s_im;
+	op.br = b_re;
+	op.bi = b_im;
+	enqueue_ew(h, op);
+	ABI_END
+}
+
+int tmfqs_hip_hadamard(tmfqs_hip_state *h, unsigned qubit) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "HipStateBackend::applyHadamard");
+	validate_qubit(h, qubit, "HipStateBackend::applyHadamard");
+	HIPC(hipSetDevice(h->cfg.device));
+	QOp op;
+	op.kind = Q_H;
+	op.b0 = pbit(h, qubit);
+	enqueue_gate(h, op);
+	ABI_END
+}
+
+int tmfqs_hip_pauli_x(tmfqs_hip_state *h, unsigned qubit) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "HipStateBackend::applyPauliX");
+	validate_qubit(h, qubit, "HipStateBackend::applyPauliX");
+	HIPC(hipSetDevice(h->cfg.device));
+	QOp op;
+	op.kind = Q_X;
+	op.b0 = pbit(h, qubit);
+	enqueue_gate(h, op);
+	ABI_END
+}
+
+int tmfqs_hip_controlled_phase(tmfqs_hip_state *h, unsigned control, unsigned target, double theta) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "HipStateBackend::applyControlledPhaseShift");
+	validate_two(h, control, target, "HipStateBackend::applyControlledPhaseShift");
+	HIPC(hipSetDevice(h->cfg.device));
+	QOp op;
+	op.kind = Q_CP;
+	op.b0 = pbit(h, control);
+	op.b1 = pbit(h, target);
+	op.theta = theta;
+	op.cr = std::cos(theta);
+	op.ci = std::sin(theta);
+	enqueue_gate(h, op);
+	ABI_END
+}
+
+int tmfqs_hip_controlled_not(tmfqs_hip_state *h, unsigned control, unsigned target) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "HipStateBackend::applyControlledNot");
+	validate_two(h, control, target, "HipStateBackend::applyControlledNot");
+	HIPC(hipSetDevice(h->cfg.device));
+	QOp op;
+	op.kind = Q_CX;
+	op.b0 = pbit(h, control);
+	op.b1 = pbit(h, target);
+	enqueue_gate(h, op);
+	ABI_END
+}
+
+int tmfqs_hip_swap(tmfqs_hip_state *h, unsigned q1, unsigned q2) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "swap");
+	validate_two(h, q1, q2, "QuantumRegister::applySwap");
+	HIPC(hipSetDevice(h->cfg.device));
+	QOp op;
+	op.kind = Q_SWAP;
+	op.b0 = pbit(h, q1);
+	op.b1 = pbit(h, q2);
+	enqueue_gate(h, op);
+	ABI_END
+}
+
+int tmfqs_hip_apply_gate(tmfqs_hip_state *h, const double *mat, unsigned k, const unsigned *qubits) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "gate application");
+	const char *scope = "HipStateBackend::applyGate";
+	// validation.h:56-78 of the reference
+	if(k > 63u) throw std::invalid_argument(std::string(scope) + " qubit list exceeds supported index width");
+	if(k > h->n) throw std::invalid_argument(std::string(scope) + " gate acts on more qubits than available");
+	std::vector<bool> seen(h->n, false);
+	for(unsigned i = 0; i < k; ++i) {
+		validate_qubit(h, qubits[i], scope);
+		if(seen[qubits[i]]) throw std::invalid_argument(std::string(scope) + " duplicate qubit index");
+		seen[qubits[i]] = true;
+	}
+	if(k > 5) throw std::invalid_argument(std::string(scope) + " supports gates on at most 5 qubits");
+	HIPC(hipSetDevice(h->cfg.device));
+	const size_t dim = 1ull << k;
+	QOp op;
+	op.mat = h->qmats.size();
+	h->qmats.insert(h->qmats.end(), mat, mat + 2 * dim * dim);
+	if(k == 0) {
+		// 1x1 matrix: a global scalar, an affine step with zero bias.
+		QOp a;
+		a.kind = Q_AFFINE;
+		a.ar = mat[0];
+		a.ai = mat[1];
+		h->qmats.resize(op.mat);
+		enqueue_ew(h, a);
+		return TMFQS_HIP_OK;
+	}
+	if(k == 1) {
+		op.kind = Q_U1;
+		op.b0 = pbit(h, qubits[0]);
+	} else {
+		op.kind = Q_UK;
+		op.k = k;
+		for(unsigned i = 0; i < k; ++i) op.bits[i] = pbit(h, qubits[i]);
+	}
+	enqueue_gate(h, op);
+	ABI_END
+}
+
+int tmfqs_hip_begin_batch(tmfqs_hip_state *h) {
+	ABI_BEGIN
+	checked(h);
+	ABI_END
+}
+
+int tmfqs_hip_end_batch(tmfqs_hip_state *h) {
+	ABI_BEGIN
+	checked(h);
+	if(h->init) {
+		HIPC(hipSetDevice(h->cfg.device));
+		flush(h);
+	}
+	ABI_END
+}
+
+int tmfqs_hip_synchronize(tmfqs_hip_state *h) {
+	ABI_BEGIN
+	checked(h);
+	HIPC(hipSetDevice(h->cfg.device));
+	if(h->init) sync_all(h);
+	HIPC(hipStreamSynchronize(h->stream));
+	ABI_END
+}
+
+int tmfqs_hip_profile_enable(tmfqs_hip_state *h, int on) {
+	ABI_BEGIN
+	checked(h)->profiling = on != 0;
+	ABI_END
+}
+
+int tmfqs_hip_profile_reset(tmfqs_hip_state *h) {
+	ABI_BEGIN
+	checked(h);
+	collect_profile(h);
+	h->prof.clear();
+	ABI_END
+}
+
+int tmfqs_hip_profile_read(tmfqs_hip_state *h, char (*names)[64], uint64_t *launches, double *ms, double *bytes, size_t *count) {
+	ABI_BEGIN
+	checked(h);
+	collect_profile(h);
+	size_t i = 0;
+	for(auto &kv : h->prof) {
+		if(i >= *count) break;
+		std::snprintf(names[i], 64, "%s", kv.first.c_str());
+		launches[i] = kv.second.launches;
+		ms[i] = kv.second.ms;
+		bytes[i] = kv.second.bytes;
+		++i;
+	}
+	*count = i;
+	ABI_END
+}
+
+int tmfqs_hip_stream(tmfqs_hip_state *h, void **stream) {
+	ABI_BEGIN
+	*stream = (void *)checked(h)->stream;
+	ABI_END
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+// Internal interfaces shared by the gfx950 kernels (kernels.hip) and the host
+// engine (engine.cpp).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace tmfqs_hip {
+
+using u64 = uint64_t;
+
+// ---------------------------------------------------------------- tile ops
+// A tiled pass loads 2^T amplitudes whose indices vary over T "local" bit
+// positions (the low C bits 0..C-1 plus up to kMaxHighBits higher positions)
+// into LDS, applies a list of TileOps, and stores them back.
+constexpr int kMaxHighBits = 10;
+constexpr int kMaxTwiddleTerms = 16;
+constexpr int kMaxTileOps = 512;
+
+enum TileOpKind : uint32_t {
+	TOP_H = 0,        // Hadamard on local bit j0 (zero pairs untouched)
+	TOP_X = 1,        // Pauli-X / controlled-NOT on local bit j0 (equal pairs untouched)
+	TOP_U1 = 2,       // 2x2 matrix on local bit j0 (reference k==1 arithmetic)
+	TOP_UK = 3,       // 2^k x 2^k matrix on local bits jl[0..k) (reference k>=2 arithmetic)
+	TOP_CP = 4,       // multiply by (cr, ci) where all bits of the condition are set
+	TOP_LADDER = 5,   // multiply by e^{2 pi i F / 2^64}, F = (g & amask) << shift, where the condition holds
+	TOP_TWIDDLE = 6,  // multiply by e^{2 pi i F / 2^64}, F = sum over terms whose bit is set of (g & mask) << shift
+};
+
+struct TwiddleTerm {
+	u64 cond;   // global bit that enables the term
+	u64 mask;   // global bits feeding F
+	uint32_t shift;
+	uint32_t pad;
+};
+
+struct TileOp {
+	uint32_t kind;
+	uint32_t j0;          // local bit (H, X, U1)
+	uint32_t k;           // UK arity
+	uint32_t mat;         // offset (doubles) into the matrix pool
+	uint32_t jl[5];       // UK local bits, jl[0] = MSB of the matrix index
+	uint32_t nterms;      // TWIDDLE terms
+	u64 lcond;            // condition on local index bits (all set)
+	u64 gcond;            // condition on global index bits (all set)
+	u64 amask;            // LADDER mask
+	uint32_t shift;       // LADDER shift
+	uint32_t twid;        // TWIDDLE: index of the first term in the term pool
+	double cr, ci;        // CP phase
+};
+
+struct TilePass {
+	u64 ntiles;           // 2^(n - T)
+	uint32_t n;           // qubits
+	uint32_t T;           // local bits
+	uint32_t C;           // contiguous low local bits
+	uint32_t nh;          // high local bits (T - C)
+	uint32_t hbits[kMaxHighBits];
+	uint32_t nops;
+	uint32_t op0;         // first op in the op pool
+};
+
+// ------------------------------------------------------ elementwise chains
+enum EwKind : uint32_t {
+	EW_DIFFUSE = 0,  // a = 2 m - a
+	EW_AFFINE = 1,   // a = s a + b (reference applyAffineTransform arithmetic)
+};
+
+struct EwStep {
+	uint32_t kind;
+	uint32_t pad;
+	double ar, ai;   // mean (DIFFUSE) or scale (AFFINE)
+	double br, bi;   // bias (AFFINE)
+};
+
+struct Scatter {
+	u64 index;
+	double re, im;
+};
+
+// ------------------------------------------------------------ launches
+// All launch helpers enqueue on `stream` and return hipError_t of the launch.
+// `prec` selects complex128 (0) or complex64 (1) storage.
+struct Launch {
+	hipStream_t stream;
+	int prec;
+};
+
+hipError_t launch_hadamard(const Launch &L, void *a, unsigned n, unsigned bit);
+hipError_t launch_pauli_x(const Launch &L, void *a, unsigned n, unsigned bit, u64 cmask);
+hipError_t launch_cphase(const Launch &L, void *a, unsigned n, unsigned tbit, u64 cmask, double cr, double ci);
+hipError_t launch_u1(const Launch &L, void *a, unsigned n, unsigned bit, const double *mat8);
+hipError_t launch_uk(const Launch &L, void *a, unsigned n, unsigned k, const unsigned *bits, const double *dmat);
+hipError_t launch_bitswap(const Launch &L, void *a, unsigned n, unsigned p, unsigned q);
+hipError_t launch_involution(const Launch &L, void *a, unsigned n, const unsigned *perm, unsigned C);
+hipError_t launch_ew_chain(const Launch &L, void *a, unsigned n, const EwStep *dsteps, unsigned nsteps);
+hipError_t launch_scatter(const Launch &L, void *a, const Scatter *dsc, unsigned count);
+hipError_t launch_tile_pass(const Launch &L, void *a, const TilePass &P, const TileOp *dops, const double *dmats,
+                            const TwiddleTerm *dterms, const double *dphase);
+// Reductions write `blocks` partials into dpart, then reduce into dout
+// (sum: 2 doubles; norm: 1 double).
+constexpr unsigned kReduceBlocks = 1024;
+hipError_t launch_sum(const Launch &L, const void *a, unsigned n, double *dpart, double *dout);
+hipError_t launch_norm(const Launch &L, const void *a, unsigned n, const double *affine4, double *dpart,
+                       double *dout);
+// Sampling support: per-chunk FP64 probability sums (chunk = kSampleChunk states).
+constexpr unsigned kSampleChunkLog = 12;
+hipError_t launch_chunk_prob(const Launch &L, const void *a, unsigned n, const double *affine4, double *dsums);
+// Per-chunk integer increment sums of the sequential chain on the grid
+// u = 2^ulog[c] (int64), with a tie flag per chunk; ulog[c] == INT32_MIN marks
+// a chunk that is resolved on the host.
+hipError_t launch_chunk_ticks(const Launch &L, const void *a, unsigned n, const double *affine4,
+                              const int32_t *dulog, long long *dticks, int32_t *dties);
+
+}  // namespace tmfqs_hip

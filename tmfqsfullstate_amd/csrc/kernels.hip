@@ -1,0 +1,854 @@
+// gfx950 kernels of the dense state-vector engine.
+//
+// Storage is interleaved complex (double2 for complex128, float2 for
+// complex64), basis index bit p holds qubit n-1-p.  Every kernel indexes with
+// 64-bit integers.  The library is compiled with -ffp-contract=off so that the
+// per-operation kernels reproduce the reference dense backend's arithmetic
+// (src/tmfqs/storage/dense/dense_state_backend.cpp) bit for bit; kernels that
+// are not bit-exact by design (phase ladders, twiddles) use explicit fma().
+#include "engine.h"
+
+#include <climits>
+#include <cmath>
+
+namespace tmfqs_hip {
+
+template <typename R> struct V2T;
+template <> struct V2T<double> { using t = double2; };
+template <> struct V2T<float> { using t = float2; };
+
+__device__ __forceinline__ u64 gtid() { return (u64)blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ u64 gstride() { return (u64)gridDim.x * blockDim.x; }
+__host__ __device__ __forceinline__ u64 ins0(u64 x, unsigned b) {
+	return ((x >> b) << (b + 1)) | (x & ((1ull << b) - 1ull));
+}
+
+static unsigned grid_for(u64 work, unsigned threads = 256, u64 maxblocks = 8192) {
+	u64 b = (work + threads - 1) / threads;
+	if(b < 1) b = 1;
+	if(b > maxblocks) b = maxblocks;
+	return (unsigned)b;
+}
+
+template <typename V> __device__ __forceinline__ V mk(double x, double y);
+template <> __device__ __forceinline__ double2 mk<double2>(double x, double y) { return make_double2(x, y); }
+template <> __device__ __forceinline__ float2 mk<float2>(double x, double y) { return make_float2((float)x, (float)y); }
+
+// Streaming (non-temporal) 16/8-byte loads and stores: a sweep touches each
+// amplitude once, so it should not displace reusable lines in L2/MALL.
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+typedef float nt_f2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 nt_load(const double2 *p) {
+	const nt_d2 v = __builtin_nontemporal_load((const nt_d2 *)p);
+	return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ float2 nt_load(const float2 *p) {
+	const nt_f2 v = __builtin_nontemporal_load((const nt_f2 *)p);
+	return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void nt_store(double2 v, double2 *p) {
+	nt_d2 w;
+	w.x = v.x;
+	w.y = v.y;
+	__builtin_nontemporal_store(w, (nt_d2 *)p);
+}
+__device__ __forceinline__ void nt_store(float2 v, float2 *p) {
+	nt_f2 w;
+	w.x = v.x;
+	w.y = v.y;
+	__builtin_nontemporal_store(w, (nt_f2 *)p);
+}
+
+// ------------------------------------------------------------------------
+// Per-operation sweeps (one pass per reference backend call).
+
+// dense_state_backend.cpp:255-273: a0' = (a0+a1)c, a1' = (a0-a1)c, all-zero pairs untouched.
+template <typename R>
+__global__ void __launch_bounds__(256) k_hadamard(typename V2T<R>::t *__restrict__ a, u64 npairs, unsigned bit, R c) {
+	for(u64 t = gtid(); t < npairs; t += gstride()) {
+		const u64 i0 = ins0(t, bit), i1 = i0 | (1ull << bit);
+		const auto x = a[i0], y = a[i1];
+		if(x.x == 0 && x.y == 0 && y.x == 0 && y.y == 0) continue;
+		typename V2T<R>::t o0, o1;
+		o0.x = (x.x + y.x) * c;
+		o0.y = (x.y + y.y) * c;
+		o1.x = (x.x - y.x) * c;
+		o1.y = (x.y - y.y) * c;
+		a[i0] = o0;
+		a[i1] = o1;
+	}
+}
+
+// dense_state_backend.cpp:276-286 / :306-318: swap the pair when state0 holds
+// every control bit, unless the two amplitudes are equal.
+template <typename R>
+__global__ void __launch_bounds__(256) k_pauli_x(typename V2T<R>::t *__restrict__ a, u64 npairs, unsigned bit, u64 cmask) {
+	for(u64 t = gtid(); t < npairs; t += gstride()) {
+		const u64 i0 = ins0(t, bit), i1 = i0 | (1ull << bit);
+		if((i0 & cmask) != cmask) continue;
+		const auto x = a[i0], y = a[i1];
+		if(x.x == y.x && x.y == y.y) continue;
+		a[i0] = y;
+		a[i1] = x;
+	}
+}
+
+// dense_state_backend.cpp:289-303: states with the target and control bits set
+// are multiplied by (cr, ci); zero amplitudes untouched.
+template <typename R>
+__global__ void __launch_bounds__(256) k_cphase(typename V2T<R>::t *__restrict__ a, u64 nquads, unsigned lo, unsigned hi,
+                                                u64 setmask, R cr, R ci) {
+	for(u64 t = gtid(); t < nquads; t += gstride()) {
+		const u64 i = ins0(ins0(t, lo), hi) | setmask;
+		const auto x = a[i];
+		if(x.x == 0 && x.y == 0) continue;
+		typename V2T<R>::t o;
+		o.x = cr * x.x - ci * x.y;
+		o.y = cr * x.y + ci * x.x;
+		a[i] = o;
+	}
+}
+
+// gate_block_apply.h:117-143: one-qubit matrix with the k==1 accumulation order.
+template <typename R>
+__global__ void __launch_bounds__(256) k_u1(typename V2T<R>::t *__restrict__ a, u64 npairs, unsigned bit, R g00r, R g00i,
+                                            R g01r, R g01i, R g10r, R g10i, R g11r, R g11i) {
+	for(u64 t = gtid(); t < npairs; t += gstride()) {
+		const u64 i0 = ins0(t, bit), i1 = i0 | (1ull << bit);
+		const auto x = a[i0], y = a[i1];
+		if((x.x == 0 && x.y == 0) && (y.x == 0 && y.y == 0)) continue;
+		typename V2T<R>::t o0, o1;
+		o0.x = g00r * x.x - g00i * x.y + g01r * y.x - g01i * y.y;
+		o0.y = g00r * x.y + g00i * x.x + g01r * y.y + g01i * y.x;
+		o1.x = g10r * x.x - g10i * x.y + g11r * y.x - g11i * y.y;
+		o1.y = g10r * x.y + g10i * x.x + g11r * y.y + g11i * y.x;
+		a[i0] = o0;
+		a[i1] = o1;
+	}
+}
+
+// gate_block_apply.h:145-211: k-qubit matrix (k >= 2), blocks gathered in the
+// order of the matrix index (bits[0] is its MSB), all-zero blocks skipped,
+// rows accumulated column by column.
+template <typename R, int K>
+__global__ void __launch_bounds__(256) k_uk(typename V2T<R>::t *__restrict__ a, u64 nblocks, const double *__restrict__ dmat,
+                                            unsigned b0, unsigned b1, unsigned b2, unsigned b3, unsigned b4) {
+	constexpr int D = 1 << K;
+	__shared__ R g[2 * D * D];
+	for(int i = threadIdx.x; i < 2 * D * D; i += blockDim.x) g[i] = (R)dmat[i];
+	__syncthreads();
+	const unsigned bits[5] = {b0, b1, b2, b3, b4};
+	unsigned sorted[5];
+	for(int i = 0; i < K; ++i) sorted[i] = bits[i];
+	for(int i = 0; i < K; ++i)
+		for(int j = i + 1; j < K; ++j)
+			if(sorted[j] < sorted[i]) {
+				const unsigned t = sorted[i];
+				sorted[i] = sorted[j];
+				sorted[j] = t;
+			}
+	for(u64 blk = gtid(); blk < nblocks; blk += gstride()) {
+		u64 base = blk;
+		for(int i = 0; i < K; ++i) base = ins0(base, sorted[i]);
+		R xr[D], xi[D];
+		u64 idx[D];
+		bool nz = false;
+#pragma unroll
+		for(int r = 0; r < D; ++r) {
+			u64 off = 0;
+#pragma unroll
+			for(int i = 0; i < K; ++i)
+				if((r >> (K - 1 - i)) & 1) off |= 1ull << bits[i];
+			idx[r] = base | off;
+			const auto v = a[idx[r]];
+			xr[r] = v.x;
+			xi[r] = v.y;
+			nz = nz || (v.x != 0 || v.y != 0);
+		}
+		if(!nz) continue;
+#pragma unroll
+		for(int r = 0; r < D; ++r) {
+			R nr = 0, ni = 0;
+#pragma unroll
+			for(int c = 0; c < D; ++c) {
+				const R gr = g[2 * (r * D + c)], gi = g[2 * (r * D + c) + 1];
+				nr += gr * xr[c] - gi * xi[c];
+				ni += gr * xi[c] + gi * xr[c];
+			}
+			typename V2T<R>::t o;
+			o.x = nr;
+			o.y = ni;
+			a[idx[r]] = o;
+		}
+	}
+}
+
+// Swap bit positions p < q: exchange states with (p=1, q=0) and (p=0, q=1).
+template <typename R>
+__global__ void __launch_bounds__(256) k_bitswap(typename V2T<R>::t *__restrict__ a, u64 nquads, unsigned p, unsigned q) {
+	for(u64 t = gtid(); t < nquads; t += gstride()) {
+		const u64 i = ins0(ins0(t, p), q) | (1ull << p);
+		const u64 j = i ^ (1ull << p) ^ (1ull << q);
+		const auto x = a[i], y = a[j];
+		if(x.x == y.x && x.y == y.y) continue;
+		a[i] = y;
+		a[j] = x;
+	}
+}
+
+// In-place bit permutation by an involution perm (perm[perm[b]] = b).  The
+// local set S = {0..C-1} U perm({0..C-1}) is closed under perm, so tile A
+// (fixed bits outside S) and tile perm(A) swap contents; each workgroup owns
+// one such pair and stages both tiles in LDS.
+struct InvParams {
+	u64 npairs_tiles;  // 2^(n - s)
+	uint32_t n, C, s, nh;
+	uint32_t perm[64];
+	uint32_t hbits[16];  // S \ {0..C-1}, ascending
+	uint32_t lperm[16];  // perm restricted to S in local-bit numbering
+};
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_involution(typename V2T<R>::t *__restrict__ a, InvParams P) {
+	using V = typename V2T<R>::t;
+	extern __shared__ char smem[];
+	const u64 nloc = 1ull << P.s;
+	V *ta = (V *)smem;
+	V *tb = ta + nloc;
+	const u64 lowmask = (1ull << P.C) - 1ull;
+	for(u64 tid = blockIdx.x; tid < P.npairs_tiles; tid += gridDim.x) {
+		u64 base = tid << P.C;
+		for(unsigned b = 0; b < P.nh; ++b) base = ins0(base, P.hbits[b]);
+		u64 pb = 0;
+		for(unsigned b = 0; b < P.n; ++b)
+			if((base >> b) & 1ull) pb |= 1ull << P.perm[b];
+		if(pb < base) continue;  // uniform: the partner tile's workgroup handles the pair
+		const bool self = (pb == base);
+		__syncthreads();
+		for(u64 l = threadIdx.x; l < nloc; l += blockDim.x) {
+			u64 hi = 0;
+			for(unsigned b = 0; b < P.nh; ++b)
+				if((l >> (P.C + b)) & 1ull) hi |= 1ull << P.hbits[b];
+			const u64 off = (l & lowmask) | hi;
+			ta[l] = a[base | off];
+			if(!self) tb[l] = a[pb | off];
+		}
+		__syncthreads();
+		for(u64 l = threadIdx.x; l < nloc; l += blockDim.x) {
+			u64 hi = 0, src = 0;
+			for(unsigned b = 0; b < P.nh; ++b)
+				if((l >> (P.C + b)) & 1ull) hi |= 1ull << P.hbits[b];
+			for(unsigned b = 0; b < P.s; ++b)
+				if((l >> b) & 1ull) src |= 1ull << P.lperm[b];
+			const u64 off = (l & lowmask) | hi;
+			if(self) {
+				a[base | off] = ta[src];
+			} else {
+				a[pb | off] = ta[src];
+				a[base | off] = tb[src];
+			}
+		}
+	}
+}
+
+// ------------------------------------------------------------------------
+// Elementwise chains: the Grover diffusion recurrence and affine overlay
+// flushes folded into one sweep.  Each step reproduces the reference's
+// arithmetic (dense_state_backend.cpp:230-252); the chain runs in FP64
+// registers for both precisions.
+template <typename R>
+__global__ void __launch_bounds__(256) k_ew_chain(typename V2T<R>::t *__restrict__ a, u64 N, const EwStep *__restrict__ steps,
+                                                  unsigned nsteps) {
+	using V = typename V2T<R>::t;
+	for(u64 i = gtid(); i < N; i += gstride()) {
+		const V v = nt_load(&a[i]);
+		double re = v.x, im = v.y;
+		for(unsigned s = 0; s < nsteps; ++s) {
+			const EwStep st = steps[s];
+			if(st.kind == EW_DIFFUSE) {
+				re = 2.0 * st.ar - re;
+				im = 2.0 * st.ai - im;
+			} else {
+				const double nr = st.ar * re - st.ai * im + st.br;
+				const double ni = st.ar * im + st.ai * re + st.bi;
+				re = nr;
+				im = ni;
+			}
+		}
+		nt_store(mk<V>(re, im), &a[i]);
+	}
+}
+
+template <typename R>
+__global__ void k_scatter(typename V2T<R>::t *__restrict__ a, const Scatter *__restrict__ sc, unsigned count) {
+	const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i < count) a[sc[i].index] = mk<typename V2T<R>::t>(sc[i].re, sc[i].im);
+}
+
+// ------------------------------------------------------------------------
+// Reductions (FP64 accumulation, deterministic fixed-shape trees).
+__device__ __forceinline__ double wave_sum(double v) {
+	for(int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+	return v;
+}
+
+__device__ __forceinline__ long long wave_sum_ll(long long v) {
+	for(int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+	return v;
+}
+
+// Block of 256 threads: returns the sum in thread 0.
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+	v = wave_sum(v);
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	if(lane == 0) sh[w] = v;
+	__syncthreads();
+	double r = 0;
+	if(threadIdx.x == 0)
+		for(int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
+	__syncthreads();
+	return r;
+}
+
+__device__ __forceinline__ long long block_sum_ll(long long v, long long *sh) {
+	v = wave_sum_ll(v);
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	if(lane == 0) sh[w] = v;
+	__syncthreads();
+	long long r = 0;
+	if(threadIdx.x == 0)
+		for(int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
+	__syncthreads();
+	return r;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_sum(const typename V2T<R>::t *__restrict__ a, u64 N, double *part) {
+	__shared__ double sh[8];
+	double sr = 0, si = 0;
+	for(u64 i = gtid(); i < N; i += gstride()) {
+		const auto v = a[i];
+		sr += (double)v.x;
+		si += (double)v.y;
+	}
+	sr = block_sum(sr, sh);
+	si = block_sum(si, sh);
+	if(threadIdx.x == 0) {
+		part[2 * blockIdx.x] = sr;
+		part[2 * blockIdx.x + 1] = si;
+	}
+}
+
+// Logical amplitude (scale*a + bias with the reference's multiply-then-add,
+// quantum_register.cpp:161-166) and its probability re^2 + im^2.
+__device__ __forceinline__ double prob_of(double re, double im, const double *af) {
+	if(af) {
+		const double mr = af[0] * re - af[1] * im;
+		const double mi = af[0] * im + af[1] * re;
+		re = mr + af[2];
+		im = mi + af[3];
+	}
+	return re * re + im * im;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_norm(const typename V2T<R>::t *__restrict__ a, u64 N, const double *af, double *part) {
+	__shared__ double sh[8];
+	double s = 0;
+	for(u64 i = gtid(); i < N; i += gstride()) {
+		const auto v = a[i];
+		s += prob_of(v.x, v.y, af);
+	}
+	s = block_sum(s, sh);
+	if(threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void k_finish(const double *part, unsigned count, unsigned width, double *out) {
+	// one thread: fixed-order sum of the partials (count <= kReduceBlocks)
+	if(threadIdx.x == 0 && blockIdx.x == 0) {
+		for(unsigned w = 0; w < width; ++w) {
+			double s = 0;
+			for(unsigned i = 0; i < count; ++i) s += part[width * i + w];
+			out[w] = s;
+		}
+	}
+}
+
+// Per-chunk probability sums (chunk = 2^kSampleChunkLog states; fewer when N is smaller).
+template <typename R>
+__global__ void __launch_bounds__(256) k_chunk_prob(const typename V2T<R>::t *__restrict__ a, u64 N, u64 chunk, u64 nchunks,
+                                                    const double *af, double *sums) {
+	__shared__ double sh[8];
+	for(u64 c = blockIdx.x; c < nchunks; c += gridDim.x) {
+		double s = 0;
+		for(u64 i = threadIdx.x; i < chunk; i += blockDim.x) {
+			const auto v = a[c * chunk + i];
+			s += prob_of(v.x, v.y, af);
+		}
+		s = block_sum(s, sh);
+		if(threadIdx.x == 0) sums[c] = s;
+	}
+}
+
+// Per-chunk integer increments of the sequential chain S = fl(S + p_i) while S
+// stays in one binade whose ulp is u = 2^ulog: each step adds rint(p_i / u)
+// ulps exactly, unless p_i / u ends in exactly one half (a tie, flagged).
+template <typename R>
+__global__ void __launch_bounds__(256) k_chunk_ticks(const typename V2T<R>::t *__restrict__ a, u64 chunk, u64 nchunks,
+                                                     const double *af, const int32_t *ulog, long long *ticks,
+                                                     int32_t *ties) {
+	__shared__ long long sh[8];
+	for(u64 c = blockIdx.x; c < nchunks; c += gridDim.x) {
+		const int32_t ul = ulog[c];
+		if(ul == INT32_MIN) {
+			if(threadIdx.x == 0) {
+				ticks[c] = 0;
+				ties[c] = 0;
+			}
+			continue;
+		}
+		long long k = 0;
+		int tie = 0;
+		for(u64 i = threadIdx.x; i < chunk; i += blockDim.x) {
+			const auto v = a[c * chunk + i];
+			const double p = prob_of(v.x, v.y, af);
+			const double q = ldexp(p, -ul);
+			const double f = floor(q);
+			if(q - f == 0.5) tie = 1;
+			k += (long long)rint(q);
+		}
+		k = block_sum_ll(k, sh);
+		const long long t = block_sum_ll((long long)tie, sh);
+		if(threadIdx.x == 0) {
+			ticks[c] = k;
+			ties[c] = t != 0;
+		}
+	}
+}
+
+// ------------------------------------------------------------------------
+// Tiled multi-operation pass.
+__device__ __forceinline__ double2 cmul(double2 p, double2 q) {
+	return make_double2(fma(p.x, q.x, -p.y * q.y), fma(p.x, q.y, p.y * q.x));
+}
+
+// e^{2 pi i F / 2^64} from five 256-entry tables of the top 40 bits of F.
+__device__ __forceinline__ double2 phase_of(u64 F, const double2 *__restrict__ tab) {
+	double2 p = tab[(F >> 56) & 255u];
+	p = cmul(p, tab[256 + ((F >> 48) & 255u)]);
+	p = cmul(p, tab[512 + ((F >> 40) & 255u)]);
+	p = cmul(p, tab[768 + ((F >> 32) & 255u)]);
+	p = cmul(p, tab[1024 + ((F >> 24) & 255u)]);
+	return p;
+}
+
+template <typename V> __device__ __forceinline__ V rot(V x, double2 p) {
+	const double xr = x.x, xi = x.y;
+	return mk<V>(fma(xr, p.x, -xi * p.y), fma(xr, p.y, xi * p.x));
+}
+
+template <typename R, int K>
+__device__ __forceinline__ void tile_uk(typename V2T<R>::t *tile, u64 nloc, const TileOp &op, const double *__restrict__ mats) {
+	constexpr int D = 1 << K;
+	unsigned sorted[K];
+	for(int i = 0; i < K; ++i) sorted[i] = op.jl[i];
+	for(int i = 0; i < K; ++i)
+		for(int j = i + 1; j < K; ++j)
+			if(sorted[j] < sorted[i]) {
+				const unsigned t = sorted[i];
+				sorted[i] = sorted[j];
+				sorted[j] = t;
+			}
+	const double *g = mats + op.mat;
+	for(u64 b = threadIdx.x; b < (nloc >> K); b += blockDim.x) {
+		u64 base = b;
+		for(int i = 0; i < K; ++i) base = ins0(base, sorted[i]);
+		R xr[D], xi[D];
+		u64 idx[D];
+		bool nz = false;
+#pragma unroll
+		for(int r = 0; r < D; ++r) {
+			u64 off = 0;
+#pragma unroll
+			for(int i = 0; i < K; ++i)
+				if((r >> (K - 1 - i)) & 1) off |= 1ull << op.jl[i];
+			idx[r] = base | off;
+			const auto v = tile[idx[r]];
+			xr[r] = v.x;
+			xi[r] = v.y;
+			nz = nz || (v.x != 0 || v.y != 0);
+		}
+		if(!nz) continue;
+#pragma unroll
+		for(int r = 0; r < D; ++r) {
+			R nr = 0, ni = 0;
+#pragma unroll
+			for(int c = 0; c < D; ++c) {
+				const R gr = (R)g[2 * (r * D + c)], gi = (R)g[2 * (r * D + c) + 1];
+				nr += gr * xr[c] - gi * xi[c];
+				ni += gr * xi[c] + gi * xr[c];
+			}
+			typename V2T<R>::t o;
+			o.x = nr;
+			o.y = ni;
+			tile[idx[r]] = o;
+		}
+	}
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_tile(typename V2T<R>::t *__restrict__ a, TilePass P, const TileOp *__restrict__ ops,
+                                              const double *__restrict__ mats, const TwiddleTerm *__restrict__ terms,
+                                              const double2 *__restrict__ ptab) {
+	using V = typename V2T<R>::t;
+	extern __shared__ char smem[];
+	V *tile = (V *)smem;
+	u64 *hitab = (u64 *)(smem + (sizeof(V) << P.T));
+	const u64 nloc = 1ull << P.T;
+	const unsigned C = P.C;
+	const u64 lowmask = (1ull << C) - 1ull;
+	for(unsigned j = threadIdx.x; j < (1u << P.nh); j += blockDim.x) {
+		u64 v = 0;
+		for(unsigned b = 0; b < P.nh; ++b)
+			if((j >> b) & 1u) v |= 1ull << P.hbits[b];
+		hitab[j] = v;
+	}
+	const R hc = (R)(1.0 / sqrt(2.0));
+	for(u64 tid = blockIdx.x; tid < P.ntiles; tid += gridDim.x) {
+		u64 base = tid << C;
+		for(unsigned b = 0; b < P.nh; ++b) base = ins0(base, P.hbits[b]);
+		__syncthreads();
+		for(u64 l = threadIdx.x; l < nloc; l += blockDim.x)
+			tile[l] = nt_load(&a[base | (l & lowmask) | hitab[l >> C]]);
+		__syncthreads();
+		for(unsigned o = 0; o < P.nops; ++o) {
+			const TileOp op = ops[P.op0 + o];
+			if((base & op.gcond) != op.gcond) continue;  // uniform across the block
+			switch(op.kind) {
+				case TOP_H: {
+					const u64 m = 1ull << op.j0;
+					for(u64 q = threadIdx.x; q < (nloc >> 1); q += blockDim.x) {
+						const u64 l0 = ins0(q, op.j0), l1 = l0 | m;
+						const V x = tile[l0], y = tile[l1];
+						if(x.x == 0 && x.y == 0 && y.x == 0 && y.y == 0) continue;
+						V o0, o1;
+						o0.x = (x.x + y.x) * hc;
+						o0.y = (x.y + y.y) * hc;
+						o1.x = (x.x - y.x) * hc;
+						o1.y = (x.y - y.y) * hc;
+						tile[l0] = o0;
+						tile[l1] = o1;
+					}
+					break;
+				}
+				case TOP_X: {
+					const u64 m = 1ull << op.j0;
+					for(u64 q = threadIdx.x; q < (nloc >> 1); q += blockDim.x) {
+						const u64 l0 = ins0(q, op.j0), l1 = l0 | m;
+						if((l0 & op.lcond) != op.lcond) continue;
+						const V x = tile[l0], y = tile[l1];
+						if(x.x == y.x && x.y == y.y) continue;
+						tile[l0] = y;
+						tile[l1] = x;
+					}
+					break;
+				}
+				case TOP_U1: {
+					const u64 m = 1ull << op.j0;
+					const double *g = mats + op.mat;
+					const R g00r = (R)g[0], g00i = (R)g[1], g01r = (R)g[2], g01i = (R)g[3];
+					const R g10r = (R)g[4], g10i = (R)g[5], g11r = (R)g[6], g11i = (R)g[7];
+					for(u64 q = threadIdx.x; q < (nloc >> 1); q += blockDim.x) {
+						const u64 l0 = ins0(q, op.j0), l1 = l0 | m;
+						const V x = tile[l0], y = tile[l1];
+						if((x.x == 0 && x.y == 0) && (y.x == 0 && y.y == 0)) continue;
+						V o0, o1;
+						o0.x = g00r * x.x - g00i * x.y + g01r * y.x - g01i * y.y;
+						o0.y = g00r * x.y + g00i * x.x + g01r * y.y + g01i * y.x;
+						o1.x = g10r * x.x - g10i * x.y + g11r * y.x - g11i * y.y;
+						o1.y = g10r * x.y + g10i * x.x + g11r * y.y + g11i * y.x;
+						tile[l0] = o0;
+						tile[l1] = o1;
+					}
+					break;
+				}
+				case TOP_UK: {
+					switch(op.k) {
+						case 2: tile_uk<R, 2>(tile, nloc, op, mats); break;
+						case 3: tile_uk<R, 3>(tile, nloc, op, mats); break;
+						case 4: tile_uk<R, 4>(tile, nloc, op, mats); break;
+						case 5: tile_uk<R, 5>(tile, nloc, op, mats); break;
+						default: break;
+					}
+					break;
+				}
+				case TOP_CP: {
+					const R cr = (R)op.cr, ci = (R)op.ci;
+					const unsigned nb = __popcll(op.lcond);
+					for(u64 q = threadIdx.x; q < (nloc >> nb); q += blockDim.x) {
+						u64 l = q;
+						u64 cm = op.lcond;
+						while(cm) {
+							const unsigned b = __ffsll((long long)cm) - 1;
+							l = ins0(l, b) | (1ull << b);
+							cm &= cm - 1;
+						}
+						const V x = tile[l];
+						if(x.x == 0 && x.y == 0) continue;
+						V o;
+						o.x = cr * x.x - ci * x.y;
+						o.y = cr * x.y + ci * x.x;
+						tile[l] = o;
+					}
+					break;
+				}
+				case TOP_LADDER: {
+					const unsigned nb = __popcll(op.lcond);
+					for(u64 q = threadIdx.x; q < (nloc >> nb); q += blockDim.x) {
+						u64 l = q;
+						u64 cm = op.lcond;
+						while(cm) {
+							const unsigned b = __ffsll((long long)cm) - 1;
+							l = ins0(l, b) | (1ull << b);
+							cm &= cm - 1;
+						}
+						const u64 g = base | (l & lowmask) | hitab[l >> C];
+						const u64 F = (g & op.amask) << op.shift;
+						if(F == 0) continue;
+						tile[l] = rot<V>(tile[l], phase_of(F, ptab));
+					}
+					break;
+				}
+				case TOP_TWIDDLE: {
+					const TwiddleTerm *tt = terms + op.twid;
+					for(u64 l = threadIdx.x; l < nloc; l += blockDim.x) {
+						const u64 g = base | (l & lowmask) | hitab[l >> C];
+						u64 F = 0;
+						for(unsigned t = 0; t < op.nterms; ++t)
+							if(g & tt[t].cond) F += (g & tt[t].mask) << tt[t].shift;
+						if(F == 0) continue;
+						tile[l] = rot<V>(tile[l], phase_of(F, ptab));
+					}
+					break;
+				}
+				default: break;
+			}
+			__syncthreads();
+		}
+		for(u64 l = threadIdx.x; l < nloc; l += blockDim.x)
+			nt_store(tile[l], &a[base | (l & lowmask) | hitab[l >> C]]);
+	}
+}
+
+// ------------------------------------------------------------------------
+// Host launch helpers.
+#define TMFQS_LAUNCH(KERNEL, GRID, BLOCK, SHMEM, STREAM, ...) \
+	hipLaunchKernelGGL(KERNEL, dim3(GRID), dim3(BLOCK), SHMEM, STREAM, __VA_ARGS__)
+
+hipError_t launch_hadamard(const Launch &L, void *a, unsigned n, unsigned bit) {
+	const u64 np = (1ull << n) >> 1;
+	const double c = 1.0 / sqrt(2.0);
+	if(L.prec == 0)
+		TMFQS_LAUNCH(k_hadamard<double>, grid_for(np), 256, 0, L.stream, (double2 *)a, np, bit, c);
+	else
+		TMFQS_LAUNCH(k_hadamard<float>, grid_for(np), 256, 0, L.stream, (float2 *)a, np, bit, (float)c);
+	return hipGetLastError();
+}
+
+hipError_t launch_pauli_x(const Launch &L, void *a, unsigned n, unsigned bit, u64 cmask) {
+	const u64 np = (1ull << n) >> 1;
+	if(L.prec == 0)
+		TMFQS_LAUNCH(k_pauli_x<double>, grid_for(np), 256, 0, L.stream, (double2 *)a, np, bit, cmask);
+	else
+		TMFQS_LAUNCH(k_pauli_x<float>, grid_for(np), 256, 0, L.stream, (float2 *)a, np, bit, cmask);
+	return hipGetLastError();
+}
+
+hipError_t launch_cphase(const Launch &L, void *a, unsigned n, unsigned tbit, u64 cmask, double cr, double ci) {
+	unsigned cbit = 0;
+	while(!((cmask >> cbit) & 1ull)) ++cbit;
+	const unsigned lo = tbit < cbit ? tbit : cbit, hi = tbit < cbit ? cbit : tbit;
+	const u64 nq = (1ull << n) >> 2;
+	const u64 setm = (1ull << tbit) | cmask;
+	if(L.prec == 0)
+		TMFQS_LAUNCH(k_cphase<double>, grid_for(nq), 256, 0, L.stream, (double2 *)a, nq, lo, hi, setm, cr, ci);
+	else
+		TMFQS_LAUNCH(k_cphase<float>, grid_for(nq), 256, 0, L.stream, (float2 *)a, nq, lo, hi, setm, (float)cr, (float)ci);
+	return hipGetLastError();
+}
+
+hipError_t launch_u1(const Launch &L, void *a, unsigned n, unsigned bit, const double *m) {
+	const u64 np = (1ull << n) >> 1;
+	if(L.prec == 0)
+		TMFQS_LAUNCH(k_u1<double>, grid_for(np), 256, 0, L.stream, (double2 *)a, np, bit, m[0], m[1], m[2], m[3], m[4], m[5],
+		             m[6], m[7]);
+	else
+		TMFQS_LAUNCH(k_u1<float>, grid_for(np), 256, 0, L.stream, (float2 *)a, np, bit, (float)m[0], (float)m[1],
+		             (float)m[2], (float)m[3], (float)m[4], (float)m[5], (float)m[6], (float)m[7]);
+	return hipGetLastError();
+}
+
+template <typename R>
+static void launch_uk_t(const Launch &L, void *a, unsigned n, unsigned k, const unsigned *b, const double *dmat) {
+	using V = typename V2T<R>::t;
+	const u64 nb = (1ull << n) >> k;
+	const unsigned g = grid_for(nb);
+	switch(k) {
+		case 2: TMFQS_LAUNCH((k_uk<R, 2>), g, 256, 0, L.stream, (V *)a, nb, dmat, b[0], b[1], 0u, 0u, 0u); break;
+		case 3: TMFQS_LAUNCH((k_uk<R, 3>), g, 256, 0, L.stream, (V *)a, nb, dmat, b[0], b[1], b[2], 0u, 0u); break;
+		case 4: TMFQS_LAUNCH((k_uk<R, 4>), g, 256, 0, L.stream, (V *)a, nb, dmat, b[0], b[1], b[2], b[3], 0u); break;
+		case 5: TMFQS_LAUNCH((k_uk<R, 5>), g, 256, 0, L.stream, (V *)a, nb, dmat, b[0], b[1], b[2], b[3], b[4]); break;
+		default: break;
+	}
+}
+
+hipError_t launch_uk(const Launch &L, void *a, unsigned n, unsigned k, const unsigned *bits, const double *dmat) {
+	if(L.prec == 0)
+		launch_uk_t<double>(L, a, n, k, bits, dmat);
+	else
+		launch_uk_t<float>(L, a, n, k, bits, dmat);
+	return hipGetLastError();
+}
+
+hipError_t launch_bitswap(const Launch &L, void *a, unsigned n, unsigned p, unsigned q) {
+	if(p > q) {
+		const unsigned t = p;
+		p = q;
+		q = t;
+	}
+	const u64 nq = (1ull << n) >> 2;
+	if(L.prec == 0)
+		TMFQS_LAUNCH(k_bitswap<double>, grid_for(nq), 256, 0, L.stream, (double2 *)a, nq, p, q);
+	else
+		TMFQS_LAUNCH(k_bitswap<float>, grid_for(nq), 256, 0, L.stream, (float2 *)a, nq, p, q);
+	return hipGetLastError();
+}
+
+hipError_t launch_involution(const Launch &L, void *a, unsigned n, const unsigned *perm, unsigned C) {
+	InvParams P{};
+	P.n = n;
+	P.C = C;
+	bool inS[64] = {false};
+	for(unsigned b = 0; b < C; ++b) {
+		inS[b] = true;
+		inS[perm[b]] = true;
+	}
+	unsigned local[64];
+	unsigned s = 0;
+	for(unsigned b = 0; b < n; ++b)
+		if(inS[b]) local[s++] = b;  // ascending; the first C are 0..C-1
+	P.s = s;
+	P.nh = s - C;
+	for(unsigned i = 0; i < P.nh; ++i) P.hbits[i] = local[C + i];
+	for(unsigned b = 0; b < n; ++b) P.perm[b] = perm[b];
+	for(unsigned i = 0; i < s; ++i) {
+		const unsigned img = perm[local[i]];
+		unsigned j = 0;
+		while(local[j] != img) ++j;
+		P.lperm[i] = j;
+	}
+	P.npairs_tiles = 1ull << (n - s);
+	const size_t esz = L.prec == 0 ? 16 : 8;
+	const size_t shmem = 2 * esz * (1ull << s);
+	const unsigned grid = (unsigned)(P.npairs_tiles < 16384 ? P.npairs_tiles : 16384);
+	if(L.prec == 0)
+		TMFQS_LAUNCH(k_involution<double>, grid, 256, shmem, L.stream, (double2 *)a, P);
+	else
+		TMFQS_LAUNCH(k_involution<float>, grid, 256, shmem, L.stream, (float2 *)a, P);
+	return hipGetLastError();
+}
+
+hipError_t launch_ew_chain(const Launch &L, void *a, unsigned n, const EwStep *dsteps, unsigned nsteps) {
+	const u64 N = 1ull << n;
+	if(L.prec == 0)
+		TMFQS_LAUNCH(k_ew_chain<double>, grid_for(N, 256, 16384), 256, 0, L.stream, (double2 *)a, N, dsteps, nsteps);
+	else
+		TMFQS_LAUNCH(k_ew_chain<float>, grid_for(N, 256, 16384), 256, 0, L.stream, (float2 *)a, N, dsteps, nsteps);
+	return hipGetLastError();
+}
+
+hipError_t launch_scatter(const Launch &L, void *a, const Scatter *dsc, unsigned count) {
+	if(count == 0) return hipSuccess;
+	const unsigned g = (count + 63) / 64;
+	if(L.prec == 0)
+		TMFQS_LAUNCH(k_scatter<double>, g, 64, 0, L.stream, (double2 *)a, dsc, count);
+	else
+		TMFQS_LAUNCH(k_scatter<float>, g, 64, 0, L.stream, (float2 *)a, dsc, count);
+	return hipGetLastError();
+}
+
+hipError_t launch_sum(const Launch &L, const void *a, unsigned n, double *dpart, double *dout) {
+	const u64 N = 1ull << n;
+	const unsigned g = grid_for(N, 256, kReduceBlocks);
+	if(L.prec == 0)
+		TMFQS_LAUNCH(k_sum<double>, g, 256, 0, L.stream, (const double2 *)a, N, dpart);
+	else
+		TMFQS_LAUNCH(k_sum<float>, g, 256, 0, L.stream, (const float2 *)a, N, dpart);
+	TMFQS_LAUNCH(k_finish, 1, 64, 0, L.stream, dpart, g, 2u, dout);
+	return hipGetLastError();
+}
+
+hipError_t launch_norm(const Launch &L, const void *a, unsigned n, const double *affine4, double *dpart, double *dout) {
+	const u64 N = 1ull << n;
+	const unsigned g = grid_for(N, 256, kReduceBlocks);
+	if(L.prec == 0)
+		TMFQS_LAUNCH(k_norm<double>, g, 256, 0, L.stream, (const double2 *)a, N, affine4, dpart);
+	else
+		TMFQS_LAUNCH(k_norm<float>, g, 256, 0, L.stream, (const float2 *)a, N, affine4, dpart);
+	TMFQS_LAUNCH(k_finish, 1, 64, 0, L.stream, dpart, g, 1u, dout);
+	return hipGetLastError();
+}
+
+static void chunk_geometry(unsigned n, u64 &chunk, u64 &nchunks) {
+	const unsigned cl = n < kSampleChunkLog ? n : kSampleChunkLog;
+	chunk = 1ull << cl;
+	nchunks = 1ull << (n - cl);
+}
+
+hipError_t launch_chunk_prob(const Launch &L, const void *a, unsigned n, const double *affine4, double *dsums) {
+	u64 chunk, nch;
+	chunk_geometry(n, chunk, nch);
+	const unsigned g = (unsigned)(nch < 16384 ? nch : 16384);
+	if(L.prec == 0)
+		TMFQS_LAUNCH(k_chunk_prob<double>, g, 256, 0, L.stream, (const double2 *)a, 1ull << n, chunk, nch, affine4, dsums);
+	else
+		TMFQS_LAUNCH(k_chunk_prob<float>, g, 256, 0, L.stream, (const float2 *)a, 1ull << n, chunk, nch, affine4, dsums);
+	return hipGetLastError();
+}
+
+hipError_t launch_chunk_ticks(const Launch &L, const void *a, unsigned n, const double *affine4, const int32_t *dulog,
+                              long long *dticks, int32_t *dties) {
+	u64 chunk, nch;
+	chunk_geometry(n, chunk, nch);
+	const unsigned g = (unsigned)(nch < 16384 ? nch : 16384);
+	if(L.prec == 0)
+		TMFQS_LAUNCH(k_chunk_ticks<double>, g, 256, 0, L.stream, (const double2 *)a, chunk, nch, affine4, dulog, dticks, dties);
+	else
+		TMFQS_LAUNCH(k_chunk_ticks<float>, g, 256, 0, L.stream, (const float2 *)a, chunk, nch, affine4, dulog, dticks, dties);
+	return hipGetLastError();
+}
+
+hipError_t launch_tile_pass(const Launch &L, void *a, const TilePass &P, const TileOp *dops, const double *dmats,
+                            const TwiddleTerm *dterms, const double *dphase) {
+	const size_t esz = L.prec == 0 ? 16 : 8;
+	const size_t shmem = (esz << P.T) + (8ull << P.nh);
+	const unsigned grid = (unsigned)(P.ntiles < 65536 ? P.ntiles : 65536);
+	if(L.prec == 0) {
+		static bool attr = false;
+		if(!attr) {
+			(void)hipFuncSetAttribute((const void *)k_tile<double>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+			attr = true;
+		}
+		TMFQS_LAUNCH(k_tile<double>, grid, 256, shmem, L.stream, (double2 *)a, P, dops, dmats, dterms, (const double2 *)dphase);
+	} else {
+		static bool attr = false;
+		if(!attr) {
+			(void)hipFuncSetAttribute((const void *)k_tile<float>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+			attr = true;
+		}
+		TMFQS_LAUNCH(k_tile<float>, grid, 256, shmem, L.stream, (float2 *)a, P, dops, dmats, dterms, (const double2 *)dphase);
+	}
+	return hipGetLastError();
+}
+
+}  // namespace tmfqs_hip
