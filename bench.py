@@ -1,0 +1,282 @@
+#!/usr/bin/env python3
+"""Benchmark of the dense state-vector hot path on one MI355X.
+
+Workload (the headline step): QFT on 30 qubits in complex128 -- makeQft(30),
+480 circuit operations -- applied through sim::Simulator's timed region
+(beginOperationBatch .. endOperationBatch, which waits for the device) to a
+register already resident in HBM that holds QuantumState::randomPhase(30, 7).
+One step = one QFT-30 circuit.  value = QFT-30 circuits per second.
+
+Also measured in the same run and reported beside the headline:
+  * Grover-30, complex128, materialized diffusion, marked {2^29 - 1}: the full
+    25735-iteration circuit through sim::Simulator::run, its marked-state
+    probability and the Mt19937(123456) measurement;
+  * complex64: QFT-30 time and fidelity against the complex128 result
+    (device comparison);
+  * the streaming-copy ceiling of this GPU;
+  * roofline of the dominant kernel from live HIP-event timing on the engine's
+    stream;
+  * cpu_baseline: the reference's own dense backend (compiled from its
+    sources into oracle/_ref) timed on this host on a bounded sample of
+    QFT-30 operations on a 2^30 state, extrapolated by operation type.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--qubits 30]
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "QFT-30/Grover-30 wall s + HBM GB/s; adjoint grads/s vs plain FD at ≤1e-9 err"
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def read_traffic(kernel, workload):
+    """HBM bytes per launch from the committed PMC summary, if present."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        d = json.load(open(path))
+        return d[workload][kernel]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def time_qft(S, n, precision, steps, warmup, seed=7):
+    reg = S.Register.random_phase(n, seed, precision=precision)
+    ops, cnt = S.qft_ops(n)
+    hs = reg.hip_state()
+    for _ in range(warmup):
+        reg.run(ops, cnt)
+    hs.synchronize()
+    hs.profile(True)
+    hs.profile_reset()
+    exec_s = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        exec_s.append(reg.run(ops, cnt))
+    hs.synchronize()
+    wall = time.perf_counter() - t0
+    prof = hs.profile_read()
+    hs.profile(False)
+    return reg, wall, exec_s, prof, cnt
+
+
+def cpu_baseline(n):
+    """The reference's dense backend on this host: time each QFT-30 operation
+    type on a 2^n state and extrapolate to the full op mix."""
+    import oracle as O
+
+    kind = "reference" if O.ref_available() else "port"
+    timer = O.ref_time_ops if kind == "reference" else None
+    if timer is None:
+        O.lib().orc_time_ops.argtypes = [ctypes.c_uint, ctypes.POINTER(O.Op), ctypes.c_size_t,
+                                         ctypes.POINTER(ctypes.c_double)]
+
+        def timer(nq, seq):
+            ops, c = O.make_ops(seq)
+            s = ctypes.c_double()
+            rc = O.lib().orc_time_ops(nq, ops, c, ctypes.byref(s))
+            if rc:
+                raise RuntimeError(f"orc_time_ops failed {rc}")
+            return s.value
+    import math
+    samples = {
+        "h": [(O.OP_H, 0)],
+        "cphase": [(O.OP_CPHASE, 1, 0, math.pi / 2), (O.OP_CPHASE, 29 % n, 3, math.pi / 2 ** 26)],
+        "swap": [(O.OP_SWAP, 0, n - 1)],
+    }
+    per = {}
+    total_s = 0.0
+    for name, seq in samples.items():
+        t = timer(n, seq)
+        total_s += t
+        per[name] = t / len(seq)
+    nh, ncp, nsw = n, n * (n - 1) // 2, n // 2
+    projected = nh * per["h"] + ncp * per["cphase"] + nsw * per["swap"]
+    return {
+        "value": 1.0 / projected,
+        "unit": f"QFT-{n} circuits/s",
+        "cores": 1,
+        "kind": kind,
+        "projected_qft_seconds": projected,
+        "per_op_seconds": per,
+        "sample": (f"1 H, 2 controlled-phase and 1 swap (3 CX) of makeQft({n}) timed on a 2^{n}-state "
+                   f"complex128 register with uniform amplitudes ({total_s:.1f} s of CPU work), extrapolated "
+                   f"to the circuit's {nh} H + {ncp} controlled-phase + {nsw} swap operations; single thread, "
+                   f"{os.cpu_count()} host CPUs, {cpu_model()}"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-grover", action="store_true")
+    ap.add_argument("--skip-c64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); the state-vector engine runs on "
+              f"a single MI355X in this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import tmfqsfullstate_amd as T
+    from tmfqsfullstate_amd import sim as S
+
+    n = args.qubits
+    N = 1 << n
+    result = {}
+
+    # ---- headline: QFT-n complex128
+    log(f"QFT-{n} complex128: generating randomPhase({n}, 7) and loading it")
+    reg, wall, exec_s, prof, nops = time_qft(S, n, T.C128, args.steps, args.warmup)
+    ms_per_step = 1000.0 * wall / args.steps
+    hs = reg.hip_state()
+    copy_ms = hs.copy_probe(3)
+    copy_gbps = 2.0 * 16 * N / (copy_ms * 1e-3) / 1e9
+    tp = reg.total_probability()
+    step_bytes = sum(v["bytes"] for v in prof.values()) / args.steps
+    launches = {k: v["launches"] / args.steps for k, v in prof.items()}
+    dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
+    dname, dstat = dom
+    avg_ms = dstat["ms"] / dstat["launches"]
+    bytes_per_launch = dstat["bytes"] / dstat["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    log(f"QFT-{n}: {ms_per_step:.2f} ms/step, kernels {launches}, copy ceiling {copy_gbps:.0f} GB/s")
+    qft = {
+        "wall_s": ms_per_step / 1000.0,
+        "execution_seconds": exec_s,
+        "ops": nops,
+        "kernel_launches_per_circuit": launches,
+        "hbm_bytes_per_circuit": step_bytes,
+        "hbm_GBps": step_bytes / (ms_per_step * 1e-3) / 1e9,
+        "hbm_fraction": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+        "gate_equivalent_GBps_not_roofline": (n + n * (n - 1) // 2 + 3 * (n // 2)) * 2.0 * 16 * N
+        / (ms_per_step * 1e-3) / 1e9,
+        "total_probability_after": tp,
+        "kernels": prof,
+    }
+    roofline = {
+        "kernel": dname,
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBPS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBPS,
+        "traffic": read_traffic(dname, f"qft{n}_c128"),
+        "algorithmic_bytes_per_launch": bytes_per_launch,
+        "avg_launch_ms": avg_ms,
+        "copy_kernel_GBps": copy_gbps,
+    }
+
+    # ---- complex64 QFT and fidelity against complex128
+    if not args.skip_c64:
+        log(f"QFT-{n} complex64")
+        reg64, wall64, exec64, prof64, _ = time_qft(S, n, T.C64, args.steps, args.warmup)
+        ms64 = 1000.0 * wall64 / args.steps
+        # same number of QFTs applied to both (warmup + steps): compare states on the device
+        cmp = hs.compare(reg64.hip_state())
+        result["complex64"] = {
+            "qft_wall_s": ms64 / 1000.0,
+            "speedup_vs_complex128": ms_per_step / ms64,
+            "fidelity_vs_complex128": cmp["fidelity"],
+            "fidelity_error": 1.0 - cmp["fidelity"],
+            "max_abs_error_vs_complex128": cmp["max_abs_amplitude_error"],
+            "kernels": prof64,
+        }
+        del reg64
+    del reg, hs
+
+    # ---- Grover-n complex128, full circuit
+    if not args.skip_grover:
+        marked = (1 << (n - 1)) - 1
+        ops, cnt = S.grover_ops(n, [marked], True)
+        log(f"Grover-{n} complex128: {cnt} ops")
+        greg = S.Register.basis(n, 0, precision=T.C128)  # QuantumState::basis(n) materialised
+        gh = greg.hip_state()
+        gh.profile(True)
+        gh.profile_reset()
+        gsecs = greg.run(ops, cnt)  # Simulator::run's timed region
+        gprof = gh.profile_read()
+        gh.profile(False)
+        a = greg.amplitude(marked)
+        meas = greg.measure_seed(123456)
+        iters = (cnt - n) // 2
+        sweeps = sum(v["launches"] for k, v in gprof.items() if k in ("k_ew_chain", "k_tile", "k_hadamard"))
+        gbytes = sum(v["bytes"] for v in gprof.values())
+        gms = sum(v["ms"] for v in gprof.values())
+        result["grover"] = {
+            "qubits": n,
+            "marked": marked,
+            "iterations": iters,
+            "wall_s": gsecs,
+            "iterations_per_s": iters / gsecs,
+            "p_marked": abs(a) ** 2,
+            "measured_seed_123456": meas,
+            "full_state_sweeps": sweeps,
+            "hbm_bytes": gbytes,
+            "hbm_GBps_over_kernel_time": gbytes / (gms * 1e-3) / 1e9 if gms > 0 else None,
+            "hbm_GBps_over_wall": gbytes / gsecs / 1e9,
+            "one_sweep_per_iteration_equivalent_GBps_not_hbm": iters * 2.0 * 16 * N / gsecs / 1e9,
+            "kernels": gprof,
+        }
+        log(f"Grover-{n}: {gsecs:.3f} s, P={abs(a) ** 2:.15f}, measured {meas}")
+        del greg, gh
+
+    cpu = None
+    if not args.skip_cpu:
+        log("CPU baseline (reference dense backend, one thread)")
+        cpu = cpu_baseline(n)
+        log(f"CPU: projected QFT-{n} {cpu['projected_qft_seconds']:.1f} s")
+
+    value = 1000.0 / ms_per_step
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": f"QFT-{n} circuits/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": f"synthetic: QuantumState::randomPhase({n}, 7) (mt19937 seed 7)",
+        "config": {"workload": f"qft{n}_c128", "qubits": n, "circuit": f"makeQft({n})", "ops": nops,
+                   "precision": "complex128", "strategy": "hip"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "qft": qft,
+    }
+    line.update(result)
+    if cpu is not None:
+        line["speedup_vs_cpu_reference"] = value / cpu["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

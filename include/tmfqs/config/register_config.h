@@ -1,0 +1,107 @@
+// Register / backend configuration (reference: include/tmfqs/config/register_config.h:9-109).
+// The device backend adds StorageStrategyKind::Hip (appended after Auto so the
+// existing ordinals keep their values) and HipConfig.
+#ifndef TMFQS_CONFIG_REGISTER_CONFIG_H
+#define TMFQS_CONFIG_REGISTER_CONFIG_H
+
+#include <cstddef>
+
+namespace tmfqs {
+
+enum class StorageStrategyKind {
+	Dense,
+	Blosc,
+	Zfp,
+	Auto,
+	/** MI355X (gfx950) device state vector; selected only explicitly. */
+	Hip
+};
+
+enum class StorageWorkloadHint { Generic, Grover, Qft };
+
+enum class ZfpCompressionMode { FixedRate, FixedPrecision, FixedAccuracy };
+
+struct BloscConfig {
+	size_t chunkStates = 32768;
+	int clevel = 1;
+	int nthreads = 4;
+	int compcode = 1;
+	bool useShuffle = true;
+	size_t gateCacheSlots = 8;
+};
+
+struct BloscTuningOverrides {
+	bool chunkStates = false;
+	bool clevel = false;
+	bool nthreads = false;
+	bool gateCacheSlots = false;
+};
+
+struct ZfpConfig {
+	ZfpCompressionMode mode = ZfpCompressionMode::FixedPrecision;
+	double rate = 32.0;
+	unsigned int precision = 40;
+	double accuracy = 1e-8;
+	size_t chunkStates = 32768;
+	int nthreads = 4;
+	size_t gateCacheSlots = 8;
+};
+
+struct ZfpTuningOverrides {
+	bool mode = false;
+	bool rate = false;
+	bool precision = false;
+	bool accuracy = false;
+	bool chunkStates = false;
+	bool nthreads = false;
+	bool gateCacheSlots = false;
+};
+
+/** Amplitude storage precision of the device backend. */
+enum class HipPrecision {
+	/** FP64 amplitudes: matches the dense backend (bit-exact except fused QFT phase ladders). */
+	Complex128,
+	/** FP32 amplitudes; FP64 reductions, Grover means and phase factors. */
+	Complex64
+};
+
+/** Device backend settings (0 means "choose automatically"). */
+struct HipConfig {
+	HipPrecision precision = HipPrecision::Complex128;
+	int device = 0;
+	/** Queue operations and fuse them into tiled passes. */
+	bool fusion = true;
+	/** Qubits per LDS tile (0: 12 for complex128, 13 for complex64). */
+	unsigned int tileQubits = 0;
+	/** Low qubits always inside a tile (0: automatic). */
+	unsigned int coalesceQubits = 0;
+	/** Most Grover/affine steps folded into one sweep (0: automatic). */
+	unsigned int groverBlock = 0;
+	/** Apply every controlled phase as its own product (bit-identical to the dense backend). */
+	bool exactPhases = false;
+};
+
+struct HipTuningOverrides {
+	bool precision = false;
+	bool tileQubits = false;
+	bool coalesceQubits = false;
+	bool groverBlock = false;
+};
+
+struct RegisterConfig {
+	/** Hip by default: this library ships only the device backend, so the
+	 *  reference's default-constructed registers and simulators run on the GPU. */
+	StorageStrategyKind strategy = StorageStrategyKind::Hip;
+	size_t autoThresholdBytes = 8u * 1024u * 1024u;
+	StorageWorkloadHint workloadHint = StorageWorkloadHint::Generic;
+	BloscConfig blosc;
+	BloscTuningOverrides bloscOverrides;
+	ZfpConfig zfp;
+	ZfpTuningOverrides zfpOverrides;
+	HipConfig hip;
+	HipTuningOverrides hipOverrides;
+};
+
+}  // namespace tmfqs
+
+#endif

@@ -173,6 +173,25 @@ int tmfqs_hip_end_batch(tmfqs_hip_state *h);
 /* Flushes the queue and waits for the device. */
 int tmfqs_hip_synchronize(tmfqs_hip_state *h);
 
+/* ---- comparison ------------------------------------------------------- */
+/* experiment::StateComparison (src/tmfqs/experiment/comparison.cpp:12-63 in
+ * the reference) computed on the device for two states of equal qubit count
+ * on one device (either precision), plus the overlap <ref|cand> and the
+ * fidelity |<ref|cand>|^2 / (<ref|ref> <cand|cand>). */
+typedef struct tmfqs_hip_comparison {
+	int bitwise_equal;
+	double max_abs_amplitude_error;
+	double max_abs_component_error;
+	double rmse_amplitude;
+	double rel_l2;
+	double max_abs_probability_error;
+	double total_probability_diff;
+	uint64_t worst_state;
+	double overlap_re, overlap_im;
+	double fidelity;
+} tmfqs_hip_comparison;
+int tmfqs_hip_compare(tmfqs_hip_state *ref, tmfqs_hip_state *cand, tmfqs_hip_comparison *out);
+
 /* ---- observability ---------------------------------------------------- */
 /* Per-kernel device timing with HIP events on the handle's stream (the device
  * analogue of ObservabilityConfig.traceOperations, simulator.h:16-19). */
@@ -183,6 +202,12 @@ int tmfqs_hip_profile_reset(tmfqs_hip_state *h);
  * *count is the capacity; on return the number of entries. */
 int tmfqs_hip_profile_read(tmfqs_hip_state *h, char (*names)[64], uint64_t *launches, double *milliseconds,
                            double *bytes, size_t *count);
+/* Streaming copy of the state into a scratch buffer of the same size (reads
+ * and writes 2^n amplitudes once each), repeated reps times; *ms is the mean
+ * time of one copy.  A plain copy kernel of known size: the practical
+ * bandwidth ceiling and the calibration for hardware byte counters.  No
+ * counterpart in the reference. */
+int tmfqs_hip_copy_probe(tmfqs_hip_state *h, int reps, double *ms);
 /* Raw stream handle (hipStream_t) for callers that time with their own events. */
 int tmfqs_hip_stream(tmfqs_hip_state *h, void **stream);
 

@@ -1,0 +1,80 @@
+/*
+ * tmfqs_sim.h -- C entry points into the C++ API (libtmfqs.so), for callers
+ * that bind C (Python ctypes, the benchmark).  Each function drives the same
+ * objects a C++ caller would: QuantumState -> QuantumRegister with
+ * StorageStrategyKind::Hip -> sim::Simulator -> circuit::Circuit, so results
+ * carry the reference's register semantics (cached amplitude sum, lazy
+ * affine overlay, swap, measurement).  The reference exposes no FFI; these
+ * mirror sim::Simulator::run (src/tmfqs/sim/simulator.cpp:54-96) and the
+ * QuantumState / QuantumRegister queries a binding would need.
+ *
+ * Status codes are those of tmfqs_hip.h; tmfqs_sim_last_error() describes the
+ * calling thread's last failure.  No exception crosses this boundary.
+ */
+#ifndef TMFQS_SIM_H
+#define TMFQS_SIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "tmfqs_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tmfqs_reg tmfqs_reg;
+
+const char *tmfqs_sim_last_error(void);
+
+/* QuantumRegister(n, state, cfg{strategy = Hip}) (quantum_register.cpp:85-89). */
+int tmfqs_reg_create_basis(unsigned n, uint64_t state, const tmfqs_hip_config *cfg, tmfqs_reg **out);
+/* QuantumRegister(n, amplitudes, cfg) (quantum_register.cpp:99-103); 2 * 2^n doubles. */
+int tmfqs_reg_create_amplitudes(unsigned n, const double *amps, const tmfqs_hip_config *cfg, tmfqs_reg **out);
+/* QuantumState::randomPhase(n, seed) materialised on the device (quantum_state.cpp:264-277). */
+int tmfqs_reg_create_random_phase(unsigned n, unsigned seed, const tmfqs_hip_config *cfg, tmfqs_reg **out);
+/* QuantumRegister copy constructor (deep copy). */
+int tmfqs_reg_clone(tmfqs_reg *r, tmfqs_reg **out);
+int tmfqs_reg_free(tmfqs_reg *r);
+
+/* The timed region of Simulator::run (simulator.cpp:66-93) on this register:
+ * ops (see tmfqs_hip_op; MATRIX ops take their matrix from mat_pool at
+ * op.state doubles and their k targets from mat_qubits at op.q0) inside one
+ * operation batch.  seconds receives RunReport.executionSeconds; trace waits
+ * for every operation (ObservabilityConfig.traceOperations). */
+int tmfqs_reg_run(tmfqs_reg *r, const tmfqs_hip_op *ops, size_t count, const double *mat_pool, const unsigned *mat_qubits,
+                  int trace, double *seconds);
+
+/* Full Simulator::run from QuantumState::basis(n, basis) (init_amps == NULL)
+ * or QuantumState::fromAmplitudes(n, init_amps, AllowUnnormalized); the result
+ * register is returned in *result. */
+int tmfqs_sim_run(unsigned n, uint64_t basis, const double *init_amps, const tmfqs_hip_config *cfg, const tmfqs_hip_op *ops,
+                  size_t count, const double *mat_pool, const unsigned *mat_qubits, double *seconds, tmfqs_reg **result);
+
+/* QuantumRegister queries. */
+int tmfqs_reg_qubit_count(tmfqs_reg *r, unsigned *out);
+int tmfqs_reg_export(tmfqs_reg *r, double *out); /* amplitudes(): 2 * 2^n doubles */
+int tmfqs_reg_amplitude(tmfqs_reg *r, uint64_t state, double *re, double *im);
+int tmfqs_reg_total_probability(tmfqs_reg *r, double *out);
+/* measure(Mt19937RandomSource(seed)). */
+int tmfqs_reg_measure_seed(tmfqs_reg *r, uint32_t seed, uint64_t *out);
+/* measure() with a source that always returns rnd (no range check of its own). */
+int tmfqs_reg_measure_fixed(tmfqs_reg *r, double rnd, uint64_t *out);
+/* The device handle behind the register (profiling, streams). */
+int tmfqs_reg_hip_handle(tmfqs_reg *r, tmfqs_hip_state **out);
+
+/* QuantumState::randomPhase(n, seed).amplitudes() into out (2 * 2^n doubles);
+ * host only (quantum_state.cpp:264-277). */
+int tmfqs_state_random_phase(unsigned n, unsigned seed, double *out);
+
+/* circuit::makeQft / makeGrover as op records (circuit.cpp:87-147).  With out
+ * == NULL only *count is written. */
+int tmfqs_circuit_qft(unsigned n, tmfqs_hip_op *out, size_t cap, size_t *count);
+int tmfqs_circuit_grover(unsigned n, const uint64_t *marked, size_t m, int materialized, tmfqs_hip_op *out, size_t cap,
+                         size_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TMFQS_SIM_H */

@@ -7,6 +7,7 @@
  * -ffp-contract=off so that no multiply-add is fused (the reference is built
  * for baseline x86-64, which has no FMA).
  */
+#define _POSIX_C_SOURCE 199309L
 #include "dense_oracle.h"
 
 #include <math.h>
@@ -756,4 +757,26 @@ long long orc_grover_ops(unsigned n, const uint64_t *marked, size_t m, int mater
 	}
 	free(sel);
 	return total;
+}
+
+/* ------------------------------------------------------------------------- */
+/* CPU-baseline timing of the restatement: ops applied to an n-qubit register
+ * whose amplitudes all equal 2^(-n/2) (so no zero-skip fires); wall seconds of
+ * the op loop only.  Used by the benchmark when the reference build is absent. */
+#include <time.h>
+int orc_time_ops(unsigned n, const orc_op *ops, size_t count, double *seconds) {
+	orc_register *r = orc_reg_create(n);
+	if(!r) return ORC_RUNTIME_ERROR;
+	const double a = 1.0 / sqrt((double)r->N);
+	for(uint64_t s = 0; s < r->N; ++s) {
+		r->amp[2 * s] = a;
+		r->amp[2 * s + 1] = 0.0;
+	}
+	struct timespec t0, t1;
+	clock_gettime(CLOCK_MONOTONIC, &t0);
+	const int rc = orc_reg_run(r, ops, count, NULL, NULL);
+	clock_gettime(CLOCK_MONOTONIC, &t1);
+	*seconds = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+	orc_reg_free(r);
+	return rc;
 }

@@ -136,6 +136,8 @@ void orc_qft_ops(unsigned n, orc_op *out);
 unsigned orc_grover_iterations(uint64_t state_count, size_t marked_count);
 /* Ops of makeGrover: returns count, or a negative error; out may be NULL to query. */
 long long orc_grover_ops(unsigned n, const uint64_t *marked, size_t m, int materialized, orc_op *out, size_t cap);
+/* Wall seconds of ops on a uniform-amplitude n-qubit register (CPU-baseline timing). */
+int orc_time_ops(unsigned n, const orc_op *ops, size_t count, double *seconds);
 
 #ifdef __cplusplus
 }

@@ -7,6 +7,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 import tmfqsfullstate_amd as T
@@ -63,3 +64,25 @@ def test_abi_version_and_errors_without_device():
     bad = T.default_config(precision=7)
     rc = lib.tmfqs_hip_create(ctypes.byref(bad), ctypes.byref(h))
     assert rc in (T.ERR_INVALID_ARGUMENT, T.ERR_DEVICE)
+
+
+def test_sim_library_exports_and_host_entry_points():
+    from tmfqsfullstate_amd import sim as S
+    names = declared("tmfqs_sim.h")
+    syms = exported(S.SIM_LIB)
+    assert not [n for n in names if n not in syms]
+    # host-only entry points of the C++ API, pinned against the oracle
+    for n, seed in ((0, 1), (5, 7), (12, 7), (16, 123456)):
+        assert np.array_equal(S.random_phase(n, seed), O.random_phase(n, seed))
+    GOLDEN = np.load(os.path.join(ROOT, "tests", "golden", "golden.npz"))
+    assert np.array_equal(S.random_phase(12, 7), GOLDEN["random_phase_12_7"])
+    for n in (1, 4, 30):
+        a, na = S.qft_ops(n)
+        b, nb = O.qft_ops(n)
+        assert na == nb and all((a[i].kind, a[i].q0, a[i].q1, a[i].theta) == (b[i].kind, b[i].q0, b[i].q1, b[i].theta)
+                                for i in range(na))
+    a, na = S.grover_ops(12, [5, 4095, 5], False)
+    b, nb = O.grover_ops(12, [5, 4095, 5], False)
+    assert na == nb and all((a[i].kind, a[i].state) == (b[i].kind, b[i].state) for i in range(na))
+    with pytest.raises(T.InvalidArgument):
+        S.grover_ops(3, [8], True)

@@ -1,0 +1,136 @@
+"""GPU tests through the public C++ API (QuantumRegister / sim::Simulator with
+the Hip strategy), via include/tmfqs_sim.h and the C++ test drivers.
+
+Tolerances: complex128 amplitudes within 1e-12 of the reference dense
+backend's (the CPU oracle, itself bit-exact with the reference build);
+measured indices equal.
+"""
+
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+import tmfqsfullstate_amd as T
+from tmfqsfullstate_amd import sim as S
+from tests import numpy_ref as R
+
+pytestmark = pytest.mark.gpu
+
+BIN = os.path.join(os.path.dirname(T.__file__), "bin")
+TOL = 1e-12
+
+
+def cplx(x):
+    return x[0::2] + 1j * x[1::2]
+
+
+@pytest.mark.parametrize("exe", ["test_hip_circuit", "test_hip_register"])
+def test_cpp_drivers(exe):
+    p = subprocess.run([os.path.join(BIN, exe)], capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "passed" in p.stdout
+
+
+def test_circuit_builders_match_oracle():
+    for n in (1, 5, 12, 30):
+        a, na = S.qft_ops(n)
+        b, nb = O.qft_ops(n)
+        assert na == nb
+        for i in range(na):
+            assert (a[i].kind, a[i].q0, a[i].q1, a[i].theta) == (b[i].kind, b[i].q0, b[i].q1, b[i].theta)
+    a, na = S.grover_ops(10, [7, 3, 7], True)
+    b, nb = O.grover_ops(10, [7, 3, 7], True)
+    assert na == nb and all(a[i].kind == b[i].kind and a[i].state == b[i].state for i in range(na))
+
+
+@pytest.mark.parametrize("materialized", [True, False])
+def test_simulator_grover(materialized):
+    for n, marked in ((10, [511]), (13, [1, 4096, 8191])):
+        ops, cnt = S.grover_ops(n, marked, materialized)
+        reg, secs = S.simulate(n, ops, cnt)
+        assert secs > 0
+        want = O.run_circuit(n, O.grover_ops(n, marked, materialized))
+        # sums, means, diffusion steps and the overlay reproduce the reference bit for bit
+        assert np.array_equal(reg.export(), want.amplitudes())
+        for seed in (123456, 7, 8):
+            assert reg.measure_seed(seed) == want.measure(O.mt_unit_draws(seed, 1)[0])
+        assert reg.total_probability() == want.total_probability()
+
+
+def test_simulator_qft_random_phase_and_exact_mode():
+    n = 14
+    init = O.random_phase(n, 7)
+    ops, cnt = S.qft_ops(n)
+    want = O.run_circuit(n, O.qft_ops(n), init=init).amplitudes()
+    reg, _ = S.simulate(n, ops, cnt, init=init)
+    assert np.abs(reg.amplitudes() - cplx(want)).max() <= TOL
+    exact, _ = S.simulate(n, ops, cnt, init=init, exact_phases=1)
+    assert np.array_equal(exact.export(), want)
+    rp = S.Register.random_phase(12, 7)
+    assert np.array_equal(rp.export(), O.random_phase(12, 7))
+
+
+def test_register_matrix_gates_and_fixed_measure():
+    n = 8
+    rng = np.random.default_rng(3)
+    init = O.random_phase(n, 5)
+    pool, mq, seq = [], [], []
+    for k in (1, 2, 3, 5):
+        u = R.random_unitary(rng, 1 << k)
+        qs = [int(q) for q in rng.choice(n, size=k, replace=False)]
+        seq.append((T.OP_MATRIX, len(mq), 0, 0.0, sum(len(p) for p in pool), k))
+        pool.append(R.interleave(u))
+        mq += qs
+    pool = np.concatenate(pool)
+    mq = np.array(mq, dtype=np.uint32)
+    reg = S.Register.from_amplitudes(n, init)
+    ops, cnt = S.make_ops(seq)
+    reg.run(ops, cnt, mat_pool=pool, mat_qubits=mq)
+    oops, ocnt = O.make_ops(seq)
+    want = O.run_circuit(n, (oops, ocnt), init=init, mat_pool=pool, mat_qubits=mq)
+    assert np.array_equal(reg.export(), want.amplitudes())
+    tp = want.total_probability()
+    assert reg.measure_fixed(tp / 2) == want.measure(tp / 2)
+
+
+def test_overlay_paths_match_oracle():
+    # lazy Grover leaves the affine overlay active: amplitudes, total
+    # probability and measurement go through the overlay-aware device reads
+    n = 11
+    ops, cnt = S.grover_ops(n, [1000], False)
+    reg, _ = S.simulate(n, ops, cnt)
+    want = O.run_circuit(n, O.grover_ops(n, [1000], False))
+    assert np.abs(reg.amplitudes() - cplx(want.amplitudes())).max() <= TOL
+    assert reg.total_probability() == want.total_probability()
+    for rnd in (0.0001, 0.3, 0.99999):
+        assert reg.measure_fixed(rnd) == want.measure(rnd)
+
+
+def test_device_compare_and_complex64():
+    n = 16
+    init = O.random_phase(n, 1)
+    ops, cnt = S.qft_ops(n)
+    a, _ = S.simulate(n, ops, cnt, init=init)
+    b, _ = S.simulate(n, ops, cnt, init=init, precision=T.C64)
+    c = a.hip_state().compare(b.hip_state())
+    host_a, host_b = a.amplitudes(), b.amplitudes()
+    assert abs(c["max_abs_amplitude_error"] - np.abs(host_a - host_b).max()) < 1e-15
+    fid = abs(np.vdot(host_a, host_b)) ** 2 / (np.vdot(host_a, host_a).real * np.vdot(host_b, host_b).real)
+    assert abs(c["fidelity"] - fid) < 1e-12
+    assert 1 - c["fidelity"] <= 1e-6
+    same = a.hip_state().compare(a.clone().hip_state())
+    assert same["bitwise_equal"] == 1 and same["max_abs_amplitude_error"] == 0.0
+
+
+def test_errors_map_to_exceptions():
+    with pytest.raises(T.OutOfRange):
+        S.simulate(3, [(T.OP_H, 3)])
+    with pytest.raises(T.InvalidArgument):
+        S.simulate(3, [(T.OP_CX, 1, 1)])
+    with pytest.raises(T.InvalidArgument):
+        S.grover_ops(3, [8], True)
+    with pytest.raises(T.InvalidArgument):
+        S.Register.basis(64)

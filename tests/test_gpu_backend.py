@@ -127,12 +127,34 @@ def test_grover_vs_oracle(materialized, mode):
         oreg = O.run_circuit(n, (ops, cnt))
         want = cplx(oreg.amplitudes())
         assert np.abs(got - want).max() <= TOL
+        if materialized:
+            assert np.array_equal(s.export(), oreg.amplitudes())
         for seed in (123456, 1, 2, 3):
             rnd = O.mt_unit_draws(seed, 1)[0]
             assert reg.measure(rnd) == oreg.measure(rnd)
         k = O.grover_iterations(n, len(set(marked)))
         p = sum(abs(got[m]) ** 2 for m in set(marked))
         assert abs(p - R.grover_p_marked(n, len(set(marked)), k)) < 1e-10
+
+
+@pytest.mark.parametrize("n", [0, 1, 5, 12, 15])
+def test_amplitude_sum_bitexact(n):
+    # signed sequential sums: random phases (partial sums cross zero), a
+    # uniform superposition (one binade per chunk), a sparse state
+    states = [rand_state(n, 2)]
+    u = np.zeros(2 << n)
+    u[0::2] = 2.0 ** (-n / 2)
+    states.append(u)
+    sp = np.zeros(2 << n)
+    sp[2 * ((1 << n) - 1)] = -0.75
+    sp[1] = 0.25
+    states.append(sp)
+    for init in states:
+        s = st().load(n, init)
+        re, im = O.ctypes.c_double(), O.ctypes.c_double()
+        O.lib().orc_amplitude_sum(init.ctypes.data_as(O._dp), n, O.ctypes.byref(re), O.ctypes.byref(im))
+        got = s.amplitude_sum()
+        assert got.real == re.value and got.imag == im.value
 
 
 @pytest.mark.parametrize("n", [4, 11, 14, 17])

@@ -54,6 +54,25 @@ class Op(ctypes.Structure):
     ]
 
 
+class Comparison(ctypes.Structure):
+    _fields_ = [
+        ("bitwise_equal", ctypes.c_int),
+        ("max_abs_amplitude_error", ctypes.c_double),
+        ("max_abs_component_error", ctypes.c_double),
+        ("rmse_amplitude", ctypes.c_double),
+        ("rel_l2", ctypes.c_double),
+        ("max_abs_probability_error", ctypes.c_double),
+        ("total_probability_diff", ctypes.c_double),
+        ("worst_state", ctypes.c_uint64),
+        ("overlap_re", ctypes.c_double),
+        ("overlap_im", ctypes.c_double),
+        ("fidelity", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class HipError(RuntimeError):
     """Raised for a non-zero status; ``code`` is the tmfqs_hip_status."""
 
@@ -119,6 +138,8 @@ def lib():
         L.tmfqs_hip_profile_reset.argtypes = [vp]
         L.tmfqs_hip_profile_read.argtypes = [vp, ctypes.c_void_p, _u64p, _dp, _dp, ctypes.POINTER(ctypes.c_size_t)]
         L.tmfqs_hip_stream.argtypes = [vp, ctypes.POINTER(vp)]
+        L.tmfqs_hip_copy_probe.argtypes = [vp, ctypes.c_int, _dp]
+        L.tmfqs_hip_compare.argtypes = [vp, vp, ctypes.POINTER(Comparison)]
         L.tmfqs_hip_qubit_count.argtypes = [vp, ctypes.POINTER(ctypes.c_uint)]
         _lib = L
     return _lib
@@ -324,6 +345,17 @@ class HipState:
             nm = bytes(names[i]).split(b"\0", 1)[0].decode()
             out[nm] = {"launches": launches[i], "ms": ms[i], "bytes": byts[i]}
         return out
+
+    def copy_probe(self, reps=3):
+        ms = ctypes.c_double()
+        check(lib().tmfqs_hip_copy_probe(self._h, reps, ctypes.byref(ms)))
+        return ms.value
+
+    def compare(self, other):
+        """Device comparison with self as the reference state."""
+        c = Comparison()
+        check(lib().tmfqs_hip_compare(self._h, other._h, ctypes.byref(c)))
+        return c.as_dict()
 
     def stream(self):
         s = ctypes.c_void_p()

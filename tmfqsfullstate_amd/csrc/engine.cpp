@@ -129,8 +129,8 @@ struct tmfqs_hip_state {
 	double *d_red = nullptr;    // reduction partials + outputs
 	double *d_phase = nullptr;  // 5 x 256 complex phase tables
 	double *d_sums = nullptr;   // sampling chunk sums
-	long long *d_ticks = nullptr;
-	int32_t *d_ulog = nullptr, *d_ties = nullptr;
+	ChunkTicks *d_ticks = nullptr;
+	int32_t *d_ulog = nullptr;
 	size_t chunk_cap = 0;
 	Arena arena;
 	char *h_stage = nullptr;  // pinned staging for bulk copies
@@ -279,11 +279,9 @@ void ensure_chunk_buffers(tmfqs_hip_state *h, size_t nch) {
 	hipFree(h->d_sums);
 	hipFree(h->d_ticks);
 	hipFree(h->d_ulog);
-	hipFree(h->d_ties);
 	HIPC(hipMalloc((void **)&h->d_sums, nch * sizeof(double)));
-	HIPC(hipMalloc((void **)&h->d_ticks, nch * sizeof(long long)));
+	HIPC(hipMalloc((void **)&h->d_ticks, nch * sizeof(ChunkTicks)));
 	HIPC(hipMalloc((void **)&h->d_ulog, nch * sizeof(int32_t)));
-	HIPC(hipMalloc((void **)&h->d_ties, nch * sizeof(int32_t)));
 	h->chunk_cap = nch;
 }
 
@@ -395,7 +393,7 @@ void sync_all(tmfqs_hip_state *h) {
 
 void exec_elementwise(tmfqs_hip_state *h, const std::vector<QOp> &ops, size_t lo, size_t hi) {
 	std::vector<EwStep> steps;
-	const unsigned block = h->cfg.grover_block > 0 ? (unsigned)h->cfg.grover_block : (h->prec == 0 ? 48u : 32u);
+	const unsigned block = h->cfg.grover_block > 0 ? (unsigned)h->cfg.grover_block : (h->prec == 0 ? 64u : 32u);
 	auto run_steps = [&]() {
 		size_t s0 = 0;
 		while(s0 < steps.size()) {
@@ -410,7 +408,7 @@ void exec_elementwise(tmfqs_hip_state *h, const std::vector<QOp> &ops, size_t lo
 	for(size_t i = lo; i < hi; ++i) {
 		const QOp &op = ops[i];
 		if(op.kind == Q_DIFF) {
-			EwStep s{EW_DIFFUSE, 0, op.ar, op.ai, 0, 0};
+			EwStep s{EW_DIFFUSE, 0, 2.0 * op.ar, 2.0 * op.ai, 0, 0};
 			steps.push_back(s);
 		} else if(op.kind == Q_AFFINE) {
 			EwStep s{EW_AFFINE, 0, op.ar, op.ai, op.br, op.bi};
@@ -483,7 +481,7 @@ void enqueue_ew(tmfqs_hip_state *h, const QOp &op) {
 	h->queue.push_back(op);
 	if(op.kind == Q_DIFF || op.kind == Q_AFFINE) {
 		++h->ew_steps_pending;
-		const size_t block = h->cfg.grover_block > 0 ? (size_t)h->cfg.grover_block : (h->prec == 0 ? 48u : 32u);
+		const size_t block = h->cfg.grover_block > 0 ? (size_t)h->cfg.grover_block : (h->prec == 0 ? 64u : 32u);
 		if(!h->cfg.fusion || h->ew_steps_pending >= block) flush(h);
 	}
 }
@@ -830,21 +828,25 @@ void two_sum(double a, double b, double &s, double &e) {
 	e = (a - (s - bb)) + (b - bb);
 }
 
-ChainResult chain_walk(tmfqs_hip_state *h, bool find, double rnd, const double *af) {
+inline double host_chain_value(double re, double im, int sel, const double *af) {
+	if(sel == 0) return host_prob(re, im, af);
+	return sel == 1 ? re : im;
+}
+
+// sel: 0 = probabilities of the (optionally affine-mapped) amplitudes, with
+// find/rnd locating the first state whose cumulative sum reaches rnd;
+// 1 / 2 = real / imaginary parts (the reference's sequential amplitude sums).
+ChainResult chain_walk(tmfqs_hip_state *h, int sel, bool find, double rnd, const double *af) {
 	sync_all(h);
 	const unsigned cl = h->n < kSampleChunkLog ? h->n : kSampleChunkLog;
 	const u64 chunk = 1ull << cl;
 	const u64 nch = 1ull << (h->n - cl);
 	ensure_chunk_buffers(h, nch);
-	double *daf = nullptr;
-	if(af) {
-		daf = h->d_red + 2 * kReduceBlocks + 8;
-		HIPC(hipMemcpyAsync(daf, af, 4 * sizeof(double), hipMemcpyHostToDevice, h->stream));
-	}
+	const double *daf = af ? (const double *)arena_upload(h, af, 4 * sizeof(double)) : nullptr;
 	std::vector<double> sums(nch);
 	{
-		ProfScope ps(h, "k_chunk_prob", (double)h->N * h->esz);
-		HIPC(launch_chunk_prob(h->L(), h->amp, h->n, daf, h->d_sums));
+		ProfScope ps(h, "k_chunk_sum", (double)h->N * h->esz);
+		HIPC(launch_chunk_sum(h->L(), h->amp, h->n, daf, sel, h->d_sums));
 	}
 	HIPC(hipMemcpyAsync(sums.data(), h->d_sums, nch * sizeof(double), hipMemcpyDeviceToHost, h->stream));
 	HIPC(hipStreamSynchronize(h->stream));
@@ -858,21 +860,20 @@ ChainResult chain_walk(tmfqs_hip_state *h, bool find, double rnd, const double *
 		el += e;
 		two_sum(s, el, eh, el);
 		const double end = eh + el;
-		if(start > 0x1p-1000 && sums[c] > 0) {
-			const int e0 = std::ilogb(start * (1.0 - 1e-6));
-			const int e1 = std::ilogb(end * (1.0 + 1e-6));
+		const double a0 = std::fabs(start), a1 = std::fabs(end);
+		if(a0 > 0x1p-1000 && a1 > 0x1p-1000 && (start > 0) == (end > 0)) {
+			const int e0 = std::ilogb(std::min(a0, a1) * (1.0 - 1e-6));
+			const int e1 = std::ilogb(std::max(a0, a1) * (1.0 + 1e-6));
 			if(e0 == e1) ulog[c] = e0 - 52;
 		}
 	}
-	std::vector<long long> ticks(nch);
-	std::vector<int32_t> ties(nch);
+	std::vector<ChunkTicks> tk(nch);
 	HIPC(hipMemcpyAsync(h->d_ulog, ulog.data(), nch * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
 	{
 		ProfScope ps(h, "k_chunk_ticks", (double)h->N * h->esz);
-		HIPC(launch_chunk_ticks(h->L(), h->amp, h->n, daf, h->d_ulog, h->d_ticks, h->d_ties));
+		HIPC(launch_chunk_ticks(h->L(), h->amp, h->n, daf, sel, h->d_ulog, h->d_ticks));
 	}
-	HIPC(hipMemcpyAsync(ticks.data(), h->d_ticks, nch * sizeof(long long), hipMemcpyDeviceToHost, h->stream));
-	HIPC(hipMemcpyAsync(ties.data(), h->d_ties, nch * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+	HIPC(hipMemcpyAsync(tk.data(), h->d_ticks, nch * sizeof(ChunkTicks), hipMemcpyDeviceToHost, h->stream));
 	HIPC(hipStreamSynchronize(h->stream));
 
 	std::vector<double> buf(2 * chunk);
@@ -885,16 +886,23 @@ ChainResult chain_walk(tmfqs_hip_state *h, bool find, double rnd, const double *
 			for(u64 i = 0; i < 2 * chunk; ++i) buf[i] = fbuf[i];
 		}
 	};
+	// Inside one binade of |S| (ulp u) every partial sum S_i = u * (K + prefix_i)
+	// is exact; it is the reference's value as long as all exact partial sums
+	// stay at least half an ulp inside the binade, which the integer prefix
+	// extremes certify with a one-ulp margin.
+	constexpr long long lo = (1ll << 52) + 1, hi = (1ll << 53) - 1;
 	ChainResult r;
 	double S = 0.0;
 	for(u64 c = 0; c < nch; ++c) {
-		if(sums[c] == 0.0) continue;  // every p_i is +0: S unchanged
+		const ChunkTicks &t = tk[c];
+		if(!t.nonzero) continue;  // every value is +-0: S unchanged
 		bool fast = false;
-		if(ulog[c] != INT32_MIN && !ties[c] && S > 0x1p-1000 && std::ilogb(S) - 52 == ulog[c]) {
-			const double K = std::ldexp(S, -ulog[c]);  // exact integer
-			const double Kend = K + (double)ticks[c];
-			if(Kend < 0x1p53) {
-				const double Send = std::ldexp(Kend, ulog[c]);
+		if(ulog[c] != INT32_MIN && !t.tie && std::fabs(S) > 0x1p-1000 && std::ilogb(S) - 52 == ulog[c]) {
+			const long long K = (long long)std::ldexp(S, -ulog[c]);  // exact integer, |K| < 2^53
+			const long long kmin = K + t.pmin, kmax = K + t.pmax, kend = K + t.ticks;
+			const bool ok = K > 0 ? (kmin >= lo && kmax <= hi) : (kmax <= -lo && kmin >= -hi);
+			if(ok) {
+				const double Send = std::ldexp((double)kend, ulog[c]);
 				if(!(find && rnd <= Send)) {
 					S = Send;
 					fast = true;
@@ -904,7 +912,7 @@ ChainResult chain_walk(tmfqs_hip_state *h, bool find, double rnd, const double *
 		if(fast) continue;
 		load_chunk(c);
 		for(u64 i = 0; i < chunk; ++i) {
-			S += host_prob(buf[2 * i], buf[2 * i + 1], af);
+			S += host_chain_value(buf[2 * i], buf[2 * i + 1], sel, af);
 			if(find && rnd <= S) {
 				r.found = true;
 				r.index = c * chunk + i;
@@ -999,7 +1007,6 @@ void destroy_handle(tmfqs_hip_state *h) {
 	hipFree(h->d_sums);
 	hipFree(h->d_ticks);
 	hipFree(h->d_ulog);
-	hipFree(h->d_ties);
 	if(h->arena.host) hipHostFree(h->arena.host);
 	if(h->arena.dev) hipFree(h->arena.dev);
 	if(h->h_stage) hipHostFree(h->h_stage);
@@ -1235,7 +1242,7 @@ int tmfqs_hip_total_probability(tmfqs_hip_state *h, double *out) {
 	checked(h);
 	require_init(h, "total probability query");
 	HIPC(hipSetDevice(h->cfg.device));
-	*out = chain_walk(h, false, 0.0, nullptr).total;
+	*out = chain_walk(h, 0, false, 0.0, nullptr).total;
 	ABI_END
 }
 
@@ -1245,7 +1252,7 @@ int tmfqs_hip_sample(tmfqs_hip_state *h, double rnd, uint64_t *out) {
 	require_init(h, "measurement");
 	if(rnd < 0.0 || rnd >= 1.0) throw std::invalid_argument("HipStateBackend::sampleMeasurement requires rnd in [0,1)");
 	HIPC(hipSetDevice(h->cfg.device));
-	const ChainResult r = chain_walk(h, true, rnd, nullptr);
+	const ChainResult r = chain_walk(h, 0, true, rnd, nullptr);
 	if(!r.found) throw std::runtime_error("HipStateBackend::sampleMeasurement cumulative probability did not reach sample");
 	*out = r.index;
 	ABI_END
@@ -1256,16 +1263,9 @@ int tmfqs_hip_amplitude_sum(tmfqs_hip_state *h, double *re, double *im) {
 	checked(h);
 	require_init(h, "amplitude chunk iteration");
 	HIPC(hipSetDevice(h->cfg.device));
-	flush(h);
-	{
-		ProfScope ps(h, "k_sum", (double)h->N * h->esz);
-		HIPC(launch_sum(h->L(), h->amp, h->n, h->d_red, h->d_red + 2 * kReduceBlocks));
-	}
-	double r[2];
-	HIPC(hipMemcpyAsync(r, h->d_red + 2 * kReduceBlocks, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-	HIPC(hipStreamSynchronize(h->stream));
-	*re = r[0];
-	*im = r[1];
+	// the reference sums real and imaginary parts sequentially in state order
+	*re = chain_walk(h, 1, false, 0.0, nullptr).total;
+	*im = chain_walk(h, 2, false, 0.0, nullptr).total;
 	ABI_END
 }
 
@@ -1275,7 +1275,7 @@ int tmfqs_hip_total_probability_affine(tmfqs_hip_state *h, double s_re, double s
 	require_init(h, "probability summation");
 	HIPC(hipSetDevice(h->cfg.device));
 	const double af[4] = {s_re, s_im, b_re, b_im};
-	*out = chain_walk(h, false, 0.0, af).total;
+	*out = chain_walk(h, 0, false, 0.0, af).total;
 	ABI_END
 }
 
@@ -1285,7 +1285,7 @@ int tmfqs_hip_sample_affine(tmfqs_hip_state *h, double rnd, double s_re, double 
 	require_init(h, "measurement");
 	HIPC(hipSetDevice(h->cfg.device));
 	const double af[4] = {s_re, s_im, b_re, b_im};
-	const ChainResult r = chain_walk(h, true, rnd, af);
+	const ChainResult r = chain_walk(h, 0, true, rnd, af);
 	if(!r.found) throw std::runtime_error("QuantumRegister::measure cumulative probability did not reach sample");
 	*out = r.index;
 	ABI_END
@@ -1389,8 +1389,10 @@ int tmfqs_hip_controlled_phase(tmfqs_hip_state *h, unsigned control, unsigned ta
 	op.b0 = pbit(h, control);
 	op.b1 = pbit(h, target);
 	op.theta = theta;
-	op.cr = std::cos(theta);
-	op.ci = std::sin(theta);
+	// dense_state_backend.cpp:296-297 computes cos and sin of theta, which the
+	// reference's g++ build turns into one sincos call; glibc's sincos can
+	// differ from separate cos/sin calls in the last bit.
+	__builtin_sincos(theta, &op.ci, &op.cr);
 	enqueue_gate(h, op);
 	ABI_END
 }
@@ -1518,6 +1520,81 @@ int tmfqs_hip_profile_read(tmfqs_hip_state *h, char (*names)[64], uint64_t *laun
 		++i;
 	}
 	*count = i;
+	ABI_END
+}
+
+int tmfqs_hip_compare(tmfqs_hip_state *ref, tmfqs_hip_state *cand, tmfqs_hip_comparison *out) {
+	ABI_BEGIN
+	checked(ref);
+	checked(cand);
+	require_init(ref, "comparison");
+	require_init(cand, "comparison");
+	if(ref->n != cand->n) throw std::invalid_argument("Compared states must have the same state count");
+	if(ref->cfg.device != cand->cfg.device) throw std::invalid_argument("tmfqs_hip_compare: states live on different devices");
+	HIPC(hipSetDevice(ref->cfg.device));
+	sync_all(cand);
+	sync_all(ref);
+	CompareBlock *d = nullptr;
+	HIPC(hipMalloc((void **)&d, kReduceBlocks * sizeof(CompareBlock)));
+	std::vector<CompareBlock> blocks(kReduceBlocks);
+	unsigned nb = 0;
+	hipError_t e = launch_compare(ref->L(), ref->prec, ref->amp, cand->prec, cand->amp, ref->n, d, &nb);
+	if(e == hipSuccess) e = hipMemcpyAsync(blocks.data(), d, nb * sizeof(CompareBlock), hipMemcpyDeviceToHost, ref->stream);
+	if(e == hipSuccess) e = hipStreamSynchronize(ref->stream);
+	hipFree(d);
+	HIPC(e);
+	double d2 = 0, r2 = 0, c2 = 0, ovr = 0, ovi = 0, neq = 0, maxd = -1, maxcomp = 0, maxdp = 0;
+	u64 worst = 0;
+	for(unsigned i = 0; i < nb; ++i) {
+		const CompareBlock &b = blocks[i];
+		d2 += b.d2;
+		r2 += b.r2;
+		c2 += b.c2;
+		ovr += b.ovr;
+		ovi += b.ovi;
+		neq += b.neq;
+		if(b.maxd > maxd || (b.maxd == maxd && b.worst < worst)) {
+			maxd = b.maxd;
+			worst = b.worst;
+		}
+		maxcomp = std::max(maxcomp, b.maxcomp);
+		maxdp = std::max(maxdp, b.maxdp);
+	}
+	out->bitwise_equal = neq == 0;
+	out->max_abs_amplitude_error = maxd > 0 ? maxd : 0.0;
+	out->max_abs_component_error = maxcomp;
+	out->rmse_amplitude = std::sqrt(d2 / (double)ref->N);
+	out->rel_l2 = r2 == 0 ? 0.0 : std::sqrt(d2 / r2);
+	out->max_abs_probability_error = maxdp;
+	out->total_probability_diff = std::abs(c2 - r2);
+	out->worst_state = maxd > 0 ? worst : 0;
+	out->overlap_re = ovr;
+	out->overlap_im = ovi;
+	out->fidelity = (r2 > 0 && c2 > 0) ? (ovr * ovr + ovi * ovi) / (r2 * c2) : 0.0;
+	ABI_END
+}
+
+int tmfqs_hip_copy_probe(tmfqs_hip_state *h, int reps, double *ms) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "copy probe");
+	if(reps < 1) throw std::invalid_argument("tmfqs_hip_copy_probe: reps must be positive");
+	HIPC(hipSetDevice(h->cfg.device));
+	sync_all(h);
+	ensure_scratch(h);
+	hipEvent_t e0 = take_event(h), e1 = take_event(h);
+	HIPC(hipEventRecord(e0, h->stream));
+	for(int i = 0; i < reps; ++i) {
+		ProfScope ps(h, "k_copy", pass_bytes(h));
+		HIPC(launch_copy(h->L(), h->amp, h->scratch, h->n));
+	}
+	HIPC(hipEventRecord(e1, h->stream));
+	HIPC(hipEventSynchronize(e1));
+	float t = 0;
+	HIPC(hipEventElapsedTime(&t, e0, e1));
+	h->event_pool.push_back(e0);
+	h->event_pool.push_back(e1);
+	*ms = t / reps;
 	ABI_END
 }
 

@@ -97,6 +97,14 @@ hipError_t launch_bitswap(const Launch &L, void *a, unsigned n, unsigned p, unsi
 hipError_t launch_involution(const Launch &L, void *a, unsigned n, const unsigned *perm, unsigned C);
 hipError_t launch_ew_chain(const Launch &L, void *a, unsigned n, const EwStep *dsteps, unsigned nsteps);
 hipError_t launch_scatter(const Launch &L, void *a, const Scatter *dsc, unsigned count);
+hipError_t launch_copy(const Launch &L, const void *src, void *dst, unsigned n);
+
+struct CompareBlock {
+	double d2, r2, c2, ovr, ovi, neq, maxd, maxcomp, maxdp;
+	u64 worst;
+};
+hipError_t launch_compare(const Launch &L, int prec_ref, const void *ref, int prec_cand, const void *cand, unsigned n,
+                          CompareBlock *dout, unsigned *blocks);
 hipError_t launch_tile_pass(const Launch &L, void *a, const TilePass &P, const TileOp *dops, const double *dmats,
                             const TwiddleTerm *dterms, const double *dphase);
 // Reductions write `blocks` partials into dpart, then reduce into dout
@@ -107,11 +115,16 @@ hipError_t launch_norm(const Launch &L, const void *a, unsigned n, const double 
                        double *dout);
 // Sampling support: per-chunk FP64 probability sums (chunk = kSampleChunk states).
 constexpr unsigned kSampleChunkLog = 12;
-hipError_t launch_chunk_prob(const Launch &L, const void *a, unsigned n, const double *affine4, double *dsums);
-// Per-chunk integer increment sums of the sequential chain on the grid
-// u = 2^ulog[c] (int64), with a tie flag per chunk; ulog[c] == INT32_MIN marks
-// a chunk that is resolved on the host.
-hipError_t launch_chunk_ticks(const Launch &L, const void *a, unsigned n, const double *affine4,
-                              const int32_t *dulog, long long *dticks, int32_t *dties);
+// Chain values: sel 0 = |scale*a + bias|^2 (affine4 may be null), 1 = Re a, 2 = Im a.
+hipError_t launch_chunk_sum(const Launch &L, const void *a, unsigned n, const double *affine4, int sel, double *dsums);
+// Per-chunk integer increments of the sequential chain on the grid
+// u = 2^ulog[c] (int64) with their prefix extremes, a tie flag and a non-zero
+// flag; ulog[c] == INT32_MIN marks a chunk that is resolved on the host.
+struct ChunkTicks {
+	long long ticks, pmin, pmax;
+	int32_t tie, nonzero;
+};
+hipError_t launch_chunk_ticks(const Launch &L, const void *a, unsigned n, const double *affine4, int sel,
+                              const int32_t *dulog, ChunkTicks *dout);
 
 }  // namespace tmfqs_hip

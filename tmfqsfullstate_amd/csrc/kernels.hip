@@ -266,8 +266,9 @@ __global__ void __launch_bounds__(256) k_ew_chain(typename V2T<R>::t *__restrict
 		for(unsigned s = 0; s < nsteps; ++s) {
 			const EwStep st = steps[s];
 			if(st.kind == EW_DIFFUSE) {
-				re = 2.0 * st.ar - re;
-				im = 2.0 * st.ai - im;
+				// (ar, ai) already hold 2m: doubling is exact, so 2m - a rounds once as in the reference
+				re = st.ar - re;
+				im = st.ai - im;
 			} else {
 				const double nr = st.ar * re - st.ai * im + st.br;
 				const double ni = st.ar * im + st.ai * re + st.bi;
@@ -277,6 +278,21 @@ __global__ void __launch_bounds__(256) k_ew_chain(typename V2T<R>::t *__restrict
 		}
 		nt_store(mk<V>(re, im), &a[i]);
 	}
+}
+
+// Plain streaming copy (bandwidth ceiling and counter calibration).
+template <typename R>
+__global__ void __launch_bounds__(256) k_copy(const typename V2T<R>::t *__restrict__ src, typename V2T<R>::t *__restrict__ dst, u64 N) {
+	for(u64 i = gtid(); i < N; i += gstride()) nt_store(nt_load(&src[i]), &dst[i]);
+}
+
+hipError_t launch_copy(const Launch &L, const void *src, void *dst, unsigned n) {
+	const u64 N = 1ull << n;
+	if(L.prec == 0)
+		hipLaunchKernelGGL(k_copy<double>, dim3(grid_for(N, 256, 16384)), dim3(256), 0, L.stream, (const double2 *)src, (double2 *)dst, N);
+	else
+		hipLaunchKernelGGL(k_copy<float>, dim3(grid_for(N, 256, 16384)), dim3(256), 0, L.stream, (const float2 *)src, (float2 *)dst, N);
+	return hipGetLastError();
 }
 
 template <typename R>
@@ -374,56 +390,195 @@ __global__ void k_finish(const double *part, unsigned count, unsigned width, dou
 	}
 }
 
-// Per-chunk probability sums (chunk = 2^kSampleChunkLog states; fewer when N is smaller).
+// Value fed to a sequential chain: 0 = probability |logical a|^2,
+// 1 = real part, 2 = imaginary part.
+__device__ __forceinline__ double chain_value(double re, double im, int sel, const double *af) {
+	if(sel == 0) return prob_of(re, im, af);
+	return sel == 1 ? re : im;
+}
+
+// Per-chunk FP64 sums of the chain values (chunk = 2^kSampleChunkLog states;
+// fewer when N is smaller).
 template <typename R>
-__global__ void __launch_bounds__(256) k_chunk_prob(const typename V2T<R>::t *__restrict__ a, u64 N, u64 chunk, u64 nchunks,
-                                                    const double *af, double *sums) {
+__global__ void __launch_bounds__(256) k_chunk_sum(const typename V2T<R>::t *__restrict__ a, u64 chunk, u64 nchunks,
+                                                   const double *af, int sel, double *sums) {
 	__shared__ double sh[8];
 	for(u64 c = blockIdx.x; c < nchunks; c += gridDim.x) {
 		double s = 0;
 		for(u64 i = threadIdx.x; i < chunk; i += blockDim.x) {
 			const auto v = a[c * chunk + i];
-			s += prob_of(v.x, v.y, af);
+			s += chain_value(v.x, v.y, sel, af);
 		}
 		s = block_sum(s, sh);
 		if(threadIdx.x == 0) sums[c] = s;
 	}
 }
 
-// Per-chunk integer increments of the sequential chain S = fl(S + p_i) while S
-// stays in one binade whose ulp is u = 2^ulog: each step adds rint(p_i / u)
-// ulps exactly, unless p_i / u ends in exactly one half (a tie, flagged).
+// Per-chunk integer increments of the sequential chain S = fl(S + x_i) on the
+// grid u = 2^ulog[c]: while every partial sum stays inside one binade of that
+// ulp, each step adds rint(x_i / u) ulps exactly, unless x_i / u ends in
+// exactly one half (a tie, flagged).  Each thread walks a run of consecutive
+// states so that the chunk's prefix minimum and maximum (in ulps, relative to
+// the chunk start) come out in state order.
 template <typename R>
 __global__ void __launch_bounds__(256) k_chunk_ticks(const typename V2T<R>::t *__restrict__ a, u64 chunk, u64 nchunks,
-                                                     const double *af, const int32_t *ulog, long long *ticks,
-                                                     int32_t *ties) {
-	__shared__ long long sh[8];
+                                                     const double *af, int sel, const int32_t *ulog, ChunkTicks *out) {
+	__shared__ long long sh[256];
+	__shared__ long long smin[256];
+	__shared__ long long smax[256];
+	__shared__ int sflag[256];
+	const unsigned T = blockDim.x;
+	const u64 per = chunk >= T ? chunk / T : 1;
+	const bool active = chunk >= T || threadIdx.x < chunk;
 	for(u64 c = blockIdx.x; c < nchunks; c += gridDim.x) {
 		const int32_t ul = ulog[c];
-		if(ul == INT32_MIN) {
-			if(threadIdx.x == 0) {
-				ticks[c] = 0;
-				ties[c] = 0;
+		long long cum = 0, lmin = LLONG_MAX, lmax = LLONG_MIN;
+		int tie = 0, nz = 0;
+		if(active) {
+			const u64 first = c * chunk + (u64)threadIdx.x * per;
+			for(u64 j = 0; j < per; ++j) {
+				const auto v = a[first + j];
+				const double x = chain_value(v.x, v.y, sel, af);
+				nz |= (x != 0.0);
+				if(ul != INT32_MIN) {
+					const double q = ldexp(x, -ul);
+					if(q - floor(q) == 0.5) tie = 1;
+					cum += (long long)rint(q);
+					lmin = cum < lmin ? cum : lmin;
+					lmax = cum > lmax ? cum : lmax;
+				}
 			}
-			continue;
 		}
-		long long k = 0;
-		int tie = 0;
-		for(u64 i = threadIdx.x; i < chunk; i += blockDim.x) {
-			const auto v = a[c * chunk + i];
-			const double p = prob_of(v.x, v.y, af);
-			const double q = ldexp(p, -ul);
-			const double f = floor(q);
-			if(q - f == 0.5) tie = 1;
-			k += (long long)rint(q);
-		}
-		k = block_sum_ll(k, sh);
-		const long long t = block_sum_ll((long long)tie, sh);
+		sh[threadIdx.x] = cum;
+		smin[threadIdx.x] = lmin;
+		smax[threadIdx.x] = lmax;
+		sflag[threadIdx.x] = tie | (nz << 1);
+		__syncthreads();
 		if(threadIdx.x == 0) {
-			ticks[c] = k;
-			ties[c] = t != 0;
+			long long run = 0, gmin = LLONG_MAX, gmax = LLONG_MIN;
+			int anyTie = 0, anyNz = 0;
+			for(unsigned t = 0; t < T; ++t) {
+				if(smin[t] != LLONG_MAX) {
+					gmin = run + smin[t] < gmin ? run + smin[t] : gmin;
+					gmax = run + smax[t] > gmax ? run + smax[t] : gmax;
+				}
+				run += sh[t];
+				anyTie |= sflag[t] & 1;
+				anyNz |= sflag[t] >> 1;
+			}
+			ChunkTicks o;
+			o.ticks = run;
+			o.pmin = gmin == LLONG_MAX ? 0 : gmin;
+			o.pmax = gmax == LLONG_MIN ? 0 : gmax;
+			o.tie = anyTie;
+			o.nonzero = anyNz;
+			out[c] = o;
+		}
+		__syncthreads();
+	}
+}
+
+// ------------------------------------------------------------------------
+// Two-state comparison (the reference's experiment::compareStates metrics,
+// comparison.cpp:12-63, plus the overlap <r|c>): per-block partials of sums
+// and maxima; the host combines the blocks.
+__device__ __forceinline__ void block_max_idx(double &v, u64 &idx, double *shv, u64 *shi) {
+	for(int off = 32; off > 0; off >>= 1) {
+		const double ov = __shfl_down(v, off, 64);
+		const u64 oi = __shfl_down(idx, off, 64);
+		if(ov > v || (ov == v && oi < idx)) {
+			v = ov;
+			idx = oi;
 		}
 	}
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	if(lane == 0) {
+		shv[w] = v;
+		shi[w] = idx;
+	}
+	__syncthreads();
+	if(threadIdx.x == 0)
+		for(int i = 1; i < (int)(blockDim.x >> 6); ++i)
+			if(shv[i] > v || (shv[i] == v && shi[i] < idx)) {
+				v = shv[i];
+				idx = shi[i];
+			}
+	__syncthreads();
+}
+
+template <typename RA, typename RB>
+__global__ void __launch_bounds__(256) k_compare(const typename V2T<RA>::t *__restrict__ r, const typename V2T<RB>::t *__restrict__ c, u64 N,
+                                                 CompareBlock *out) {
+	__shared__ double sh[8];
+	__shared__ double shv[8];
+	__shared__ u64 shi[8];
+	double d2 = 0, r2 = 0, c2 = 0, ovr = 0, ovi = 0, neq = 0;
+	double maxd = -1, maxcomp = 0, maxdp = 0;
+	u64 worst = 0;
+	for(u64 i = gtid(); i < N; i += gstride()) {
+		const auto a = r[i];
+		const auto b = c[i];
+		const double ar = a.x, ai = a.y, br = b.x, bi = b.y;
+		if(ar != br || ai != bi) neq += 1;
+		const double dr = br - ar, di = bi - ai;
+		const double dd = dr * dr + di * di;
+		const double ae = sqrt(dd);
+		if(ae > maxd) {
+			maxd = ae;
+			worst = i;
+		}
+		maxcomp = fmax(maxcomp, fmax(fabs(dr), fabs(di)));
+		d2 += dd;
+		const double pa = ar * ar + ai * ai, pb = br * br + bi * bi;
+		r2 += pa;
+		c2 += pb;
+		maxdp = fmax(maxdp, fabs(pb - pa));
+		ovr += ar * br + ai * bi;
+		ovi += ar * bi - ai * br;
+	}
+	d2 = block_sum(d2, sh);
+	r2 = block_sum(r2, sh);
+	c2 = block_sum(c2, sh);
+	ovr = block_sum(ovr, sh);
+	ovi = block_sum(ovi, sh);
+	neq = block_sum(neq, sh);
+	block_max_idx(maxd, worst, shv, shi);
+	double tmp = maxcomp;
+	u64 dummy = 0;
+	block_max_idx(tmp, dummy, shv, shi);
+	maxcomp = tmp;
+	tmp = maxdp;
+	block_max_idx(tmp, dummy, shv, shi);
+	maxdp = tmp;
+	if(threadIdx.x == 0) {
+		CompareBlock &o = out[blockIdx.x];
+		o.d2 = d2;
+		o.r2 = r2;
+		o.c2 = c2;
+		o.ovr = ovr;
+		o.ovi = ovi;
+		o.neq = neq;
+		o.maxd = maxd;
+		o.worst = worst;
+		o.maxcomp = maxcomp;
+		o.maxdp = maxdp;
+	}
+}
+
+hipError_t launch_compare(const Launch &L, int prec_ref, const void *ref, int prec_cand, const void *cand, unsigned n,
+                          CompareBlock *dout, unsigned *blocks) {
+	const u64 N = 1ull << n;
+	const unsigned g = grid_for(N, 256, kReduceBlocks);
+	*blocks = g;
+	if(prec_ref == 0 && prec_cand == 0)
+		hipLaunchKernelGGL((k_compare<double, double>), dim3(g), dim3(256), 0, L.stream, (const double2 *)ref, (const double2 *)cand, N, dout);
+	else if(prec_ref == 0 && prec_cand == 1)
+		hipLaunchKernelGGL((k_compare<double, float>), dim3(g), dim3(256), 0, L.stream, (const double2 *)ref, (const float2 *)cand, N, dout);
+	else if(prec_ref == 1 && prec_cand == 0)
+		hipLaunchKernelGGL((k_compare<float, double>), dim3(g), dim3(256), 0, L.stream, (const float2 *)ref, (const double2 *)cand, N, dout);
+	else
+		hipLaunchKernelGGL((k_compare<float, float>), dim3(g), dim3(256), 0, L.stream, (const float2 *)ref, (const float2 *)cand, N, dout);
+	return hipGetLastError();
 }
 
 // ------------------------------------------------------------------------
@@ -805,26 +960,26 @@ static void chunk_geometry(unsigned n, u64 &chunk, u64 &nchunks) {
 	nchunks = 1ull << (n - cl);
 }
 
-hipError_t launch_chunk_prob(const Launch &L, const void *a, unsigned n, const double *affine4, double *dsums) {
+hipError_t launch_chunk_sum(const Launch &L, const void *a, unsigned n, const double *affine4, int sel, double *dsums) {
 	u64 chunk, nch;
 	chunk_geometry(n, chunk, nch);
 	const unsigned g = (unsigned)(nch < 16384 ? nch : 16384);
 	if(L.prec == 0)
-		TMFQS_LAUNCH(k_chunk_prob<double>, g, 256, 0, L.stream, (const double2 *)a, 1ull << n, chunk, nch, affine4, dsums);
+		TMFQS_LAUNCH(k_chunk_sum<double>, g, 256, 0, L.stream, (const double2 *)a, chunk, nch, affine4, sel, dsums);
 	else
-		TMFQS_LAUNCH(k_chunk_prob<float>, g, 256, 0, L.stream, (const float2 *)a, 1ull << n, chunk, nch, affine4, dsums);
+		TMFQS_LAUNCH(k_chunk_sum<float>, g, 256, 0, L.stream, (const float2 *)a, chunk, nch, affine4, sel, dsums);
 	return hipGetLastError();
 }
 
-hipError_t launch_chunk_ticks(const Launch &L, const void *a, unsigned n, const double *affine4, const int32_t *dulog,
-                              long long *dticks, int32_t *dties) {
+hipError_t launch_chunk_ticks(const Launch &L, const void *a, unsigned n, const double *affine4, int sel, const int32_t *dulog,
+                              ChunkTicks *dout) {
 	u64 chunk, nch;
 	chunk_geometry(n, chunk, nch);
 	const unsigned g = (unsigned)(nch < 16384 ? nch : 16384);
 	if(L.prec == 0)
-		TMFQS_LAUNCH(k_chunk_ticks<double>, g, 256, 0, L.stream, (const double2 *)a, chunk, nch, affine4, dulog, dticks, dties);
+		TMFQS_LAUNCH(k_chunk_ticks<double>, g, 256, 0, L.stream, (const double2 *)a, chunk, nch, affine4, sel, dulog, dout);
 	else
-		TMFQS_LAUNCH(k_chunk_ticks<float>, g, 256, 0, L.stream, (const float2 *)a, chunk, nch, affine4, dulog, dticks, dties);
+		TMFQS_LAUNCH(k_chunk_ticks<float>, g, 256, 0, L.stream, (const float2 *)a, chunk, nch, affine4, sel, dulog, dout);
 	return hipGetLastError();
 }
 
