@@ -128,6 +128,7 @@ struct tmfqs_hip_state {
 	hipStream_t stream = nullptr;
 	double *d_red = nullptr;    // reduction partials + outputs
 	double *d_phase = nullptr;  // 5 x 256 complex phase tables
+	double *d_cin = nullptr;    // e^{i pi v / 2^j}, j = 0..3, v < 2^j (QFT in-block constants)
 	double *d_sums = nullptr;   // sampling chunk sums
 	ChunkTicks *d_ticks = nullptr;
 	int32_t *d_ulog = nullptr;
@@ -297,6 +298,16 @@ void build_phase_tables(tmfqs_hip_state *h) {
 	}
 	HIPC(hipMalloc((void **)&h->d_phase, t.size() * sizeof(double)));
 	HIPC(hipMemcpy(h->d_phase, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+	std::vector<double> c;
+	for(int j = 0; j < 4; ++j)
+		for(int v = 0; v < (1 << j); ++v) {
+			double s, co;
+			__builtin_sincos(kPi * (double)v / (double)(1 << j), &s, &co);
+			c.push_back(co);
+			c.push_back(s);
+		}
+	HIPC(hipMalloc((void **)&h->d_cin, c.size() * sizeof(double)));
+	HIPC(hipMemcpy(h->d_cin, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice));
 }
 
 double pass_bytes(const tmfqs_hip_state *h) { return 2.0 * (double)h->N * (double)h->esz; }
@@ -393,14 +404,27 @@ void sync_all(tmfqs_hip_state *h) {
 
 void exec_elementwise(tmfqs_hip_state *h, const std::vector<QOp> &ops, size_t lo, size_t hi) {
 	std::vector<EwStep> steps;
-	const unsigned block = h->cfg.grover_block > 0 ? (unsigned)h->cfg.grover_block : (h->prec == 0 ? 64u : 32u);
+	const unsigned block = h->cfg.grover_block > 0 ? (unsigned)h->cfg.grover_block : (h->prec == 0 ? 96u : 48u);
 	auto run_steps = [&]() {
 		size_t s0 = 0;
 		while(s0 < steps.size()) {
 			const size_t cnt = std::min<size_t>(block, steps.size() - s0);
-			const EwStep *d = (const EwStep *)arena_upload(h, steps.data() + s0, cnt * sizeof(EwStep));
-			ProfScope ps(h, "k_ew_chain", pass_bytes(h));
-			HIPC(launch_ew_chain(h->L(), h->amp, h->n, d, (unsigned)cnt));
+			bool pure = true;
+			for(size_t i = s0; i < s0 + cnt; ++i) pure = pure && steps[i].kind == EW_DIFFUSE;
+			if(pure) {
+				std::vector<double> m2(2 * cnt);
+				for(size_t i = 0; i < cnt; ++i) {
+					m2[2 * i] = steps[s0 + i].ar;
+					m2[2 * i + 1] = steps[s0 + i].ai;
+				}
+				const double *d = (const double *)arena_upload(h, m2.data(), m2.size() * sizeof(double));
+				ProfScope ps(h, "k_diffuse_chain", pass_bytes(h));
+				HIPC(launch_diffuse_chain(h->L(), h->amp, h->n, d, (unsigned)cnt));
+			} else {
+				const EwStep *d = (const EwStep *)arena_upload(h, steps.data() + s0, cnt * sizeof(EwStep));
+				ProfScope ps(h, "k_ew_chain", pass_bytes(h));
+				HIPC(launch_ew_chain(h->L(), h->amp, h->n, d, (unsigned)cnt));
+			}
 			s0 += cnt;
 		}
 		steps.clear();
@@ -481,7 +505,7 @@ void enqueue_ew(tmfqs_hip_state *h, const QOp &op) {
 	h->queue.push_back(op);
 	if(op.kind == Q_DIFF || op.kind == Q_AFFINE) {
 		++h->ew_steps_pending;
-		const size_t block = h->cfg.grover_block > 0 ? (size_t)h->cfg.grover_block : (h->prec == 0 ? 64u : 32u);
+		const size_t block = h->cfg.grover_block > 0 ? (size_t)h->cfg.grover_block : (h->prec == 0 ? 96u : 48u);
 		if(!h->cfg.fusion || h->ew_steps_pending >= block) flush(h);
 	}
 }
@@ -579,6 +603,44 @@ bool is_qft_ladder_cp(const QOp &op, unsigned &hi, unsigned &lo) {
 	return op.theta == want;
 }
 
+// Phase-table chunks (chunk j holds F bits 56-8j .. 63-8j, five chunks) that an
+// F with bits only in [lo, hi] can touch; bits below the table are dropped by
+// the lookup anyway.
+static void phase_chunks(unsigned lo, unsigned hi, uint32_t &j0, uint32_t &j1) {
+	j0 = std::min(4u, (63u - std::min(hi, 63u)) / 8u);
+	j1 = std::min(4u, (63u - std::min(lo, 63u)) / 8u);
+}
+
+// A pass that is exactly eight consecutive QFT targets t = pbot+7 .. pbot,
+// each an H on t followed by the complete ladder of controlled phases
+// pi/2^(t-c) from every lower bit c (in any order), with pbot >= C.  Such a
+// pass runs as one k_qft_block launch.
+bool match_qft_block(const std::vector<QOp> &ops, const std::vector<size_t> &idx, unsigned C, unsigned &pbot) {
+	size_t k = 0;
+	int targets = 0;
+	unsigned prev = 0;
+	while(k < idx.size()) {
+		const QOp &hop = ops[idx[k]];
+		if(hop.kind != Q_H) return false;
+		const unsigned t = hop.b0;
+		if(targets > 0 && t + 1 != prev) return false;
+		++k;
+		u64 seen = 0;
+		while(k < idx.size() && ops[idx[k]].kind == Q_CP) {
+			unsigned hb, lb;
+			if(!is_qft_ladder_cp(ops[idx[k]], hb, lb) || hb != t || ((seen >> lb) & 1ull)) return false;
+			seen |= 1ull << lb;
+			++k;
+		}
+		if(seen != ((1ull << t) - 1ull)) return false;
+		prev = t;
+		++targets;
+	}
+	if(targets != 8) return false;
+	pbot = prev;
+	return pbot >= C;
+}
+
 void run_permutation(tmfqs_hip_state *h, std::vector<unsigned> map) {
 	// Data of nominal bit b sits at physical map[b]; move it back to b.
 	const unsigned n = h->n;
@@ -653,6 +715,12 @@ void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_
 
 	// 3. emit each pass as a tile kernel.
 	for(const PassPlan &P : passes) {
+		unsigned pbot = 0;
+		if(!h->cfg.exact_phases && n <= 40 && match_qft_block(ops, P.ops, qft_block_coalesce_bits(h->prec), pbot)) {
+			ProfScope ps(h, "k_qft_block", pass_bytes(h));
+			HIPC(launch_qft_block(h->L(), h->amp, n, pbot, h->d_phase, h->d_cin));
+			continue;
+		}
 		unsigned Cp = T;
 		std::vector<unsigned> high;
 		for(;;) {
@@ -746,6 +814,8 @@ void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_
 							add_cond(t, hb);
 							t.amask = keep;
 							t.shift = 63u - hb;
+							phase_chunks((unsigned)__builtin_ctzll(keep) + t.shift, 63u - (unsigned)__builtin_clzll(keep) + t.shift,
+							             t.cj0, t.cj1);
 							tops.push_back(t);
 						}
 						if(defer) terms.push_back(TwiddleTerm{1ull << hb, defer, 63u - hb, 0});
@@ -771,7 +841,18 @@ void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_
 			t.kind = TOP_TWIDDLE;
 			t.twid = (unsigned)allterms.size();
 			t.nterms = (unsigned)std::min<size_t>(kMaxTwiddleTerms, terms.size() - s);
-			for(unsigned q = 0; q < t.nterms; ++q) allterms.push_back(terms[s + q]);
+			// F is a sum of the terms: its lowest bit is the lowest term bit, its
+			// highest is bounded by the sum of the term maxima (wrapping mod 2^64).
+			unsigned lo = 63;
+			unsigned __int128 fmax = 0;
+			for(unsigned q = 0; q < t.nterms; ++q) {
+				const TwiddleTerm &w = terms[s + q];
+				allterms.push_back(w);
+				lo = std::min(lo, (unsigned)__builtin_ctzll(w.mask) + w.shift);
+				fmax += (unsigned __int128)w.mask << w.shift;
+			}
+			const unsigned hi = (fmax >> 64) ? 63u : 63u - (unsigned)__builtin_clzll((u64)fmax);
+			phase_chunks(lo, hi, t.cj0, t.cj1);
 			tops.push_back(t);
 		}
 		if(tops.empty()) continue;
@@ -1004,6 +1085,7 @@ void destroy_handle(tmfqs_hip_state *h) {
 	hipFree(h->scratch);
 	hipFree(h->d_red);
 	hipFree(h->d_phase);
+	hipFree(h->d_cin);
 	hipFree(h->d_sums);
 	hipFree(h->d_ticks);
 	hipFree(h->d_ulog);

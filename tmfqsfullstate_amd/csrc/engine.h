@@ -48,6 +48,7 @@ struct TileOp {
 	u64 amask;            // LADDER mask
 	uint32_t shift;       // LADDER shift
 	uint32_t twid;        // TWIDDLE: index of the first term in the term pool
+	uint32_t cj0, cj1;    // LADDER/TWIDDLE: phase-table chunks [cj0, cj1] F can occupy
 	double cr, ci;        // CP phase
 };
 
@@ -97,6 +98,13 @@ hipError_t launch_bitswap(const Launch &L, void *a, unsigned n, unsigned p, unsi
 hipError_t launch_involution(const Launch &L, void *a, unsigned n, const unsigned *perm, unsigned C);
 hipError_t launch_ew_chain(const Launch &L, void *a, unsigned n, const EwStep *dsteps, unsigned nsteps);
 hipError_t launch_scatter(const Launch &L, void *a, const Scatter *dsc, unsigned count);
+// QFT block pass over targets pbot+7 .. pbot (see kernels.hip); dcin holds the
+// 15 constants e^{i pi v / 2^j}, j = 0..3, v < 2^j.
+hipError_t launch_qft_block(const Launch &L, void *a, unsigned n, unsigned pbot, const double *dphase, const double *dcin);
+// Low contiguous bits a QFT block tile keeps (4 for complex128, 5 for complex64).
+unsigned qft_block_coalesce_bits(int prec);
+// Chain of diffusions only: dm2 holds nsteps pairs (2 Re m, 2 Im m).
+hipError_t launch_diffuse_chain(const Launch &L, void *a, unsigned n, const double *dm2, unsigned nsteps);
 hipError_t launch_copy(const Launch &L, const void *src, void *dst, unsigned n);
 
 struct CompareBlock {

@@ -295,6 +295,53 @@ hipError_t launch_copy(const Launch &L, const void *src, void *dst, unsigned n) 
 	return hipGetLastError();
 }
 
+// Pure diffusion chains (the Grover loop): steps hold 2m per diffusion.  Each
+// thread streams 4 elements of one 1024-element block (4 x 16 B in flight per
+// lane) and applies every step to all four in FP64 registers, so one sweep of
+// HBM carries the whole chain.  a = 2m - a rounds once per step exactly as the
+// reference's inversionAboutMean(mean) does.
+template <typename R>
+__global__ void __launch_bounds__(256) k_diffuse_chain(typename V2T<R>::t *__restrict__ a, u64 N, const double2 *__restrict__ m2,
+                                                       unsigned nsteps) {
+	using V = typename V2T<R>::t;
+	constexpr int U = 4;
+	const u64 stride = (u64)gridDim.x * blockDim.x * U;
+	for(u64 base = (u64)blockIdx.x * blockDim.x * U + threadIdx.x; base < N; base += stride) {
+		double re[U], im[U];
+#pragma unroll
+		for(int u = 0; u < U; ++u) {
+			const u64 i = base + (u64)u * blockDim.x;
+			V v = i < N ? nt_load(&a[i]) : mk<V>(0.0, 0.0);
+			re[u] = v.x;
+			im[u] = v.y;
+		}
+		for(unsigned s = 0; s < nsteps; ++s) {
+			const double2 m = m2[s];
+#pragma unroll
+			for(int u = 0; u < U; ++u) {
+				re[u] = m.x - re[u];
+				im[u] = m.y - im[u];
+			}
+		}
+#pragma unroll
+		for(int u = 0; u < U; ++u) {
+			const u64 i = base + (u64)u * blockDim.x;
+			if(i < N) nt_store(mk<V>(re[u], im[u]), &a[i]);
+		}
+	}
+}
+
+hipError_t launch_diffuse_chain(const Launch &L, void *a, unsigned n, const double *dm2, unsigned nsteps) {
+	const u64 N = 1ull << n;
+	u64 blocks = (N + 1023) / 1024;
+	if(blocks > 262144) blocks = 262144;
+	if(L.prec == 0)
+		hipLaunchKernelGGL(k_diffuse_chain<double>, dim3((unsigned)blocks), dim3(256), 0, L.stream, (double2 *)a, N, (const double2 *)dm2, nsteps);
+	else
+		hipLaunchKernelGGL(k_diffuse_chain<float>, dim3((unsigned)blocks), dim3(256), 0, L.stream, (float2 *)a, N, (const double2 *)dm2, nsteps);
+	return hipGetLastError();
+}
+
 template <typename R>
 __global__ void k_scatter(typename V2T<R>::t *__restrict__ a, const Scatter *__restrict__ sc, unsigned count) {
 	const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -597,10 +644,177 @@ __device__ __forceinline__ double2 phase_of(u64 F, const double2 *__restrict__ t
 	return p;
 }
 
+// Same, over the table chunks [j0, j1] only (F has no bits outside them).
+__device__ __forceinline__ double2 phase_of_range(u64 F, const double2 *__restrict__ tab, unsigned j0, unsigned j1) {
+	double2 p = tab[j0 * 256u + ((F >> (56u - 8u * j0)) & 255u)];
+	for(unsigned j = j0 + 1; j <= j1; ++j) p = cmul(p, tab[j * 256u + ((F >> (56u - 8u * j)) & 255u)]);
+	return p;
+}
+
 template <typename V> __device__ __forceinline__ V rot(V x, double2 p) {
 	const double xr = x.x, xi = x.y;
 	return mk<V>(fma(xr, p.x, -xi * p.y), fma(xr, p.y, xi * p.x));
 }
+
+// ------------------------------------------------------------------------
+// QFT block pass.  Targets are the 8 bit positions pbot+7 .. pbot (MSB first,
+// as makeQft orders its target qubits); for each target t it applies H on t
+// and then every controlled phase pi/2^(t-c) from the lower bits c < t, i.e.
+// the factor e^{i pi (x mod 2^t) / 2^t} on states with bit t set.  Splitting
+// x mod 2^t into the part inside the block and the part below it, the inner
+// factors are constants c[j][v] = e^{i pi v / 2^j} and the outer ones commute
+// with the remaining H gates of the block, so they are applied last as one
+// twiddle e^{2 pi i r W(y) / 2^64} per state (r = bits below the block,
+// W(y) = sum over set block bits j of 2^(63 - pbot - j)).
+//
+// A workgroup owns 2^(8+C) states: the 8 block bits times the C contiguous
+// low bits (coalesced 2^C-state runs).  Each thread holds 16 states in
+// registers: first the 16 values of block bits 7..4 (stage A), then, after an
+// LDS transpose, the 16 values of block bits 3..0 (stage B).
+struct QftBlockParams {
+	u64 ntiles;  // 2^(n - 8 - C)
+	uint32_t n, pbot;
+};
+
+__device__ __forceinline__ double2 ld_phase(const double2 *__restrict__ p) { return *p; }
+
+template <typename V> __device__ __forceinline__ void hadamard_pair(V &x, V &y, double c) {
+	const double xr = x.x, xi = x.y, yr = y.x, yi = y.y;
+	x = mk<V>((xr + yr) * c, (xi + yi) * c);
+	y = mk<V>((xr - yr) * c, (xi - yi) * c);
+}
+
+template <typename R, int C>
+__global__ void __launch_bounds__(16 << C) k_qft_block(typename V2T<R>::t *__restrict__ a, QftBlockParams P,
+                                                        const double2 *__restrict__ ptab, const double2 *__restrict__ cin) {
+	using V = typename V2T<R>::t;
+	constexpr unsigned TILE = 256u << C;
+	extern __shared__ char smem[];
+	V *tile = (V *)smem;
+	const unsigned t = threadIdx.x;
+	const unsigned lo = t & ((1u << C) - 1u);
+	const unsigned f4 = t >> C;  // stage A: block bits 3..0; stage B: block bits 7..4
+	const double hc = 1.0 / sqrt(2.0);
+	const unsigned pb = P.pbot;
+	const u64 lowmask = (1ull << pb) - 1ull;
+	// in-block ladder constants c[j][v], j = 0..3 (offset 2^j - 1)
+	double2 cst[15];
+#pragma unroll
+	for(int i = 0; i < 15; ++i) cst[i] = cin[i];
+	// stage-A thread phases e^{i pi f4 / 2^j}, j = 4..7: top-byte table entries
+	double2 tpa[4];
+#pragma unroll
+	for(int j = 4; j < 8; ++j) tpa[j - 4] = ptab[(f4 << (7 - j)) & 255u];
+	for(u64 tid = blockIdx.x; tid < P.ntiles; tid += gridDim.x) {
+		const u64 x = tid << C;
+		const u64 base = ((x >> pb) << (pb + 8)) | (x & lowmask);
+		V v[16];
+		// ---- stage A: registers index block bits 7..4
+#pragma unroll
+		for(int ra = 0; ra < 16; ++ra) {
+			const u64 y = ((u64)ra << 4) | f4;
+			v[ra] = nt_load(&a[base | (y << pb) | lo]);
+		}
+#pragma unroll
+		for(int j = 7; j >= 4; --j) {
+			const int b = j - 4;
+#pragma unroll
+			for(int ra = 0; ra < 16; ++ra)
+				if(!(ra & (1 << b))) hadamard_pair(v[ra], v[ra | (1 << b)], hc);
+#pragma unroll
+			for(int ra = 0; ra < 16; ++ra) {
+				if(!(ra & (1 << b))) continue;
+				const int low = ra & ((1 << b) - 1);
+				const double2 ph = cmul(cst[(1 << b) - 1 + low], tpa[b]);
+				v[ra] = rot<V>(v[ra], ph);
+			}
+		}
+		__syncthreads();  // previous tile's stage-B reads are complete
+#pragma unroll
+		for(int ra = 0; ra < 16; ++ra) {
+			const unsigned l = ((((unsigned)ra << 4) | f4) << C) | lo;
+			tile[l + (l >> (4 + C))] = v[ra];
+		}
+		__syncthreads();
+		// ---- stage B: registers index block bits 3..0
+#pragma unroll
+		for(int rb = 0; rb < 16; ++rb) {
+			const unsigned l = (((f4 << 4) | (unsigned)rb) << C) | lo;
+			v[rb] = tile[l + (l >> (4 + C))];
+		}
+#pragma unroll
+		for(int j = 3; j >= 0; --j) {
+#pragma unroll
+			for(int rb = 0; rb < 16; ++rb)
+				if(!(rb & (1 << j))) hadamard_pair(v[rb], v[rb | (1 << j)], hc);
+			if(j == 0) continue;  // c[0][0] = 1
+#pragma unroll
+			for(int rb = 0; rb < 16; ++rb) {
+				if(!(rb & (1 << j))) continue;
+				const int low = rb & ((1 << j) - 1);
+				if(low == 0) continue;
+				v[rb] = rot<V>(v[rb], cst[(1 << j) - 1 + low]);
+			}
+		}
+		// ---- deferred twiddle for the bits below the block
+		const u64 r = (base & lowmask) | lo;
+		if(r != 0) {
+			u64 whi = 0;
+#pragma unroll
+			for(int j = 4; j < 8; ++j)
+				if((f4 >> (j - 4)) & 1u) whi += 1ull << (63 - pb - j);
+			double2 tw[16];
+			tw[0] = phase_of(r * whi, ptab);
+#pragma unroll
+			for(int j = 0; j < 4; ++j) {
+				const double2 phj = phase_of(r << (63 - pb - j), ptab);
+#pragma unroll
+				for(int m = 0; m < (1 << j); ++m) tw[m | (1 << j)] = cmul(tw[m], phj);
+			}
+#pragma unroll
+			for(int rb = 0; rb < 16; ++rb) v[rb] = rot<V>(v[rb], tw[rb]);
+		}
+#pragma unroll
+		for(int rb = 0; rb < 16; ++rb) {
+			const u64 y = ((u64)f4 << 4) | (unsigned)rb;
+			nt_store(v[rb], &a[base | (y << pb) | lo]);
+		}
+	}
+}
+
+hipError_t launch_qft_block(const Launch &L, void *a, unsigned n, unsigned pbot, const double *dphase, const double *dcin) {
+	QftBlockParams P;
+	P.n = n;
+	P.pbot = pbot;
+	if(L.prec == 0) {
+		constexpr int C = 4;
+		P.ntiles = 1ull << (n - 8 - C);
+		const unsigned grid = (unsigned)(P.ntiles < (1u << 20) ? P.ntiles : (1u << 20));
+		const size_t shmem = (size_t)((256u << C) + (16u)) * 16;
+		static bool attr = false;
+		if(!attr) {
+			(void)hipFuncSetAttribute((const void *)k_qft_block<double, C>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+			attr = true;
+		}
+		hipLaunchKernelGGL((k_qft_block<double, C>), dim3(grid), dim3(16 << C), shmem, L.stream, (double2 *)a, P,
+		                   (const double2 *)dphase, (const double2 *)dcin);
+	} else {
+		constexpr int C = 5;
+		P.ntiles = 1ull << (n - 8 - C);
+		const unsigned grid = (unsigned)(P.ntiles < (1u << 20) ? P.ntiles : (1u << 20));
+		const size_t shmem = (size_t)((256u << C) + (16u)) * 8;
+		static bool attr = false;
+		if(!attr) {
+			(void)hipFuncSetAttribute((const void *)k_qft_block<float, C>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+			attr = true;
+		}
+		hipLaunchKernelGGL((k_qft_block<float, C>), dim3(grid), dim3(16 << C), shmem, L.stream, (float2 *)a, P,
+		                   (const double2 *)dphase, (const double2 *)dcin);
+	}
+	return hipGetLastError();
+}
+
+unsigned qft_block_coalesce_bits(int prec) { return prec == 0 ? 4u : 5u; }
 
 template <typename R, int K>
 __device__ __forceinline__ void tile_uk(typename V2T<R>::t *tile, u64 nloc, const TileOp &op, const double *__restrict__ mats) {
@@ -770,7 +984,7 @@ __global__ void __launch_bounds__(256) k_tile(typename V2T<R>::t *__restrict__ a
 						const u64 g = base | (l & lowmask) | hitab[l >> C];
 						const u64 F = (g & op.amask) << op.shift;
 						if(F == 0) continue;
-						tile[l] = rot<V>(tile[l], phase_of(F, ptab));
+						tile[l] = rot<V>(tile[l], phase_of_range(F, ptab, op.cj0, op.cj1));
 					}
 					break;
 				}
@@ -782,7 +996,7 @@ __global__ void __launch_bounds__(256) k_tile(typename V2T<R>::t *__restrict__ a
 						for(unsigned t = 0; t < op.nterms; ++t)
 							if(g & tt[t].cond) F += (g & tt[t].mask) << tt[t].shift;
 						if(F == 0) continue;
-						tile[l] = rot<V>(tile[l], phase_of(F, ptab));
+						tile[l] = rot<V>(tile[l], phase_of_range(F, ptab, op.cj0, op.cj1));
 					}
 					break;
 				}
