@@ -3,8 +3,8 @@
 // state_backend_factory.cpp).  This library ships the device backend only:
 // Dense, Blosc and Zfp report unavailable and constructing them throws
 // std::runtime_error exactly as an unavailable codec backend does in the
-// reference.  Auto keeps the reference's resolution rule (Hip is never chosen
-// automatically).
+// reference.  Auto resolves to the device backend, the only one available in
+// this build (the reference's size rule would otherwise pick Dense).
 #include <stdexcept>
 
 #include "tmfqs/core/state_space.h"
@@ -16,6 +16,7 @@ namespace tmfqs {
 
 StorageStrategyKind StorageStrategyRegistry::resolve(unsigned int numQubits, const RegisterConfig &cfg) {
 	if(cfg.strategy != StorageStrategyKind::Auto) return cfg.strategy;
+	if(isHipStateBackendAvailable()) return StorageStrategyKind::Hip;
 	const size_t bytes = checkedAmplitudeElementCount(numQubits) * sizeof(double);
 	if(bytes < cfg.autoThresholdBytes) return StorageStrategyKind::Dense;
 	if(isAvailable(StorageStrategyKind::Blosc)) return StorageStrategyKind::Blosc;

@@ -64,7 +64,11 @@ void registryAvailability() {
 	RegisterConfig autoCfg;
 	autoCfg.strategy = StorageStrategyKind::Auto;
 	autoCfg.autoThresholdBytes = 1u;
-	HIP_TEST_REQUIRE(StorageStrategyRegistry::resolve(3, autoCfg) == StorageStrategyKind::Dense);
+	HIP_TEST_REQUIRE(StorageStrategyRegistry::resolve(3, autoCfg) == StorageStrategyKind::Hip);
+	{
+		QuantumRegister viaAuto(3, 5, autoCfg);
+		HIP_TEST_REQUIRE(viaAuto.amplitude(5).real == 1.0);
+	}
 	RegisterConfig keep = hip_test::hipConfig();
 	keep.hip.tileQubits = 9;
 	keep.hipOverrides.tileQubits = true;
