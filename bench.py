@@ -13,6 +13,10 @@ Also measured in the same run and reported beside the headline:
     probability and the Mt19937(123456) measurement;
   * complex64: QFT-30 time and fidelity against the complex128 result
     (device comparison);
+  * adjoint gradients: <Z_0> + |0><0| after 8 ring layers on 24 qubits (192
+    controlled-phase angles from mt19937(3)), one forward + one backward sweep,
+    against 384 forward circuits of plain central differences on the same GPU
+    (time) and the parameter-shift rule (accuracy), complex128 and complex64;
   * the streaming-copy ceiling of this GPU;
   * roofline of the dominant kernel from live HIP-event timing on the engine's
     stream;
@@ -26,6 +30,8 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--qubits 30]
 import argparse
 import ctypes
 import json
+
+import numpy as np
 import os
 import platform
 import sys
@@ -80,6 +86,69 @@ def time_qft(S, n, precision, steps, warmup, seed=7):
     prof = hs.profile_read()
     hs.profile(False)
     return reg, wall, exec_s, prof, cnt
+
+
+def time_adjoint(T, S, precision, n=24, layers=8, seed=3, steps=3):
+    """Adjoint gradient of <Z_0> + |0><0| for makeRingLayers(n, layers, seed)
+    against plain central differences (h = 1e-6) and the parameter-shift rule,
+    both from forward runs of the same engine on the same GPU."""
+    ops, cnt = S.ring_ops(n, layers, seed)
+    terms = [(T.DIAG_Z, 1.0, 1 << 0), (T.DIAG_PROJECTOR, 1.0, 0)]
+    params = [i for i in range(cnt) if ops[i].kind == T.OP_CPHASE]
+    out = {"qubits": n, "layers": layers, "angles": len(params), "gates": cnt,
+           "precision": "complex128" if precision == T.C128 else "complex64"}
+    with T.HipState(precision=precision) as s:
+        s.init_basis(n, 0)
+        s.adjoint(ops, cnt, terms)  # warmup: allocates the checkpoint buffers
+        times = []
+        for k in range(steps):
+            s.init_basis(n, 0)
+            s.synchronize()
+            if k == steps - 1:
+                s.profile(True)
+                s.profile_reset()
+            t0 = time.perf_counter()
+            e, g = s.adjoint(ops, cnt, terms)
+            times.append(time.perf_counter() - t0)
+        prof = s.profile_read()
+        s.profile(False)
+        adj_s = sorted(times)[len(times) // 2]
+
+        def energy():
+            s.init_basis(n, 0)
+            s.run(ops, cnt)
+            return s.expectation(terms)
+
+        def two_point(delta):
+            vals = []
+            for i in params:
+                th = ops[i].theta
+                ops[i].theta = th + delta
+                ep = energy()
+                ops[i].theta = th - delta
+                em = energy()
+                ops[i].theta = th
+                vals.append((ep, em))
+            return vals
+
+        t0 = time.perf_counter()
+        fd = np.array([(a - b) / 2e-6 for a, b in two_point(1e-6)])
+        fd_s = time.perf_counter() - t0
+        ps = np.array([0.5 * (a - b) for a, b in two_point(np.pi / 2)])
+    out.update({
+        "expectation": e,
+        "adjoint_s": adj_s,
+        "adjoint_runs_s": times,
+        "grads_per_s": len(params) / adj_s,
+        "fd_s": fd_s,
+        "fd_forward_circuits": 2 * len(params),
+        "fd_grads_per_s": len(params) / fd_s,
+        "speedup_vs_fd": fd_s / adj_s,
+        "max_abs_diff_vs_param_shift": float(np.max(np.abs(g - ps))),
+        "max_abs_diff_vs_central_fd": float(np.max(np.abs(g - fd))),
+        "kernels": prof,
+    })
+    return out, g
 
 
 def cpu_baseline(n):
@@ -137,6 +206,7 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-grover", action="store_true")
     ap.add_argument("--skip-c64", action="store_true")
+    ap.add_argument("--skip-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); the state-vector engine runs on "
@@ -225,7 +295,8 @@ def main():
         a = greg.amplitude(marked)
         meas = greg.measure_seed(123456)
         iters = (cnt - n) // 2
-        sweeps = sum(v["launches"] for k, v in gprof.items() if k in ("k_ew_chain", "k_tile", "k_hadamard"))
+        sweeps = sum(v["launches"] for k, v in gprof.items()
+                     if k in ("k_diffuse_chain", "k_ew_chain", "k_tile", "k_hadamard"))
         gbytes = sum(v["bytes"] for v in gprof.values())
         gms = sum(v["ms"] for v in gprof.values())
         result["grover"] = {
@@ -245,6 +316,19 @@ def main():
         }
         log(f"Grover-{n}: {gsecs:.3f} s, P={abs(a) ** 2:.15f}, measured {meas}")
         del greg, gh
+
+    # ---- adjoint gradients: n = 24, 8 ring layers, 192 angles
+    if not args.skip_adjoint:
+        log("adjoint n=24 complex128 (and central differences / parameter shift on the same GPU)")
+        adj, g128 = time_adjoint(T, S, T.C128)
+        log(f"adjoint: {adj['adjoint_s'] * 1e3:.2f} ms, FD {adj['fd_s']:.2f} s, speedup {adj['speedup_vs_fd']:.1f}x, "
+            f"|g - shift| {adj['max_abs_diff_vs_param_shift']:.2e}")
+        if not args.skip_c64:
+            adj64, g64 = time_adjoint(T, S, T.C64, steps=3)
+            adj64["max_abs_diff_vs_complex128_adjoint"] = float(np.max(np.abs(g64 - g128)))
+            adj64["speedup_vs_complex128_adjoint"] = adj["adjoint_s"] / adj64["adjoint_s"]
+            adj["complex64"] = adj64
+        result["adjoint"] = adj
 
     cpu = None
     if not args.skip_cpu:

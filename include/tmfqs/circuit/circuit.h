@@ -73,6 +73,12 @@ class Circuit {
 
 std::string operationName(const Operation &operation);
 
+/** Layered ring circuit of the adjoint benchmark (no reference counterpart):
+ *  `layers` x (H on every qubit, then controlledPhase(i, (i+1) mod n, theta)
+ *  for i = 0..n-1), theta drawn from std::mt19937(seed) with
+ *  std::uniform_real_distribution<double>(0, 2 acos(-1)) in order n * l + i. */
+Circuit makeRingLayers(unsigned int qubits, unsigned int layers, unsigned int seed);
+
 void appendQft(Circuit &circuit);
 Circuit makeQft(unsigned int qubits);
 

@@ -67,6 +67,10 @@ class QuantumRegister {
 		IStateBackend *backend() const noexcept { return backend_.get(); }
 		/** True when no lazy affine overlay is pending (stored == logical amplitudes). */
 		bool overlayIsIdentity() const noexcept { return !affineOverlayActive_; }
+		/** Hands the backend to an operation outside the register's gate set (the
+		 *  adjoint gradient): flushes a pending overlay and drops the cached
+		 *  amplitude sum, since the caller is about to change the state. */
+		IStateBackend *beginExternalUpdate();
 
 	private:
 		unsigned int numQubits_ = 0;

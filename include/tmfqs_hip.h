@@ -77,7 +77,9 @@ typedef enum tmfqs_hip_op_kind {
 	TMFQS_HIP_OP_PHASE_FLIP = 5,
 	TMFQS_HIP_OP_DIFFUSE_LAZY = 6,
 	TMFQS_HIP_OP_DIFFUSE_MATERIALIZED = 7,
-	TMFQS_HIP_OP_MATRIX = 8
+	TMFQS_HIP_OP_MATRIX = 8,
+	TMFQS_HIP_OP_PHASE = 9  /* diag(1, e^{i theta}) on qubit q0; no circuit::Operation counterpart
+	                           (the reference applies it as QuantumGate::Phase through applyGate) */
 } tmfqs_hip_op_kind;
 
 typedef struct tmfqs_hip_op {
@@ -210,6 +212,50 @@ int tmfqs_hip_profile_read(tmfqs_hip_state *h, char (*names)[64], uint64_t *laun
 int tmfqs_hip_copy_probe(tmfqs_hip_state *h, int reps, double *ms);
 /* Raw stream handle (hipStream_t) for callers that time with their own events. */
 int tmfqs_hip_stream(tmfqs_hip_state *h, void **stream);
+
+/* ---- operation lists -------------------------------------------------- */
+/* Enqueues a list of operation records, as sim::Simulator::executeOperation
+ * (src/tmfqs/sim/simulator.cpp:17-51) dispatches them to the backend.  MATRIX
+ * records take their qubits from qubit_pool[q0 ..] and their row-major
+ * interleaved matrix from mat_pool[state ..].  PHASE_FLIP calls
+ * phaseFlipBasisState; both DIFFUSE kinds call the backend's
+ * inversionAboutMean() (register-level laziness lives above this layer). */
+int tmfqs_hip_run(tmfqs_hip_state *h, const tmfqs_hip_op *ops, size_t count, const double *mat_pool, const unsigned *qubit_pool);
+
+/* ---- diagonal observables and adjoint differentiation ----------------- */
+/* No counterpart in the reference (its experiments only compare states).
+ * A diagonal observable is a sum of terms: Z-strings coef * prod_{q in bits}
+ * Z_q (bits = mask of qubits in the reference numbering; Z_q is +1 when qubit
+ * q is 0) and basis projectors coef * |bits><bits|.  At most 16 terms of
+ * each kind. */
+typedef enum tmfqs_hip_diag_kind {
+	TMFQS_HIP_DIAG_Z = 0,
+	TMFQS_HIP_DIAG_PROJECTOR = 1
+} tmfqs_hip_diag_kind;
+
+typedef struct tmfqs_hip_diag_term {
+	int32_t kind;       /* tmfqs_hip_diag_kind */
+	uint32_t reserved;
+	double coef;
+	uint64_t bits;      /* Z: qubit mask; PROJECTOR: basis state */
+} tmfqs_hip_diag_term;
+
+/* <psi|O|psi> of the current state (FP64 accumulation in both precisions). */
+int tmfqs_hip_expectation(tmfqs_hip_state *h, const tmfqs_hip_diag_term *terms, size_t nterms, double *out);
+
+/* Adjoint gradient: runs the gate list (H, X, CX, CPHASE, PHASE, SWAP,
+ * MATRIX) forward from the handle's current state, leaving the final state in
+ * the handle, and returns E = <psi|O|psi> and dE/dtheta for every CPHASE and
+ * PHASE record, in record order (grad_count must equal their number).  The
+ * forward sweep stores the state at a pass boundary roughly every
+ * checkpoint_gates gates (0: ceil(sqrt(count))); the backward sweep restarts
+ * psi from each checkpoint and steps psi and lambda = O psi back through the
+ * inverse gates on two device buffers, reducing -2 Im <lambda|P|psi> per
+ * parameter per wavefront.  Extra device memory: one state per checkpoint plus
+ * one (fewer checkpoints are taken when HBM is short). */
+int tmfqs_hip_adjoint(tmfqs_hip_state *h, const tmfqs_hip_op *ops, size_t count, const double *mat_pool, const unsigned *qubit_pool,
+                      const tmfqs_hip_diag_term *terms, size_t nterms, unsigned checkpoint_gates, double *expval, double *grads,
+                      size_t grad_count);
 
 #ifdef __cplusplus
 }

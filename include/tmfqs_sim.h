@@ -73,6 +73,18 @@ int tmfqs_circuit_qft(unsigned n, tmfqs_hip_op *out, size_t cap, size_t *count);
 int tmfqs_circuit_grover(unsigned n, const uint64_t *marked, size_t m, int materialized, tmfqs_hip_op *out, size_t cap,
                          size_t *count);
 
+/* circuit::makeRingLayers(n, layers, seed): the adjoint benchmark circuit. */
+int tmfqs_circuit_ring(unsigned n, unsigned layers, unsigned seed, tmfqs_hip_op *out, size_t cap, size_t *count);
+
+/* sim::adjointGradient on this register (ops as for tmfqs_reg_run, gate
+ * kinds only); grads receives one value per CPHASE op (grad_count of them);
+ * seconds receives AdjointResult.executionSeconds. */
+int tmfqs_reg_adjoint(tmfqs_reg *r, const tmfqs_hip_op *ops, size_t count, const double *mat_pool, const unsigned *mat_qubits,
+                      const tmfqs_hip_diag_term *terms, size_t nterms, unsigned checkpoint_gates, double *expval,
+                      double *grads, size_t grad_count, double *seconds);
+/* sim::expectation of the register's state. */
+int tmfqs_reg_expectation(tmfqs_reg *r, const tmfqs_hip_diag_term *terms, size_t nterms, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,6 +13,7 @@
 #include "tmfqs/experiment/report.h"
 #include "tmfqs/gates/quantum_gate.h"
 #include "tmfqs/register/quantum_register.h"
+#include "tmfqs/sim/adjoint.h"
 #include "tmfqs/sim/simulator.h"
 #include "tmfqs/state/quantum_state.h"
 #include "tmfqs/storage/factory/state_backend_factory.h"

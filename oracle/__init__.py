@@ -24,6 +24,13 @@ Pins (what checks each oracle function; the tests in ``tests/test_oracle_*.py``)
   build (bit-exact amplitudes and measured index); golden vectors.
 * register semantics (cached sum, lazy affine overlay, measure): the
   reference build and its 8-amplitude fixed vector test.
+* ``expectation`` (diagonal observables) and ``ring_ops`` (the adjoint
+  benchmark circuit; no reference counterpart): closed forms (one layer at
+  theta = 0 gives |+>^n, two give |0>), a numpy restatement of the
+  observable, and the gradient checks below agreeing with each other.
+* ``param_shift_grad`` / ``central_fd_grad``: the exact two-term shift rule
+  for generators with eigenvalues {0, 1} and FP64 central differences agree
+  to O(h^2) on random circuits.
 """
 
 import ctypes
@@ -37,7 +44,17 @@ REF_PATH = os.path.join(HERE, "_ref", "libtmfqs_ref.so")
 
 OK, OUT_OF_RANGE, INVALID_ARGUMENT, LOGIC_ERROR, RUNTIME_ERROR = 0, -1, -2, -3, -4
 
-OP_H, OP_X, OP_CX, OP_CPHASE, OP_SWAP, OP_PHASE_FLIP, OP_DIFFUSE_LAZY, OP_DIFFUSE_MAT, OP_MATRIX = range(9)
+OP_H, OP_X, OP_CX, OP_CPHASE, OP_SWAP, OP_PHASE_FLIP, OP_DIFFUSE_LAZY, OP_DIFFUSE_MAT, OP_MATRIX, OP_PHASE = range(10)
+DIAG_Z, DIAG_PROJECTOR = 0, 1
+
+
+class DiagTerm(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("reserved", ctypes.c_uint32),
+        ("coef", ctypes.c_double),
+        ("bits", ctypes.c_uint64),
+    ]
 
 
 class Op(ctypes.Structure):
@@ -100,6 +117,9 @@ def lib():
         L.orc_reg_inversion_materialized.argtypes = [ctypes.c_void_p]
         L.orc_reg_amplitude.argtypes = [ctypes.c_void_p, ctypes.c_uint64, _dp, _dp]
         L.orc_reg_create.argtypes = [ctypes.c_uint]
+        L.orc_expectation.argtypes = [_dp, ctypes.c_uint, ctypes.POINTER(DiagTerm), ctypes.c_size_t, _dp]
+        L.orc_ring_ops.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.c_uint32, ctypes.POINTER(Op)]
+        L.orc_ring_ops.restype = ctypes.c_size_t
         _lib = L
     return _lib
 
@@ -281,6 +301,67 @@ def sample(amps, n, rnd):
 def total_probability(amps, n):
     a = np.ascontiguousarray(amps, dtype=np.float64)
     return lib().orc_total_probability(_dptr(a), ctypes.c_uint(n))
+
+
+def make_terms(terms):
+    """[(kind, coef, bits), ...] -> ctypes array of DiagTerm."""
+    arr = (DiagTerm * max(1, len(terms)))()
+    for i, (kind, coef, bits) in enumerate(terms):
+        arr[i].kind = kind
+        arr[i].coef = coef
+        arr[i].bits = bits
+    return arr, len(terms)
+
+
+def expectation(amps, n, terms):
+    arr, cnt = make_terms(terms)
+    a = np.ascontiguousarray(amps, dtype=np.float64)
+    out = ctypes.c_double()
+    _check(lib().orc_expectation(_dptr(a), ctypes.c_uint(n), arr, ctypes.c_size_t(cnt), ctypes.byref(out)), "expectation")
+    return out.value
+
+
+def ring_ops(n, layers, seed):
+    cnt = 2 * n * layers
+    arr = (Op * max(1, cnt))()
+    got = lib().orc_ring_ops(ctypes.c_uint(n), ctypes.c_uint(layers), ctypes.c_uint32(seed), arr)
+    assert got == cnt
+    return arr, cnt
+
+
+def param_indices(ops, cnt):
+    return [i for i in range(cnt) if ops[i].kind in (OP_CPHASE, OP_PHASE)]
+
+
+def energy(n, ops, cnt, terms, init=None, mat_pool=None, mat_qubits=None):
+    reg = run_circuit(n, (ops, cnt), init=init, mat_pool=mat_pool, mat_qubits=mat_qubits)
+    return expectation(reg.amplitudes(), n, terms)
+
+
+def _shifted(ops, cnt, i, delta):
+    cp = (Op * max(1, cnt))()
+    ctypes.memmove(cp, ops, ctypes.sizeof(Op) * cnt)
+    cp[i].theta = ops[i].theta + delta
+    return cp
+
+
+def param_shift_grad(n, ops, cnt, terms, init=None, mat_pool=None, mat_qubits=None):
+    """dE/dtheta for every CPHASE/PHASE op: (E(theta + pi/2) - E(theta - pi/2)) / 2."""
+    out = []
+    for i in param_indices(ops, cnt):
+        ep = energy(n, _shifted(ops, cnt, i, np.pi / 2), cnt, terms, init, mat_pool, mat_qubits)
+        em = energy(n, _shifted(ops, cnt, i, -np.pi / 2), cnt, terms, init, mat_pool, mat_qubits)
+        out.append(0.5 * (ep - em))
+    return np.array(out)
+
+
+def central_fd_grad(n, ops, cnt, terms, h=1e-6, init=None, mat_pool=None, mat_qubits=None):
+    out = []
+    for i in param_indices(ops, cnt):
+        ep = energy(n, _shifted(ops, cnt, i, h), cnt, terms, init, mat_pool, mat_qubits)
+        em = energy(n, _shifted(ops, cnt, i, -h), cnt, terms, init, mat_pool, mat_qubits)
+        out.append((ep - em) / (2 * h))
+    return np.array(out)
 
 
 # ------------------------------------------------------- reference build

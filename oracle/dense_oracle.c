@@ -668,6 +668,13 @@ int orc_reg_run(orc_register *r, const orc_op *ops, size_t count, const double *
 			case ORC_OP_MATRIX:
 				rc = orc_reg_apply_gate(r, mat_pool + op->state, op->k, mat_qubits + op->q0);
 				break;
+			case ORC_OP_PHASE: {
+				/* quantum_gate.cpp:162-169: [[1, 0], [0, complexExp(i theta)]] */
+				const double m[8] = {1.0, 0.0, 0.0, 0.0, 0.0, 0.0, cos(op->theta), sin(op->theta)};
+				const unsigned q = op->q0;
+				rc = orc_reg_apply_gate(r, m, 1, &q);
+				break;
+			}
 			default: rc = ORC_INVALID_ARGUMENT; break;
 		}
 		if(rc) return rc;
@@ -779,4 +786,63 @@ int orc_time_ops(unsigned n, const orc_op *ops, size_t count, double *seconds) {
 	*seconds = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
 	orc_reg_free(r);
 	return rc;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Diagonal observable expectation and the layered ring circuit (no
+ * counterpart in the reference; they define the adjoint benchmark). */
+int orc_expectation(const double *a, unsigned n, const orc_diag_term *terms, size_t nterms, double *out) {
+	if(n > 62) return ORC_INVALID_ARGUMENT;
+	const uint64_t N = 1ull << n;
+	for(size_t t = 0; t < nterms; ++t) {
+		if(terms[t].kind == 0) {
+			if(n < 64 && (terms[t].bits >> n) != 0) return ORC_OUT_OF_RANGE;
+		} else if(terms[t].kind == 1) {
+			if(terms[t].bits >= N) return ORC_OUT_OF_RANGE;
+		} else {
+			return ORC_INVALID_ARGUMENT;
+		}
+	}
+	double e = 0.0;
+	for(uint64_t x = 0; x < N; ++x) {
+		double o = 0.0;
+		for(size_t t = 0; t < nterms; ++t) {
+			if(terms[t].kind == 0) {
+				int parity = 0;
+				for(unsigned q = 0; q < n; ++q)
+					if((terms[t].bits >> q) & 1u) parity ^= (int)((x >> (n - 1 - q)) & 1u);
+				o += parity ? -terms[t].coef : terms[t].coef;
+			} else if(x == terms[t].bits) {
+				o += terms[t].coef;
+			}
+		}
+		const double re = a[2 * x], im = a[2 * x + 1];
+		e += o * (re * re + im * im);
+	}
+	*out = e;
+	return ORC_OK;
+}
+
+size_t orc_ring_ops(unsigned n, unsigned layers, uint32_t seed, orc_op *out) {
+	orc_mt19937 g;
+	orc_mt_seed(&g, seed);
+	const double hi = 2.0 * acos(-1.0);
+	size_t c = 0;
+	for(unsigned l = 0; l < layers; ++l) {
+		for(unsigned q = 0; q < n; ++q) {
+			orc_op *o = &out[c++];
+			memset(o, 0, sizeof *o);
+			o->kind = ORC_OP_H;
+			o->q0 = q;
+		}
+		for(unsigned i = 0; i < n; ++i) {
+			orc_op *o = &out[c++];
+			memset(o, 0, sizeof *o);
+			o->kind = ORC_OP_CPHASE;
+			o->q0 = i;
+			o->q1 = (i + 1) % n;
+			o->theta = orc_mt_uniform(&g, 0.0, hi);
+		}
+	}
+	return c;
 }

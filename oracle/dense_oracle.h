@@ -80,7 +80,8 @@ enum {
 	ORC_OP_PHASE_FLIP = 5,
 	ORC_OP_DIFFUSE_LAZY = 6,
 	ORC_OP_DIFFUSE_MATERIALIZED = 7,
-	ORC_OP_MATRIX = 8
+	ORC_OP_MATRIX = 8,
+	ORC_OP_PHASE = 9  /* QuantumGate::PhaseShift(theta) on qubit q0 through applyGate */
 };
 
 typedef struct {
@@ -138,6 +139,21 @@ unsigned orc_grover_iterations(uint64_t state_count, size_t marked_count);
 long long orc_grover_ops(unsigned n, const uint64_t *marked, size_t m, int materialized, orc_op *out, size_t cap);
 /* Wall seconds of ops on a uniform-amplitude n-qubit register (CPU-baseline timing). */
 int orc_time_ops(unsigned n, const orc_op *ops, size_t count, double *seconds);
+
+/* ---- diagonal observables and the adjoint benchmark circuit ---- */
+/* O = sum of Z-strings coef * prod_{q in bits} Z_q (Z_q = +1 when qubit q is 0)
+ * and projectors coef * |bits><bits|; *out = sum_x O(x) |a_x|^2 in index order. */
+typedef struct {
+	int32_t kind; /* 0: Z-string (bits = qubit mask), 1: projector (bits = basis state) */
+	uint32_t reserved;
+	double coef;
+	uint64_t bits;
+} orc_diag_term;
+int orc_expectation(const double *a, unsigned n, const orc_diag_term *terms, size_t nterms, double *out);
+/* `layers` x (H on every qubit, then controlledPhase(i, (i+1) mod n, theta) for
+ * i = 0..n-1), theta from mt19937(seed) uniform [0, 2 acos(-1)) in order
+ * n * l + i.  Writes 2 n layers ops; n >= 2. */
+size_t orc_ring_ops(unsigned n, unsigned layers, uint32_t seed, orc_op *out);
 
 #ifdef __cplusplus
 }

@@ -28,7 +28,8 @@ ERR_RUNTIME = -4
 ERR_OUT_OF_MEMORY = -5
 ERR_DEVICE = -6
 
-OP_H, OP_X, OP_CX, OP_CPHASE, OP_SWAP, OP_PHASE_FLIP, OP_DIFFUSE_LAZY, OP_DIFFUSE_MAT, OP_MATRIX = range(9)
+OP_H, OP_X, OP_CX, OP_CPHASE, OP_SWAP, OP_PHASE_FLIP, OP_DIFFUSE_LAZY, OP_DIFFUSE_MAT, OP_MATRIX, OP_PHASE = range(10)
+DIAG_Z, DIAG_PROJECTOR = 0, 1
 
 
 class Config(ctypes.Structure):
@@ -52,6 +53,32 @@ class Op(ctypes.Structure):
         ("theta", ctypes.c_double),
         ("state", ctypes.c_uint64),
     ]
+
+
+class DiagTerm(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("reserved", ctypes.c_uint32),
+        ("coef", ctypes.c_double),
+        ("bits", ctypes.c_uint64),
+    ]
+
+
+def make_terms(terms):
+    """[(kind, coef, bits), ...] -> (ctypes DiagTerm array, count)."""
+    arr = (DiagTerm * max(1, len(terms)))()
+    for i, (kind, coef, bits) in enumerate(terms):
+        arr[i].kind = kind
+        arr[i].coef = coef
+        arr[i].bits = bits
+    return arr, len(terms)
+
+
+def _pools(mat_pool, qubit_pool):
+    mp = None if mat_pool is None else np.ascontiguousarray(mat_pool, dtype=np.float64)
+    qp = None if qubit_pool is None else np.ascontiguousarray(qubit_pool, dtype=np.uint32)
+    return (mp, qp, None if mp is None else mp.ctypes.data_as(_dp),
+            None if qp is None else qp.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)))
 
 
 class Comparison(ctypes.Structure):
@@ -141,6 +168,11 @@ def lib():
         L.tmfqs_hip_copy_probe.argtypes = [vp, ctypes.c_int, _dp]
         L.tmfqs_hip_compare.argtypes = [vp, vp, ctypes.POINTER(Comparison)]
         L.tmfqs_hip_qubit_count.argtypes = [vp, ctypes.POINTER(ctypes.c_uint)]
+        L.tmfqs_hip_run.argtypes = [vp, ctypes.POINTER(Op), ctypes.c_size_t, _dp, ctypes.POINTER(ctypes.c_uint)]
+        L.tmfqs_hip_expectation.argtypes = [vp, ctypes.POINTER(DiagTerm), ctypes.c_size_t, _dp]
+        L.tmfqs_hip_adjoint.argtypes = [vp, ctypes.POINTER(Op), ctypes.c_size_t, _dp, ctypes.POINTER(ctypes.c_uint),
+                                        ctypes.POINTER(DiagTerm), ctypes.c_size_t, ctypes.c_uint, _dp, _dp,
+                                        ctypes.c_size_t]
         _lib = L
     return _lib
 
@@ -361,6 +393,29 @@ class HipState:
         s = ctypes.c_void_p()
         check(lib().tmfqs_hip_stream(self._h, ctypes.byref(s)))
         return s.value
+
+    # ---- operation lists, observables, adjoint gradients
+    def run(self, ops, count, mat_pool=None, qubit_pool=None):
+        """tmfqs_hip_run: enqueue op records (see include/tmfqs_hip.h)."""
+        mp, qp, mpp, qpp = _pools(mat_pool, qubit_pool)
+        check(lib().tmfqs_hip_run(self._h, ops, count, mpp, qpp))
+
+    def expectation(self, terms):
+        arr, cnt = make_terms(terms)
+        out = ctypes.c_double()
+        check(lib().tmfqs_hip_expectation(self._h, arr, cnt, ctypes.byref(out)))
+        return out.value
+
+    def adjoint(self, ops, count, terms, checkpoint_gates=0, mat_pool=None, qubit_pool=None):
+        """Runs the ops from the current state; returns (E, dE/dtheta per CPHASE/PHASE op)."""
+        nparams = sum(1 for i in range(count) if ops[i].kind in (OP_CPHASE, OP_PHASE))
+        grads = np.zeros(max(1, nparams))
+        arr, cnt = make_terms(terms)
+        mp, qp, mpp, qpp = _pools(mat_pool, qubit_pool)
+        e = ctypes.c_double()
+        check(lib().tmfqs_hip_adjoint(self._h, ops, count, mpp, qpp, arr, cnt, checkpoint_gates, ctypes.byref(e),
+                                      grads.ctypes.data_as(_dp), nparams))
+        return e.value, grads[:nparams]
 
     # ---- convenience: run a list of (kind, q0, q1, theta, state) ops with the
     # backend semantics (swap as one permutation; diffusion needs the mean)

@@ -74,7 +74,9 @@ int map_exception() {
 }
 
 // Queued operation; bit arguments are physical bit positions (p = n-1-q).
-enum QKind { Q_H, Q_X, Q_CX, Q_CP, Q_SWAP, Q_U1, Q_UK, Q_FLIP, Q_SET, Q_DIFF, Q_AFFINE, Q_SCATTER };
+enum QKind { Q_H, Q_X, Q_CX, Q_CP, Q_SWAP, Q_U1, Q_UK, Q_FLIP, Q_SET, Q_DIFF, Q_AFFINE, Q_SCATTER,
+             Q_PH,     // phase (cr, ci) on states with bit b0 set
+             Q_GRAD };  // adjoint only: gradient reduction over states with bit b0 (and b1 when k == 2) set; slot = state
 
 struct QOp {
 	int kind;
@@ -147,6 +149,12 @@ struct tmfqs_hip_state {
 	std::vector<ProfRec> recs;
 	std::vector<hipEvent_t> event_pool;
 	std::map<std::string, ProfAgg> prof;
+	// adjoint: checkpoint/lambda buffers (kept between calls), gradient partials and outputs
+	std::vector<void *> adj_bufs;
+	size_t adj_buf_bytes = 0;
+	double *d_gpart = nullptr;
+	double *d_grads = nullptr;
+	size_t grads_cap = 0;
 	Launch L() const { return Launch{stream, prec}; }
 };
 
@@ -345,7 +353,17 @@ void allocate_state(tmfqs_hip_state *h, unsigned n) {
 
 // ------------------------------------------------------------ execution
 void exec_op_unfused(tmfqs_hip_state *h, const QOp &op);
-void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &ops, size_t lo, size_t hi);
+// Where exec_gates applies a gate run: the handle's state by default, or two
+// states stepped through the same ops with gradient reductions (adjoint).
+struct GateTarget {
+	void *a = nullptr;
+	void *b = nullptr;         // second state (dual passes), or null
+	int tile_delta = 0;        // tile bits relative to the handle's default
+	double *gpart = nullptr;   // dual: kMaxGradSlots * tile_dual_grid partials
+	double *dgrads = nullptr;  // dual: gradient outputs (device), indexed by Q_GRAD's slot
+	double grad_scale = 1.0;
+};
+void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &ops, size_t lo, size_t hi, const GateTarget *tgt = nullptr);
 void exec_elementwise(tmfqs_hip_state *h, const std::vector<QOp> &ops, size_t lo, size_t hi);
 
 bool is_elementwise(int kind) {
@@ -556,6 +574,12 @@ void exec_op_unfused(tmfqs_hip_state *h, const QOp &op) {
 			HIPC(launch_uk(L, h->amp, h->n, op.k, op.bits, d));
 			break;
 		}
+		case Q_PH: {
+			const double m[8] = {1.0, 0.0, 0.0, 0.0, 0.0, 0.0, op.cr, op.ci};
+			ProfScope ps(h, "k_u1", pb);
+			HIPC(launch_u1(L, h->amp, h->n, op.b0, m));
+			break;
+		}
 		default: break;
 	}
 }
@@ -641,7 +665,7 @@ bool match_qft_block(const std::vector<QOp> &ops, const std::vector<size_t> &idx
 	return pbot >= C;
 }
 
-void run_permutation(tmfqs_hip_state *h, std::vector<unsigned> map) {
+void run_permutation(tmfqs_hip_state *h, std::vector<unsigned> map, void *buf) {
 	// Data of nominal bit b sits at physical map[b]; move it back to b.
 	const unsigned n = h->n;
 	std::vector<unsigned> pi(n);
@@ -657,7 +681,7 @@ void run_permutation(tmfqs_hip_state *h, std::vector<unsigned> map) {
 	const unsigned C = h->prec == 0 ? 4u : 5u;
 	if(invol && n >= 2 * C + 1) {
 		ProfScope ps(h, "k_involution", pass_bytes(h));
-		HIPC(launch_involution(h->L(), h->amp, n, pi.data(), C));
+		HIPC(launch_involution(h->L(), buf, n, pi.data(), C));
 		return;
 	}
 	// General permutation: transpositions, one in-place pass each.
@@ -669,7 +693,7 @@ void run_permutation(tmfqs_hip_state *h, std::vector<unsigned> map) {
 		const unsigned p = cur[b];
 		const unsigned other = who[b];
 		ProfScope ps(h, "k_bitswap", pass_bytes(h) / 2);
-		HIPC(launch_bitswap(h->L(), h->amp, n, p, b));
+		HIPC(launch_bitswap(h->L(), buf, n, p, b));
 		who[p] = other;
 		cur[other] = p;
 		who[b] = b;
@@ -677,8 +701,12 @@ void run_permutation(tmfqs_hip_state *h, std::vector<unsigned> map) {
 	}
 }
 
-void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_t hi) {
+void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_t hi, const GateTarget *tgt) {
 	const unsigned n = h->n;
+	void *const A = tgt ? tgt->a : h->amp;
+	void *const B = tgt ? tgt->b : nullptr;
+	const bool dual = B != nullptr;
+	const bool ladders = !h->cfg.exact_phases && !dual;
 	// 1. relabel swaps into a bit map; op bits become physical positions.
 	std::vector<unsigned> map(n);
 	for(unsigned b = 0; b < n; ++b) map[b] = b;
@@ -696,7 +724,8 @@ void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_
 		ops.push_back(op);
 	}
 	// 2. greedy grouping into passes.
-	const unsigned T = tile_bits(h);
+	unsigned T = tile_bits(h);
+	if(tgt && tgt->tile_delta) T = (unsigned)std::max<int>(1, std::min<int>((int)n, (int)T + tgt->tile_delta));
 	const unsigned C = coalesce_bits(h, T);
 	const u64 lowC = (C >= 64) ? ~0ull : ((1ull << C) - 1ull);
 	std::vector<PassPlan> passes;
@@ -716,9 +745,9 @@ void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_
 	// 3. emit each pass as a tile kernel.
 	for(const PassPlan &P : passes) {
 		unsigned pbot = 0;
-		if(!h->cfg.exact_phases && n <= 40 && match_qft_block(ops, P.ops, qft_block_coalesce_bits(h->prec), pbot)) {
+		if(ladders && n <= 40 && match_qft_block(ops, P.ops, qft_block_coalesce_bits(h->prec), pbot)) {
 			ProfScope ps(h, "k_qft_block", pass_bytes(h));
-			HIPC(launch_qft_block(h->L(), h->amp, n, pbot, h->d_phase, h->d_cin));
+			HIPC(launch_qft_block(h->L(), A, n, pbot, h->d_phase, h->d_cin));
 			continue;
 		}
 		unsigned Cp = T;
@@ -795,7 +824,7 @@ void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_
 				}
 				case Q_CP: {
 					unsigned hb, lb;
-					if(!h->cfg.exact_phases && is_qft_ladder_cp(op, hb, lb)) {
+					if(ladders && is_qft_ladder_cp(op, hb, lb)) {
 						// maximal run of ladder CPs sharing the high bit
 						u64 amask = 0;
 						size_t e = k;
@@ -831,6 +860,24 @@ void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_
 					}
 					break;
 				}
+				case Q_PH:
+					t.kind = TOP_CP;
+					add_cond(t, op.b0);
+					t.cr = op.cr;
+					t.ci = op.ci;
+					tops.push_back(t);
+					++k;
+					break;
+				case Q_GRAD:
+					if(dual) {
+						t.kind = TOP_GRAD;
+						add_cond(t, op.b0);
+						if(op.k == 2) add_cond(t, op.b1);
+						t.twid = (uint32_t)op.state;  // global slot; t.mat gets the launch-local slot
+						tops.push_back(t);
+					}
+					++k;
+					break;
 				default: ++k; break;
 			}
 		}
@@ -863,21 +910,324 @@ void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_
 		tp.nh = (unsigned)high.size();
 		for(size_t i = 0; i < high.size(); ++i) tp.hbits[i] = high[i];
 		tp.ntiles = 1ull << (n - Tp);
-		// op lists longer than kMaxTileOps are split across launches.
-		for(size_t s = 0; s < tops.size(); s += kMaxTileOps) {
-			const size_t cnt = std::min<size_t>(kMaxTileOps, tops.size() - s);
-			const TileOp *dops = (const TileOp *)arena_upload(h, tops.data() + s, cnt * sizeof(TileOp));
+		// op lists longer than kMaxTileOps (or with more than kMaxGradSlots
+		// gradient reductions) are split across launches.
+		size_t s0 = 0;
+		while(s0 < tops.size()) {
+			size_t e = s0;
+			std::vector<uint32_t> gidx;
+			while(e < tops.size() && e - s0 < (size_t)kMaxTileOps) {
+				if(tops[e].kind == TOP_GRAD) {
+					if(gidx.size() == kMaxGradSlots) break;
+					tops[e].mat = (uint32_t)gidx.size();
+					gidx.push_back(tops[e].twid);
+				}
+				++e;
+			}
+			const size_t cnt = e - s0;
+			const TileOp *dops = (const TileOp *)arena_upload(h, tops.data() + s0, cnt * sizeof(TileOp));
 			const double *dm = mats.empty() ? nullptr : (const double *)arena_upload(h, mats.data(), mats.size() * sizeof(double));
 			const TwiddleTerm *dt =
 			    allterms.empty() ? nullptr : (const TwiddleTerm *)arena_upload(h, allterms.data(), allterms.size() * sizeof(TwiddleTerm));
 			tp.nops = (unsigned)cnt;
 			tp.op0 = 0;
-			ProfScope ps(h, "k_tile", pass_bytes(h));
-			HIPC(launch_tile_pass(h->L(), h->amp, tp, dops, dm, dt, h->d_phase));
+			if(dual) {
+				{
+					ProfScope ps(h, "k_tile_dual", 2 * pass_bytes(h));
+					HIPC(launch_tile_pass_dual(h->L(), A, B, tp, dops, dm, dt, h->d_phase, tgt->gpart, (unsigned)gidx.size()));
+				}
+				if(!gidx.empty()) {
+					const uint32_t *dg = (const uint32_t *)arena_upload(h, gidx.data(), gidx.size() * sizeof(uint32_t));
+					HIPC(launch_grad_finish(h->L(), tgt->gpart, tile_dual_grid(tp), (unsigned)gidx.size(), dg, tgt->grad_scale,
+					                        tgt->dgrads));
+				}
+			} else {
+				ProfScope ps(h, "k_tile", pass_bytes(h));
+				HIPC(launch_tile_pass(h->L(), A, tp, dops, dm, dt, h->d_phase));
+			}
+			s0 = e;
 		}
 	}
 	// 4. restore the identity bit layout.
-	run_permutation(h, map);
+	run_permutation(h, map, A);
+	if(dual) run_permutation(h, map, B);
+}
+
+// ------------------------------------------------------------ op records
+// tmfqs_hip_op -> queued ops on physical bits, with the reference's argument
+// checks; records the index of every parameterized op (CPHASE, PHASE).
+std::vector<QOp> convert_ops(tmfqs_hip_state *h, const tmfqs_hip_op *ops, size_t count, const double *mat_pool,
+                             const unsigned *qubit_pool, bool allow_elementwise, std::vector<size_t> *params) {
+	std::vector<QOp> out;
+	out.reserve(count);
+	for(size_t i = 0; i < count; ++i) {
+		const tmfqs_hip_op &o = ops[i];
+		QOp q;
+		switch(o.kind) {
+			case TMFQS_HIP_OP_H:
+				validate_qubit(h, o.q0, "HipStateBackend::applyHadamard");
+				q.kind = Q_H;
+				q.b0 = pbit(h, o.q0);
+				break;
+			case TMFQS_HIP_OP_X:
+				validate_qubit(h, o.q0, "HipStateBackend::applyPauliX");
+				q.kind = Q_X;
+				q.b0 = pbit(h, o.q0);
+				break;
+			case TMFQS_HIP_OP_CX:
+				validate_two(h, o.q0, o.q1, "HipStateBackend::applyControlledNot");
+				q.kind = Q_CX;
+				q.b0 = pbit(h, o.q0);
+				q.b1 = pbit(h, o.q1);
+				break;
+			case TMFQS_HIP_OP_CPHASE:
+				validate_two(h, o.q0, o.q1, "HipStateBackend::applyControlledPhaseShift");
+				q.kind = Q_CP;
+				q.b0 = pbit(h, o.q0);
+				q.b1 = pbit(h, o.q1);
+				q.theta = o.theta;
+				__builtin_sincos(o.theta, &q.ci, &q.cr);
+				if(params) params->push_back(out.size());
+				break;
+			case TMFQS_HIP_OP_PHASE:
+				validate_qubit(h, o.q0, "phase gate");
+				q.kind = Q_PH;
+				q.b0 = pbit(h, o.q0);
+				q.theta = o.theta;
+				__builtin_sincos(o.theta, &q.ci, &q.cr);
+				if(params) params->push_back(out.size());
+				break;
+			case TMFQS_HIP_OP_SWAP:
+				validate_two(h, o.q0, o.q1, "QuantumRegister::applySwap");
+				q.kind = Q_SWAP;
+				q.b0 = pbit(h, o.q0);
+				q.b1 = pbit(h, o.q1);
+				break;
+			case TMFQS_HIP_OP_MATRIX: {
+				const char *scope = "HipStateBackend::applyGate";
+				if(!mat_pool || !qubit_pool) throw std::invalid_argument(std::string(scope) + " needs the matrix and qubit pools");
+				if(o.k < 1 || o.k > 5 || o.k > h->n) throw std::invalid_argument(std::string(scope) + " supports gates on 1 to 5 qubits");
+				const unsigned *qs = qubit_pool + o.q0;
+				u64 seen = 0;
+				for(unsigned j = 0; j < o.k; ++j) {
+					validate_qubit(h, qs[j], scope);
+					if((seen >> qs[j]) & 1ull) throw std::invalid_argument(std::string(scope) + " duplicate qubit index");
+					seen |= 1ull << qs[j];
+				}
+				const size_t dim = 1ull << o.k;
+				q.mat = h->qmats.size();
+				h->qmats.insert(h->qmats.end(), mat_pool + o.state, mat_pool + o.state + 2 * dim * dim);
+				if(o.k == 1) {
+					q.kind = Q_U1;
+					q.b0 = pbit(h, qs[0]);
+				} else {
+					q.kind = Q_UK;
+					q.k = o.k;
+					for(unsigned j = 0; j < o.k; ++j) q.bits[j] = pbit(h, qs[j]);
+				}
+				break;
+			}
+			case TMFQS_HIP_OP_PHASE_FLIP:
+			case TMFQS_HIP_OP_DIFFUSE_LAZY:
+			case TMFQS_HIP_OP_DIFFUSE_MATERIALIZED:
+				if(!allow_elementwise) throw std::invalid_argument("adjoint: only gate operations can be differentiated");
+				q.kind = -1 - o.kind;  // dispatched by the caller
+				q.state = o.state;
+				break;
+			default: throw std::invalid_argument("unknown operation kind " + std::to_string(o.kind));
+		}
+		out.push_back(q);
+	}
+	return out;
+}
+
+// ------------------------------------------------------------ adjoint
+ObsParams make_observable(const tmfqs_hip_state *h, const tmfqs_hip_diag_term *terms, size_t nterms) {
+	ObsParams O{};
+	if(nterms && !terms) throw std::invalid_argument("diagonal observable: null term list");
+	for(size_t i = 0; i < nterms; ++i) {
+		const tmfqs_hip_diag_term &t = terms[i];
+		if(t.kind == TMFQS_HIP_DIAG_Z) {
+			if(O.nz == kMaxObsTerms) throw std::invalid_argument("diagonal observable: too many Z terms");
+			u64 m = 0;
+			for(unsigned q = 0; q < 64; ++q)
+				if((t.bits >> q) & 1ull) {
+					validate_qubit(h, q, "diagonal observable");
+					m |= 1ull << pbit(h, q);
+				}
+			O.zmask[O.nz] = m;
+			O.zcoef[O.nz++] = t.coef;
+		} else if(t.kind == TMFQS_HIP_DIAG_PROJECTOR) {
+			if(O.np == kMaxObsTerms) throw std::invalid_argument("diagonal observable: too many projector terms");
+			validate_state(h, t.bits, "diagonal observable");
+			O.pstate[O.np] = t.bits;
+			O.pcoef[O.np++] = t.coef;
+		} else {
+			throw std::invalid_argument("diagonal observable: unknown term kind");
+		}
+	}
+	return O;
+}
+
+double device_expectation(tmfqs_hip_state *h, const void *a, void *lam, const ObsParams &O) {
+	{
+		ProfScope ps(h, "k_observable", (lam ? 2.0 : 1.0) * (double)h->N * (double)h->esz);
+		HIPC(launch_observable(h->L(), a, lam, h->n, O, h->d_red, h->d_red + 2 * kReduceBlocks));
+	}
+	double e = 0;
+	HIPC(hipMemcpyAsync(&e, h->d_red + 2 * kReduceBlocks, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+	HIPC(hipStreamSynchronize(h->stream));
+	return e;
+}
+
+// The inverse of a gate op (matrices are conjugate-transposed into qmats).
+QOp inverse_op(tmfqs_hip_state *h, const QOp &op) {
+	QOp r = op;
+	switch(op.kind) {
+		case Q_CP:
+		case Q_PH:
+			r.ci = -op.ci;
+			r.theta = -op.theta;
+			break;
+		case Q_U1:
+		case Q_UK: {
+			const unsigned k = op.kind == Q_U1 ? 1u : op.k;
+			const size_t dim = 1ull << k;
+			r.mat = h->qmats.size();
+			h->qmats.resize(r.mat + 2 * dim * dim);
+			for(size_t i = 0; i < dim; ++i)
+				for(size_t j = 0; j < dim; ++j) {
+					h->qmats[r.mat + 2 * (i * dim + j)] = h->qmats[op.mat + 2 * (j * dim + i)];
+					h->qmats[r.mat + 2 * (i * dim + j) + 1] = -h->qmats[op.mat + 2 * (j * dim + i) + 1];
+				}
+			break;
+		}
+		default: break;  // H, X, CX, swap are involutions
+	}
+	return r;
+}
+
+// Segment boundaries for forward checkpoints: the first pass boundary at
+// least `seg` ops after the previous checkpoint (forced after 4 * seg ops),
+// so checkpointing adds no pass to the fused forward sweep.
+std::vector<size_t> checkpoint_bounds(const tmfqs_hip_state *h, const std::vector<QOp> &ops, size_t seg) {
+	const unsigned n = h->n;
+	const unsigned T = tile_bits(h);
+	const unsigned C = coalesce_bits(h, T);
+	const u64 lowC = (C >= 64) ? ~0ull : ((1ull << C) - 1ull);
+	std::vector<size_t> b{0};
+	std::vector<unsigned> map(n);
+	for(unsigned i = 0; i < n; ++i) map[i] = i;
+	u64 active = 0;
+	bool empty = true;
+	size_t last = 0;
+	for(size_t i = 0; i < ops.size(); ++i) {
+		QOp op = ops[i];
+		if(op.kind == Q_SWAP) {
+			std::swap(map[op.b0], map[op.b1]);
+			continue;
+		}
+		op.b0 = map[op.b0];
+		op.b1 = map[op.b1];
+		for(unsigned j = 0; j < op.k; ++j) op.bits[j] = map[op.bits[j]];
+		const u64 need = nondiag_bits(op);
+		const bool newpass = !empty && (unsigned)__builtin_popcountll(active | need | lowC) > T;
+		if((newpass && i - last >= seg) || i - last >= 4 * seg) {
+			b.push_back(i);
+			last = i;
+			for(unsigned j = 0; j < n; ++j) map[j] = j;
+			op = ops[i];
+			active = 0;
+		} else if(newpass) {
+			active = 0;
+		}
+		active |= nondiag_bits(op);
+		empty = false;
+	}
+	b.push_back(ops.size());
+	return b;
+}
+
+void adjoint_run(tmfqs_hip_state *h, std::vector<QOp> &qops, const std::vector<size_t> &params, const ObsParams &O,
+                 unsigned checkpoint_gates, double *expval, double *grads) {
+	const size_t G = qops.size();
+	size_t seg = checkpoint_gates ? checkpoint_gates : (size_t)std::ceil(std::sqrt((double)std::max<size_t>(G, 1)));
+	std::vector<size_t> bounds = checkpoint_bounds(h, qops, seg);
+	// one buffer per segment end plus lambda; fewer checkpoints when HBM is short
+	size_t freeb = 0, totalb = 0;
+	HIPC(hipMemGetInfo(&freeb, &totalb));
+	freeb += h->adj_bufs.size() * h->adj_buf_bytes;  // reusable
+	const size_t budget = (size_t)(0.9 * (double)freeb);
+	while(bounds.size() > 2 && bounds.size() * h->amp_bytes > budget) {
+		seg *= 2;
+		bounds = checkpoint_bounds(h, qops, seg);
+	}
+	const size_t S = bounds.size() - 1;
+	const size_t nbuf = S + 1;
+	if(h->adj_buf_bytes != h->amp_bytes) {
+		HIPC(hipStreamSynchronize(h->stream));
+		for(void *p : h->adj_bufs) hipFree(p);
+		h->adj_bufs.clear();
+		h->adj_buf_bytes = h->amp_bytes;
+	}
+	while(h->adj_bufs.size() < nbuf) {
+		void *p = nullptr;
+		HIPC(hipMalloc(&p, h->amp_bytes));
+		h->adj_bufs.push_back(p);
+	}
+	void *lam = h->adj_bufs[S];
+	const size_t P = params.size();
+	if(h->grads_cap < std::max<size_t>(P, 1)) {
+		hipFree(h->d_grads);
+		h->d_grads = nullptr;
+		HIPC(hipMalloc((void **)&h->d_grads, std::max<size_t>(P, 1) * sizeof(double)));
+		h->grads_cap = std::max<size_t>(P, 1);
+	}
+	if(!h->d_gpart) HIPC(hipMalloc((void **)&h->d_gpart, (size_t)kMaxGradSlots * 4096 * sizeof(double)));
+	// forward sweep with a checkpoint at every segment end
+	for(size_t s = 0; s < S; ++s) {
+		exec_gates(h, qops, bounds[s], bounds[s + 1]);
+		ProfScope ps(h, "k_copy", pass_bytes(h));
+		HIPC(launch_copy(h->L(), h->amp, h->adj_bufs[s], h->n));
+	}
+	*expval = device_expectation(h, h->adj_bufs[S - 1], lam, O);
+	if(P == 0) return;
+	HIPC(hipMemsetAsync(h->d_grads, 0, P * sizeof(double), h->stream));
+	std::vector<int64_t> slot(G, -1);
+	for(size_t p = 0; p < P; ++p) slot[params[p]] = (int64_t)p;
+	// backward sweep: psi restarts from each segment's checkpoint, lambda runs through
+	GateTarget tg;
+	tg.b = lam;
+	tg.tile_delta = -1;
+	tg.gpart = h->d_gpart;
+	tg.dgrads = h->d_grads;
+	tg.grad_scale = -2.0;  // dE/dtheta = -2 Im <lambda| P |psi>
+	for(size_t s = S; s-- > 0;) {
+		std::vector<QOp> rev;
+		for(size_t i = bounds[s + 1]; i-- > bounds[s];) {
+			const QOp &op = qops[i];
+			if(slot[i] >= 0) {
+				QOp g;
+				g.kind = Q_GRAD;
+				g.b0 = op.b0;
+				g.b1 = op.b1;
+				g.k = op.kind == Q_CP ? 2u : 1u;
+				g.state = (u64)slot[i];
+				rev.push_back(g);
+			}
+			rev.push_back(inverse_op(h, op));
+		}
+		if(s == 0) {  // nothing before the first parameter needs undoing
+			size_t keep = 0;
+			for(size_t i = 0; i < rev.size(); ++i)
+				if(rev[i].kind == Q_GRAD) keep = i + 1;
+			rev.resize(keep);
+		}
+		tg.a = h->adj_bufs[s];
+		exec_gates(h, rev, 0, rev.size(), &tg);
+	}
+	HIPC(hipMemcpyAsync(grads, h->d_grads, P * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+	HIPC(hipStreamSynchronize(h->stream));
 }
 
 // ------------------------------------------------------------ sampling
@@ -1089,6 +1439,9 @@ void destroy_handle(tmfqs_hip_state *h) {
 	hipFree(h->d_sums);
 	hipFree(h->d_ticks);
 	hipFree(h->d_ulog);
+	for(void *p : h->adj_bufs) hipFree(p);
+	hipFree(h->d_gpart);
+	hipFree(h->d_grads);
 	if(h->arena.host) hipHostFree(h->arena.host);
 	if(h->arena.dev) hipFree(h->arena.dev);
 	if(h->h_stage) hipHostFree(h->h_stage);
@@ -1100,6 +1453,18 @@ void destroy_handle(tmfqs_hip_state *h) {
 
 // =========================================================================
 // extern "C" ABI
+static void throw_status(int rc) {
+	switch(rc) {
+		case TMFQS_HIP_OK: return;
+		case TMFQS_HIP_ERR_OUT_OF_RANGE: throw std::out_of_range(g_last_error);
+		case TMFQS_HIP_ERR_INVALID_ARGUMENT: throw std::invalid_argument(g_last_error);
+		case TMFQS_HIP_ERR_LOGIC: throw std::logic_error(g_last_error);
+		case TMFQS_HIP_ERR_OUT_OF_MEMORY: throw std::bad_alloc();
+		case TMFQS_HIP_ERR_DEVICE: throw DeviceError(g_last_error);
+		default: throw std::runtime_error(g_last_error);
+	}
+}
+
 #define ABI_BEGIN try {
 #define ABI_END                 \
 	return TMFQS_HIP_OK;        \
@@ -1683,6 +2048,67 @@ int tmfqs_hip_copy_probe(tmfqs_hip_state *h, int reps, double *ms) {
 int tmfqs_hip_stream(tmfqs_hip_state *h, void **stream) {
 	ABI_BEGIN
 	*stream = (void *)checked(h)->stream;
+	ABI_END
+}
+
+int tmfqs_hip_run(tmfqs_hip_state *h, const tmfqs_hip_op *ops, size_t count, const double *mat_pool, const unsigned *qubit_pool) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "operation list");
+	if(count && !ops) throw std::invalid_argument("tmfqs_hip_run: null operation list");
+	HIPC(hipSetDevice(h->cfg.device));
+	for(size_t i = 0; i < count; ++i) {
+		// one record at a time: unfused execution clears the matrix pool after each gate
+		const std::vector<QOp> q = convert_ops(h, ops + i, 1, mat_pool, qubit_pool, true, nullptr);
+		int rc = TMFQS_HIP_OK;
+		switch(ops[i].kind) {
+			case TMFQS_HIP_OP_PHASE_FLIP: rc = tmfqs_hip_phase_flip(h, ops[i].state); break;
+			case TMFQS_HIP_OP_DIFFUSE_LAZY:
+			case TMFQS_HIP_OP_DIFFUSE_MATERIALIZED: rc = tmfqs_hip_inversion_about_mean(h); break;
+			default: enqueue_gate(h, q[0]); break;
+		}
+		throw_status(rc);
+	}
+	ABI_END
+}
+
+int tmfqs_hip_expectation(tmfqs_hip_state *h, const tmfqs_hip_diag_term *terms, size_t nterms, double *out) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "expectation");
+	if(!out) throw std::invalid_argument("tmfqs_hip_expectation: null output");
+	const ObsParams O = make_observable(h, terms, nterms);
+	HIPC(hipSetDevice(h->cfg.device));
+	flush(h);
+	*out = device_expectation(h, h->amp, nullptr, O);
+	ABI_END
+}
+
+int tmfqs_hip_adjoint(tmfqs_hip_state *h, const tmfqs_hip_op *ops, size_t count, const double *mat_pool, const unsigned *qubit_pool,
+                      const tmfqs_hip_diag_term *terms, size_t nterms, unsigned checkpoint_gates, double *expval, double *grads,
+                      size_t grad_count) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "adjoint differentiation");
+	if(count && !ops) throw std::invalid_argument("tmfqs_hip_adjoint: null operation list");
+	if(!expval) throw std::invalid_argument("tmfqs_hip_adjoint: null expectation output");
+	const ObsParams O = make_observable(h, terms, nterms);
+	HIPC(hipSetDevice(h->cfg.device));
+	flush(h);
+	try {
+		std::vector<size_t> params;
+		std::vector<QOp> q = convert_ops(h, ops, count, mat_pool, qubit_pool, false, &params);
+		if(params.size() != grad_count)
+			throw std::invalid_argument("tmfqs_hip_adjoint: grad_count must equal the number of CPHASE and PHASE operations (" +
+			                            std::to_string(params.size()) + ")");
+		if(grad_count && !grads) throw std::invalid_argument("tmfqs_hip_adjoint: null gradient output");
+		h->tracked.clear();
+		adjoint_run(h, q, params, O, checkpoint_gates, expval, grads);
+	} catch(...) {
+		h->qmats.clear();  // the queue is empty: no queued op refers to the pool
+		throw;
+	}
+	h->qmats.clear();
 	ABI_END
 }
 

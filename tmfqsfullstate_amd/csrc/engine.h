@@ -27,7 +27,11 @@ enum TileOpKind : uint32_t {
 	TOP_CP = 4,       // multiply by (cr, ci) where all bits of the condition are set
 	TOP_LADDER = 5,   // multiply by e^{2 pi i F / 2^64}, F = (g & amask) << shift, where the condition holds
 	TOP_TWIDDLE = 6,  // multiply by e^{2 pi i F / 2^64}, F = sum over terms whose bit is set of (g & mask) << shift
+	TOP_GRAD = 7,     // dual passes only: accumulate Im sum conj(b) a where the condition holds into slot `mat`
 };
+
+// Gradient slots one dual-pass launch can accumulate.
+constexpr unsigned kMaxGradSlots = 64;
 
 struct TwiddleTerm {
 	u64 cond;   // global bit that enables the term
@@ -115,6 +119,27 @@ hipError_t launch_compare(const Launch &L, int prec_ref, const void *ref, int pr
                           CompareBlock *dout, unsigned *blocks);
 hipError_t launch_tile_pass(const Launch &L, void *a, const TilePass &P, const TileOp *dops, const double *dmats,
                             const TwiddleTerm *dterms, const double *dphase);
+// Grid (workgroups) of a dual pass over P; gpart must hold nslots * grid doubles.
+unsigned tile_dual_grid(const TilePass &P);
+hipError_t launch_tile_pass_dual(const Launch &L, void *a, void *b, const TilePass &P, const TileOp *dops, const double *dmats,
+                                 const TwiddleTerm *dterms, const double *dphase, double *gpart, unsigned nslots);
+// grads[gidx[s]] = scale * sum_i gpart[s * blocks + i], s < nslots.
+hipError_t launch_grad_finish(const Launch &L, const double *gpart, unsigned blocks, unsigned nslots, const uint32_t *gidx,
+                              double scale, double *grads);
+
+// Diagonal observable: Z-string terms (basis-index bit masks) and basis-state projectors.
+constexpr unsigned kMaxObsTerms = 16;
+struct ObsParams {
+	u64 zmask[kMaxObsTerms];
+	double zcoef[kMaxObsTerms];
+	u64 pstate[kMaxObsTerms];
+	double pcoef[kMaxObsTerms];
+	uint32_t nz, np;
+};
+// *dout = sum_x O(x) |a_x|^2 (FP64); when lam is non-null also lam = O a.
+// dpart holds kReduceBlocks doubles.
+hipError_t launch_observable(const Launch &L, const void *a, void *lam, unsigned n, const ObsParams &O, double *dpart,
+                             double *dout);
 // Reductions write `blocks` partials into dpart, then reduce into dout
 // (sum: 2 doubles; norm: 1 double).
 constexpr unsigned kReduceBlocks = 1024;

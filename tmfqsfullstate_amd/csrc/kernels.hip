@@ -865,6 +865,137 @@ __device__ __forceinline__ void tile_uk(typename V2T<R>::t *tile, u64 nloc, cons
 	}
 }
 
+// Applies one tile op to a tile of 2^T amplitudes held in LDS.  `base` is the
+// global index of the tile's element 0; local index l maps to global
+// base | (l & lowmask) | hitab[l >> C].
+template <typename R>
+__device__ __forceinline__ void tile_apply(typename V2T<R>::t *tile, const TileOp &op, u64 nloc, u64 base, unsigned C,
+                                           const u64 *hitab, const double *__restrict__ mats,
+                                           const TwiddleTerm *__restrict__ terms, const double2 *__restrict__ ptab) {
+	using V = typename V2T<R>::t;
+	const u64 lowmask = (1ull << C) - 1ull;
+	const R hc = (R)(1.0 / sqrt(2.0));
+	switch(op.kind) {
+		case TOP_H: {
+			const u64 m = 1ull << op.j0;
+			for(u64 q = threadIdx.x; q < (nloc >> 1); q += blockDim.x) {
+				const u64 l0 = ins0(q, op.j0), l1 = l0 | m;
+				const V x = tile[l0], y = tile[l1];
+				if(x.x == 0 && x.y == 0 && y.x == 0 && y.y == 0) continue;
+				V o0, o1;
+				o0.x = (x.x + y.x) * hc;
+				o0.y = (x.y + y.y) * hc;
+				o1.x = (x.x - y.x) * hc;
+				o1.y = (x.y - y.y) * hc;
+				tile[l0] = o0;
+				tile[l1] = o1;
+			}
+			break;
+		}
+		case TOP_X: {
+			const u64 m = 1ull << op.j0;
+			for(u64 q = threadIdx.x; q < (nloc >> 1); q += blockDim.x) {
+				const u64 l0 = ins0(q, op.j0), l1 = l0 | m;
+				if((l0 & op.lcond) != op.lcond) continue;
+				const V x = tile[l0], y = tile[l1];
+				if(x.x == y.x && x.y == y.y) continue;
+				tile[l0] = y;
+				tile[l1] = x;
+			}
+			break;
+		}
+		case TOP_U1: {
+			const u64 m = 1ull << op.j0;
+			const double *g = mats + op.mat;
+			const R g00r = (R)g[0], g00i = (R)g[1], g01r = (R)g[2], g01i = (R)g[3];
+			const R g10r = (R)g[4], g10i = (R)g[5], g11r = (R)g[6], g11i = (R)g[7];
+			for(u64 q = threadIdx.x; q < (nloc >> 1); q += blockDim.x) {
+				const u64 l0 = ins0(q, op.j0), l1 = l0 | m;
+				const V x = tile[l0], y = tile[l1];
+				if((x.x == 0 && x.y == 0) && (y.x == 0 && y.y == 0)) continue;
+				V o0, o1;
+				o0.x = g00r * x.x - g00i * x.y + g01r * y.x - g01i * y.y;
+				o0.y = g00r * x.y + g00i * x.x + g01r * y.y + g01i * y.x;
+				o1.x = g10r * x.x - g10i * x.y + g11r * y.x - g11i * y.y;
+				o1.y = g10r * x.y + g10i * x.x + g11r * y.y + g11i * y.x;
+				tile[l0] = o0;
+				tile[l1] = o1;
+			}
+			break;
+		}
+		case TOP_UK: {
+			switch(op.k) {
+				case 2: tile_uk<R, 2>(tile, nloc, op, mats); break;
+				case 3: tile_uk<R, 3>(tile, nloc, op, mats); break;
+				case 4: tile_uk<R, 4>(tile, nloc, op, mats); break;
+				case 5: tile_uk<R, 5>(tile, nloc, op, mats); break;
+				default: break;
+			}
+			break;
+		}
+		case TOP_CP: {
+			const R cr = (R)op.cr, ci = (R)op.ci;
+			const unsigned nb = __popcll(op.lcond);
+			for(u64 q = threadIdx.x; q < (nloc >> nb); q += blockDim.x) {
+				u64 l = q;
+				u64 cm = op.lcond;
+				while(cm) {
+					const unsigned b = __ffsll((long long)cm) - 1;
+					l = ins0(l, b) | (1ull << b);
+					cm &= cm - 1;
+				}
+				const V x = tile[l];
+				if(x.x == 0 && x.y == 0) continue;
+				V o;
+				o.x = cr * x.x - ci * x.y;
+				o.y = cr * x.y + ci * x.x;
+				tile[l] = o;
+			}
+			break;
+		}
+		case TOP_LADDER: {
+			const unsigned nb = __popcll(op.lcond);
+			for(u64 q = threadIdx.x; q < (nloc >> nb); q += blockDim.x) {
+				u64 l = q;
+				u64 cm = op.lcond;
+				while(cm) {
+					const unsigned b = __ffsll((long long)cm) - 1;
+					l = ins0(l, b) | (1ull << b);
+					cm &= cm - 1;
+				}
+				const u64 g = base | (l & lowmask) | hitab[l >> C];
+				const u64 F = (g & op.amask) << op.shift;
+				if(F == 0) continue;
+				tile[l] = rot<V>(tile[l], phase_of_range(F, ptab, op.cj0, op.cj1));
+			}
+			break;
+		}
+		case TOP_TWIDDLE: {
+			const TwiddleTerm *tt = terms + op.twid;
+			for(u64 l = threadIdx.x; l < nloc; l += blockDim.x) {
+				const u64 g = base | (l & lowmask) | hitab[l >> C];
+				u64 F = 0;
+				for(unsigned t = 0; t < op.nterms; ++t)
+					if(g & tt[t].cond) F += (g & tt[t].mask) << tt[t].shift;
+				if(F == 0) continue;
+				tile[l] = rot<V>(tile[l], phase_of_range(F, ptab, op.cj0, op.cj1));
+			}
+			break;
+		}
+		default: break;
+	}
+}
+
+template <typename R>
+__device__ __forceinline__ void tile_hitab(u64 *hitab, const TilePass &P) {
+	for(unsigned j = threadIdx.x; j < (1u << P.nh); j += blockDim.x) {
+		u64 v = 0;
+		for(unsigned b = 0; b < P.nh; ++b)
+			if((j >> b) & 1u) v |= 1ull << P.hbits[b];
+		hitab[j] = v;
+	}
+}
+
 template <typename R>
 __global__ void __launch_bounds__(256) k_tile(typename V2T<R>::t *__restrict__ a, TilePass P, const TileOp *__restrict__ ops,
                                               const double *__restrict__ mats, const TwiddleTerm *__restrict__ terms,
@@ -876,13 +1007,7 @@ __global__ void __launch_bounds__(256) k_tile(typename V2T<R>::t *__restrict__ a
 	const u64 nloc = 1ull << P.T;
 	const unsigned C = P.C;
 	const u64 lowmask = (1ull << C) - 1ull;
-	for(unsigned j = threadIdx.x; j < (1u << P.nh); j += blockDim.x) {
-		u64 v = 0;
-		for(unsigned b = 0; b < P.nh; ++b)
-			if((j >> b) & 1u) v |= 1ull << P.hbits[b];
-		hitab[j] = v;
-	}
-	const R hc = (R)(1.0 / sqrt(2.0));
+	tile_hitab<R>(hitab, P);
 	for(u64 tid = blockIdx.x; tid < P.ntiles; tid += gridDim.x) {
 		u64 base = tid << C;
 		for(unsigned b = 0; b < P.nh; ++b) base = ins0(base, P.hbits[b]);
@@ -893,120 +1018,118 @@ __global__ void __launch_bounds__(256) k_tile(typename V2T<R>::t *__restrict__ a
 		for(unsigned o = 0; o < P.nops; ++o) {
 			const TileOp op = ops[P.op0 + o];
 			if((base & op.gcond) != op.gcond) continue;  // uniform across the block
-			switch(op.kind) {
-				case TOP_H: {
-					const u64 m = 1ull << op.j0;
-					for(u64 q = threadIdx.x; q < (nloc >> 1); q += blockDim.x) {
-						const u64 l0 = ins0(q, op.j0), l1 = l0 | m;
-						const V x = tile[l0], y = tile[l1];
-						if(x.x == 0 && x.y == 0 && y.x == 0 && y.y == 0) continue;
-						V o0, o1;
-						o0.x = (x.x + y.x) * hc;
-						o0.y = (x.y + y.y) * hc;
-						o1.x = (x.x - y.x) * hc;
-						o1.y = (x.y - y.y) * hc;
-						tile[l0] = o0;
-						tile[l1] = o1;
-					}
-					break;
-				}
-				case TOP_X: {
-					const u64 m = 1ull << op.j0;
-					for(u64 q = threadIdx.x; q < (nloc >> 1); q += blockDim.x) {
-						const u64 l0 = ins0(q, op.j0), l1 = l0 | m;
-						if((l0 & op.lcond) != op.lcond) continue;
-						const V x = tile[l0], y = tile[l1];
-						if(x.x == y.x && x.y == y.y) continue;
-						tile[l0] = y;
-						tile[l1] = x;
-					}
-					break;
-				}
-				case TOP_U1: {
-					const u64 m = 1ull << op.j0;
-					const double *g = mats + op.mat;
-					const R g00r = (R)g[0], g00i = (R)g[1], g01r = (R)g[2], g01i = (R)g[3];
-					const R g10r = (R)g[4], g10i = (R)g[5], g11r = (R)g[6], g11i = (R)g[7];
-					for(u64 q = threadIdx.x; q < (nloc >> 1); q += blockDim.x) {
-						const u64 l0 = ins0(q, op.j0), l1 = l0 | m;
-						const V x = tile[l0], y = tile[l1];
-						if((x.x == 0 && x.y == 0) && (y.x == 0 && y.y == 0)) continue;
-						V o0, o1;
-						o0.x = g00r * x.x - g00i * x.y + g01r * y.x - g01i * y.y;
-						o0.y = g00r * x.y + g00i * x.x + g01r * y.y + g01i * y.x;
-						o1.x = g10r * x.x - g10i * x.y + g11r * y.x - g11i * y.y;
-						o1.y = g10r * x.y + g10i * x.x + g11r * y.y + g11i * y.x;
-						tile[l0] = o0;
-						tile[l1] = o1;
-					}
-					break;
-				}
-				case TOP_UK: {
-					switch(op.k) {
-						case 2: tile_uk<R, 2>(tile, nloc, op, mats); break;
-						case 3: tile_uk<R, 3>(tile, nloc, op, mats); break;
-						case 4: tile_uk<R, 4>(tile, nloc, op, mats); break;
-						case 5: tile_uk<R, 5>(tile, nloc, op, mats); break;
-						default: break;
-					}
-					break;
-				}
-				case TOP_CP: {
-					const R cr = (R)op.cr, ci = (R)op.ci;
-					const unsigned nb = __popcll(op.lcond);
-					for(u64 q = threadIdx.x; q < (nloc >> nb); q += blockDim.x) {
-						u64 l = q;
-						u64 cm = op.lcond;
-						while(cm) {
-							const unsigned b = __ffsll((long long)cm) - 1;
-							l = ins0(l, b) | (1ull << b);
-							cm &= cm - 1;
-						}
-						const V x = tile[l];
-						if(x.x == 0 && x.y == 0) continue;
-						V o;
-						o.x = cr * x.x - ci * x.y;
-						o.y = cr * x.y + ci * x.x;
-						tile[l] = o;
-					}
-					break;
-				}
-				case TOP_LADDER: {
-					const unsigned nb = __popcll(op.lcond);
-					for(u64 q = threadIdx.x; q < (nloc >> nb); q += blockDim.x) {
-						u64 l = q;
-						u64 cm = op.lcond;
-						while(cm) {
-							const unsigned b = __ffsll((long long)cm) - 1;
-							l = ins0(l, b) | (1ull << b);
-							cm &= cm - 1;
-						}
-						const u64 g = base | (l & lowmask) | hitab[l >> C];
-						const u64 F = (g & op.amask) << op.shift;
-						if(F == 0) continue;
-						tile[l] = rot<V>(tile[l], phase_of_range(F, ptab, op.cj0, op.cj1));
-					}
-					break;
-				}
-				case TOP_TWIDDLE: {
-					const TwiddleTerm *tt = terms + op.twid;
-					for(u64 l = threadIdx.x; l < nloc; l += blockDim.x) {
-						const u64 g = base | (l & lowmask) | hitab[l >> C];
-						u64 F = 0;
-						for(unsigned t = 0; t < op.nterms; ++t)
-							if(g & tt[t].cond) F += (g & tt[t].mask) << tt[t].shift;
-						if(F == 0) continue;
-						tile[l] = rot<V>(tile[l], phase_of_range(F, ptab, op.cj0, op.cj1));
-					}
-					break;
-				}
-				default: break;
-			}
+			tile_apply<R>(tile, op, nloc, base, C, hitab, mats, terms, ptab);
 			__syncthreads();
 		}
 		for(u64 l = threadIdx.x; l < nloc; l += blockDim.x)
 			nt_store(tile[l], &a[base | (l & lowmask) | hitab[l >> C]]);
 	}
+}
+
+// Two states stepped through the same op list (the adjoint's backward sweep:
+// a = psi, b = lambda).  TOP_GRAD accumulates Im sum conj(b) a over the states
+// where the op's condition holds into gradient slot op.mat: per wavefront in
+// LDS across the block's tiles, then one partial per block in
+// gpart[slot * gridDim.x + blockIdx.x] (fixed order, so results are
+// reproducible run to run).
+template <typename R>
+__global__ void __launch_bounds__(256) k_tile_dual(typename V2T<R>::t *__restrict__ a, typename V2T<R>::t *__restrict__ b,
+                                                   TilePass P, const TileOp *__restrict__ ops, const double *__restrict__ mats,
+                                                   const TwiddleTerm *__restrict__ terms, const double2 *__restrict__ ptab,
+                                                   double *__restrict__ gpart, unsigned nslots) {
+	using V = typename V2T<R>::t;
+	extern __shared__ char smem[];
+	V *ta = (V *)smem;
+	V *tb = ta + (1ull << P.T);
+	u64 *hitab = (u64 *)(tb + (1ull << P.T));
+	double *gacc = (double *)(hitab + (1u << P.nh));  // [4 waves][kMaxGradSlots]
+	const u64 nloc = 1ull << P.T;
+	const unsigned C = P.C;
+	const u64 lowmask = (1ull << C) - 1ull;
+	const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+	tile_hitab<R>(hitab, P);
+	for(unsigned i = threadIdx.x; i < 4 * kMaxGradSlots; i += blockDim.x) gacc[i] = 0.0;
+	for(u64 tid = blockIdx.x; tid < P.ntiles; tid += gridDim.x) {
+		u64 base = tid << C;
+		for(unsigned bb = 0; bb < P.nh; ++bb) base = ins0(base, P.hbits[bb]);
+		__syncthreads();
+		for(u64 l = threadIdx.x; l < nloc; l += blockDim.x) {
+			const u64 g = base | (l & lowmask) | hitab[l >> C];
+			ta[l] = nt_load(&a[g]);
+			tb[l] = nt_load(&b[g]);
+		}
+		__syncthreads();
+		for(unsigned o = 0; o < P.nops; ++o) {
+			const TileOp op = ops[P.op0 + o];
+			if((base & op.gcond) != op.gcond) continue;  // uniform across the block
+			if(op.kind == TOP_GRAD) {
+				const unsigned nb = __popcll(op.lcond);
+				double acc = 0.0;
+				for(u64 q = threadIdx.x; q < (nloc >> nb); q += blockDim.x) {
+					u64 l = q;
+					u64 cm = op.lcond;
+					while(cm) {
+						const unsigned bit = __ffsll((long long)cm) - 1;
+						l = ins0(l, bit) | (1ull << bit);
+						cm &= cm - 1;
+					}
+					const V x = ta[l], y = tb[l];
+					acc += (double)y.x * (double)x.y - (double)y.y * (double)x.x;
+				}
+				acc = wave_sum(acc);
+				if(lane == 0) gacc[wave * kMaxGradSlots + op.mat] += acc;
+				__syncthreads();  // the next op may overwrite what other waves still read
+				continue;
+			}
+			tile_apply<R>(ta, op, nloc, base, C, hitab, mats, terms, ptab);
+			tile_apply<R>(tb, op, nloc, base, C, hitab, mats, terms, ptab);
+			__syncthreads();
+		}
+		__syncthreads();
+		for(u64 l = threadIdx.x; l < nloc; l += blockDim.x) {
+			const u64 g = base | (l & lowmask) | hitab[l >> C];
+			nt_store(ta[l], &a[g]);
+			nt_store(tb[l], &b[g]);
+		}
+	}
+	__syncthreads();
+	for(unsigned s = threadIdx.x; s < nslots; s += blockDim.x)
+		gpart[(u64)s * gridDim.x + blockIdx.x] =
+		    ((gacc[s] + gacc[kMaxGradSlots + s]) + gacc[2 * kMaxGradSlots + s]) + gacc[3 * kMaxGradSlots + s];
+}
+
+// grads[gidx[s]] = scale * sum over the blocks of gpart[s][.] (one block per slot).
+__global__ void __launch_bounds__(256) k_grad_finish(const double *__restrict__ gpart, unsigned blocks,
+                                                     const uint32_t *__restrict__ gidx, double scale, double *__restrict__ grads) {
+	__shared__ double sh[8];
+	const unsigned s = blockIdx.x;
+	double v = 0.0;
+	for(unsigned i = threadIdx.x; i < blocks; i += blockDim.x) v += gpart[(u64)s * blocks + i];
+	v = block_sum(v, sh);
+	if(threadIdx.x == 0) grads[gidx[s]] = scale * v;
+}
+
+// Diagonal observable O(x) = sum_z coef_z (-1)^popcount(x & mask_z)
+// + sum_p coef_p [x == state_p]: per-block partials of sum O(x)|a_x|^2 and,
+// when lam is given, lam = O a.
+template <typename R>
+__global__ void __launch_bounds__(256) k_observable(const typename V2T<R>::t *__restrict__ a, typename V2T<R>::t *__restrict__ lam,
+                                                    u64 N, ObsParams O, double *__restrict__ part) {
+	using V = typename V2T<R>::t;
+	__shared__ double sh[8];
+	double acc = 0.0;
+	for(u64 i = gtid(); i < N; i += gstride()) {
+		double o = 0.0;
+		for(unsigned t = 0; t < O.nz; ++t) o += (__popcll(i & O.zmask[t]) & 1) ? -O.zcoef[t] : O.zcoef[t];
+		for(unsigned t = 0; t < O.np; ++t)
+			if(i == O.pstate[t]) o += O.pcoef[t];
+		const V v = a[i];
+		const double re = v.x, im = v.y;
+		acc += o * (re * re + im * im);
+		if(lam) lam[i] = mk<V>(o * re, o * im);
+	}
+	acc = block_sum(acc, sh);
+	if(threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
 // ------------------------------------------------------------------------
@@ -1217,6 +1340,53 @@ hipError_t launch_tile_pass(const Launch &L, void *a, const TilePass &P, const T
 		}
 		TMFQS_LAUNCH(k_tile<float>, grid, 256, shmem, L.stream, (float2 *)a, P, dops, dmats, dterms, (const double2 *)dphase);
 	}
+	return hipGetLastError();
+}
+
+unsigned tile_dual_grid(const TilePass &P) { return (unsigned)(P.ntiles < 4096 ? P.ntiles : 4096); }
+
+hipError_t launch_tile_pass_dual(const Launch &L, void *a, void *b, const TilePass &P, const TileOp *dops, const double *dmats,
+                                 const TwiddleTerm *dterms, const double *dphase, double *gpart, unsigned nslots) {
+	const size_t esz = L.prec == 0 ? 16 : 8;
+	const size_t shmem = 2 * (esz << P.T) + (8ull << P.nh) + 4 * kMaxGradSlots * sizeof(double);
+	if(shmem > 160 * 1024 || nslots > kMaxGradSlots) return hipErrorInvalidValue;
+	const unsigned grid = tile_dual_grid(P);
+	if(L.prec == 0) {
+		static bool attr = false;
+		if(!attr) {
+			(void)hipFuncSetAttribute((const void *)k_tile_dual<double>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+			attr = true;
+		}
+		TMFQS_LAUNCH(k_tile_dual<double>, grid, 256, shmem, L.stream, (double2 *)a, (double2 *)b, P, dops, dmats, dterms,
+		             (const double2 *)dphase, gpart, nslots);
+	} else {
+		static bool attr = false;
+		if(!attr) {
+			(void)hipFuncSetAttribute((const void *)k_tile_dual<float>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+			attr = true;
+		}
+		TMFQS_LAUNCH(k_tile_dual<float>, grid, 256, shmem, L.stream, (float2 *)a, (float2 *)b, P, dops, dmats, dterms,
+		             (const double2 *)dphase, gpart, nslots);
+	}
+	return hipGetLastError();
+}
+
+hipError_t launch_grad_finish(const Launch &L, const double *gpart, unsigned blocks, unsigned nslots, const uint32_t *gidx,
+                              double scale, double *grads) {
+	if(nslots == 0) return hipSuccess;
+	TMFQS_LAUNCH(k_grad_finish, nslots, 256, 0, L.stream, gpart, blocks, gidx, scale, grads);
+	return hipGetLastError();
+}
+
+hipError_t launch_observable(const Launch &L, const void *a, void *lam, unsigned n, const ObsParams &O, double *dpart,
+                             double *dout) {
+	const u64 N = 1ull << n;
+	const unsigned g = grid_for(N, 256, kReduceBlocks);
+	if(L.prec == 0)
+		TMFQS_LAUNCH(k_observable<double>, g, 256, 0, L.stream, (const double2 *)a, (double2 *)lam, N, O, dpart);
+	else
+		TMFQS_LAUNCH(k_observable<float>, g, 256, 0, L.stream, (const float2 *)a, (float2 *)lam, N, O, dpart);
+	TMFQS_LAUNCH(k_finish, 1, 64, 0, L.stream, dpart, g, 1u, dout);
 	return hipGetLastError();
 }
 

@@ -89,6 +89,25 @@ tmfqs::circuit::Circuit toCircuit(unsigned n, const tmfqs_hip_op *ops, size_t co
 	return c;
 }
 
+tmfqs::sim::DiagonalObservable toObservable(const tmfqs_hip_diag_term *terms, size_t nterms) {
+	if(nterms && !terms) throw std::invalid_argument("diagonal observable: null term list");
+	tmfqs::sim::DiagonalObservable obs;
+	for(size_t i = 0; i < nterms; ++i) {
+		if(terms[i].kind == TMFQS_HIP_DIAG_Z) {
+			tmfqs::sim::DiagonalObservable::ZString z;
+			z.coef = terms[i].coef;
+			for(unsigned q = 0; q < 64; ++q)
+				if((terms[i].bits >> q) & 1ull) z.qubits.push_back(q);
+			obs.zStrings.push_back(z);
+		} else if(terms[i].kind == TMFQS_HIP_DIAG_PROJECTOR) {
+			obs.projectors.push_back({terms[i].bits, terms[i].coef});
+		} else {
+			throw std::invalid_argument("diagonal observable: unknown term kind");
+		}
+	}
+	return obs;
+}
+
 size_t toOps(const tmfqs::circuit::Circuit &c, tmfqs_hip_op *out, size_t cap) {
 	using namespace tmfqs;
 	const auto &ops = c.operations();
@@ -288,6 +307,37 @@ int tmfqs_circuit_grover(unsigned n, const uint64_t *marked, size_t m, int mater
 	o.markedStates = tmfqs::BasisStateList(std::vector<tmfqs::StateIndex>(marked, marked + m));
 	o.materializedDiffusion = materialized != 0;
 	*count = toOps(tmfqs::circuit::makeGrover(n, o), out, cap);
+	CAPI_END
+}
+
+int tmfqs_circuit_ring(unsigned n, unsigned layers, unsigned seed, tmfqs_hip_op *out, size_t cap, size_t *count) {
+	CAPI_BEGIN
+	*count = toOps(tmfqs::circuit::makeRingLayers(n, layers, seed), out, cap);
+	CAPI_END
+}
+
+int tmfqs_reg_adjoint(tmfqs_reg *r, const tmfqs_hip_op *ops, size_t count, const double *mat_pool, const unsigned *mat_qubits,
+                      const tmfqs_hip_diag_term *terms, size_t nterms, unsigned checkpoint_gates, double *expval,
+                      double *grads, size_t grad_count, double *seconds) {
+	CAPI_BEGIN
+	if(!r || !expval) throw std::invalid_argument("tmfqs_reg_adjoint: null argument");
+	const tmfqs::sim::DiagonalObservable obs = toObservable(terms, nterms);
+	const tmfqs::circuit::Circuit c = toCircuit(r->reg.qubitCount(), ops, count, mat_pool, mat_qubits);
+	const tmfqs::sim::AdjointResult res = tmfqs::sim::adjointGradient(c, r->reg, obs, checkpoint_gates);
+	if(res.gradients.size() != grad_count)
+		throw std::invalid_argument("tmfqs_reg_adjoint: grad_count must equal the number of CPHASE ops (" +
+		                            std::to_string(res.gradients.size()) + ")");
+	for(size_t i = 0; i < grad_count; ++i) grads[i] = res.gradients[i];
+	*expval = res.expectation;
+	if(seconds) *seconds = res.executionSeconds;
+	CAPI_END
+}
+
+int tmfqs_reg_expectation(tmfqs_reg *r, const tmfqs_hip_diag_term *terms, size_t nterms, double *out) {
+	CAPI_BEGIN
+	if(!r || !out) throw std::invalid_argument("tmfqs_reg_expectation: null argument");
+	const tmfqs::sim::DiagonalObservable obs = toObservable(terms, nterms);
+	*out = tmfqs::sim::expectation(r->reg, obs);
 	CAPI_END
 }
 

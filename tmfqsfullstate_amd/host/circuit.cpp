@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <random>
 #include <stdexcept>
 #include <type_traits>
 
@@ -98,6 +99,18 @@ void appendGrover(Circuit &c, const GroverCircuitOptions &options) {
 Circuit makeGrover(unsigned int qubits, GroverCircuitOptions options) {
 	Circuit c(qubits);
 	appendGrover(c, options);
+	return c;
+}
+
+Circuit makeRingLayers(unsigned int qubits, unsigned int layers, unsigned int seed) {
+	if(qubits < 2) throw std::invalid_argument("ring layers need at least two qubits");
+	Circuit c(qubits);
+	std::mt19937 gen(seed);
+	std::uniform_real_distribution<double> dist(0.0, 2.0 * std::acos(-1.0));
+	for(unsigned int l = 0; l < layers; ++l) {
+		for(unsigned int q = 0; q < qubits; ++q) c.h(q);
+		for(unsigned int i = 0; i < qubits; ++i) c.controlledPhase(i, (i + 1) % qubits, dist(gen));
+	}
 	return c;
 }
 

@@ -185,6 +185,13 @@ void QuantumRegister::flushAffineOverlay() {
 	cachedLogicalAmplitudeSum_ = saved;
 }
 
+IStateBackend *QuantumRegister::beginExternalUpdate() {
+	requireInitialized("external update");
+	flushAffineOverlay();
+	cachedLogicalAmplitudeSum_.reset();
+	return backend_.get();
+}
+
 unsigned int QuantumRegister::qubitCount() const { return numQubits_; }
 StateIndex QuantumRegister::stateCount() const { return numStates_; }
 size_t QuantumRegister::amplitudeElementCount() const { return static_cast<size_t>(numStates_) * 2u; }

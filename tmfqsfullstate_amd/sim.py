@@ -7,7 +7,7 @@ import os
 
 import numpy as np
 
-from . import LIB_DIR, Config, HipState, HipError, Op, _EXC, default_config, lib as hip_lib
+from . import LIB_DIR, Config, DiagTerm, HipState, HipError, Op, OP_CPHASE, _EXC, default_config, make_terms, lib as hip_lib
 
 SIM_LIB = os.path.join(LIB_DIR, "libtmfqs.so")
 
@@ -44,6 +44,12 @@ def lib():
         L.tmfqs_circuit_qft.argtypes = [ctypes.c_uint, ctypes.POINTER(Op), ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         L.tmfqs_circuit_grover.argtypes = [ctypes.c_uint, _u64p, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(Op),
                                            ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+        L.tmfqs_circuit_ring.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.POINTER(Op), ctypes.c_size_t,
+                                         ctypes.POINTER(ctypes.c_size_t)]
+        L.tmfqs_reg_adjoint.argtypes = [_vp, ctypes.POINTER(Op), ctypes.c_size_t, _dp, ctypes.POINTER(ctypes.c_uint),
+                                        ctypes.POINTER(DiagTerm), ctypes.c_size_t, ctypes.c_uint, _dp, _dp, ctypes.c_size_t,
+                                        _dp]
+        L.tmfqs_reg_expectation.argtypes = [_vp, ctypes.POINTER(DiagTerm), ctypes.c_size_t, _dp]
         _lib = L
     return _lib
 
@@ -73,6 +79,15 @@ def grover_ops(n, marked, materialized):
     arr = (Op * max(1, cnt.value))()
     check(lib().tmfqs_circuit_grover(n, m.ctypes.data_as(_u64p), len(m), int(materialized), arr, cnt.value,
                                      ctypes.byref(cnt)))
+    return arr, cnt.value
+
+
+def ring_ops(n, layers, seed):
+    """circuit::makeRingLayers(n, layers, seed) as op records."""
+    cnt = ctypes.c_size_t()
+    check(lib().tmfqs_circuit_ring(n, layers, seed, None, 0, ctypes.byref(cnt)))
+    arr = (Op * max(1, cnt.value))()
+    check(lib().tmfqs_circuit_ring(n, layers, seed, arr, cnt.value, ctypes.byref(cnt)))
     return arr, cnt.value
 
 
@@ -174,6 +189,26 @@ class Register:
         o = ctypes.c_uint64()
         check(lib().tmfqs_reg_measure_fixed(self._h, rnd, ctypes.byref(o)))
         return o.value
+
+    def adjoint(self, ops, count, terms, checkpoint_gates=0, mat_pool=None, mat_qubits=None):
+        """sim::adjointGradient: returns (E, gradients per CPHASE op, executionSeconds)."""
+        nparams = sum(1 for i in range(count) if ops[i].kind == OP_CPHASE)
+        grads = np.zeros(max(1, nparams))
+        arr, cnt = make_terms(terms)
+        mp = None if mat_pool is None else np.ascontiguousarray(mat_pool, dtype=np.float64)
+        mq = None if mat_qubits is None else np.ascontiguousarray(mat_qubits, dtype=np.uint32)
+        e, secs = ctypes.c_double(), ctypes.c_double()
+        check(lib().tmfqs_reg_adjoint(self._h, ops, count, None if mp is None else mp.ctypes.data_as(_dp),
+                                      None if mq is None else mq.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), arr, cnt,
+                                      checkpoint_gates, ctypes.byref(e), grads.ctypes.data_as(_dp), nparams,
+                                      ctypes.byref(secs)))
+        return e.value, grads[:nparams], secs.value
+
+    def expectation(self, terms):
+        arr, cnt = make_terms(terms)
+        out = ctypes.c_double()
+        check(lib().tmfqs_reg_expectation(self._h, arr, cnt, ctypes.byref(out)))
+        return out.value
 
     def hip_state(self):
         """The device handle behind the register, as a non-owning HipState."""
