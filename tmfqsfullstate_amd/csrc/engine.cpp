@@ -130,7 +130,7 @@ struct tmfqs_hip_state {
 	hipStream_t stream = nullptr;
 	double *d_red = nullptr;    // reduction partials + outputs
 	double *d_phase = nullptr;  // 5 x 256 complex phase tables
-	double *d_cin = nullptr;    // e^{i pi v / 2^j}, j = 0..3, v < 2^j (QFT in-block constants)
+	double *d_cin = nullptr;    // e^{i pi v / 2^j} at 2^j - 1 + v, j = 0..5 (QFT in-block constants)
 	double *d_sums = nullptr;   // sampling chunk sums
 	ChunkTicks *d_ticks = nullptr;
 	int32_t *d_ulog = nullptr;
@@ -307,7 +307,7 @@ void build_phase_tables(tmfqs_hip_state *h) {
 	HIPC(hipMalloc((void **)&h->d_phase, t.size() * sizeof(double)));
 	HIPC(hipMemcpy(h->d_phase, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
 	std::vector<double> c;
-	for(int j = 0; j < 4; ++j)
+	for(int j = 0; j < 6; ++j)
 		for(int v = 0; v < (1 << j); ++v) {
 			double s, co;
 			__builtin_sincos(kPi * (double)v / (double)(1 << j), &s, &co);
@@ -665,6 +665,52 @@ bool match_qft_block(const std::vector<QOp> &ops, const std::vector<size_t> &idx
 	return pbot >= C;
 }
 
+// Recognises a complete QFT as makeQft emits it (circuit.cpp:87-106): for
+// t = n-1 .. 0, H on t and then the ladder from every lower bit, with the
+// swap network leaving the full bit reversal in `map`.  Plans it as an
+// optional leading pass of r = (n - 6) mod 8 targets, passes of eight targets
+// (k_qft_block), and the last six targets fused with the reversal.
+bool plan_qft(const std::vector<QOp> &ops, const std::vector<unsigned> &map, unsigned n, std::vector<PassPlan> &passes) {
+	if(n < 12 || n > 40) return false;
+	for(unsigned b = 0; b < n; ++b)
+		if(map[b] != n - 1 - b) return false;
+	std::vector<size_t> start(n + 1);  // start[i]: first op of target n-1-i
+	size_t k = 0;
+	for(unsigned i = 0; i < n; ++i) {
+		const unsigned t = n - 1 - i;
+		if(k >= ops.size() || ops[k].kind != Q_H || ops[k].b0 != t) return false;
+		start[i] = k++;
+		u64 seen = 0;
+		while(k < ops.size() && ops[k].kind == Q_CP) {
+			unsigned hb, lb;
+			if(!is_qft_ladder_cp(ops[k], hb, lb) || hb != t || ((seen >> lb) & 1ull)) return false;
+			seen |= 1ull << lb;
+			++k;
+		}
+		if(seen != ((1ull << t) - 1ull)) return false;
+	}
+	if(k != ops.size()) return false;
+	start[n] = ops.size();
+	passes.clear();
+	auto add = [&](unsigned i0, unsigned i1) {  // targets n-1-i0 down to n-1-(i1-1)
+		PassPlan P;
+		for(size_t j = start[i0]; j < start[i1]; ++j) {
+			P.ops.push_back(j);
+			P.active |= nondiag_bits(ops[j]);
+		}
+		passes.push_back(P);
+	};
+	const unsigned r = (n - 6) % 8;
+	unsigned i = 0;
+	if(r) {
+		add(0, r);
+		i = r;
+	}
+	for(; i + 6 < n; i += 8) add(i, i + 8);
+	add(n - 6, n);
+	return true;
+}
+
 void run_permutation(tmfqs_hip_state *h, std::vector<unsigned> map, void *buf) {
 	// Data of nominal bit b sits at physical map[b]; move it back to b.
 	const unsigned n = h->n;
@@ -699,6 +745,59 @@ void run_permutation(tmfqs_hip_state *h, std::vector<unsigned> map, void *buf) {
 		who[b] = b;
 		cur[b] = b;
 	}
+}
+
+// Groups a launch's tile ops into register stages of W local bits (ops that
+// touch a bit outside the stage start a new one; diagonal ops join any stage)
+// and LDS steps for k-qubit matrices.  Sets each op's position sj among its
+// stage's bits.
+std::vector<TileStage> plan_stages(TileOp *ops, size_t cnt, unsigned T, unsigned W) {
+	std::vector<TileStage> out;
+	TileStage cur{};
+	bool open = false;
+	uint32_t mask = 0;
+	auto close = [&]() {
+		if(!open) return;
+		// pad with the highest free local bits: lanes then step through low
+		// (contiguous) LDS addresses
+		uint32_t m = mask;
+		for(int b = (int)T - 1; b >= 0 && (unsigned)__builtin_popcount(m) < W; --b) m |= 1u << b;
+		unsigned j = 0;
+		for(unsigned b = 0; b < T; ++b)
+			if((m >> b) & 1u) cur.bits[j++] = b;
+		for(uint32_t o = cur.op0; o < cur.op0 + cur.nops; ++o) {
+			TileOp &t = ops[o];
+			if(t.kind == TOP_H || t.kind == TOP_X || t.kind == TOP_U1)
+				for(unsigned q = 0; q < W; ++q)
+					if(cur.bits[q] == t.j0) t.sj = q;
+		}
+		out.push_back(cur);
+		open = false;
+	};
+	for(size_t o = 0; o < cnt; ++o) {
+		const TileOp &t = ops[o];
+		if(t.kind == TOP_UK) {
+			close();
+			TileStage l{};
+			l.op0 = (uint32_t)o;
+			l.nops = 1;
+			l.lds = 1;
+			out.push_back(l);
+			continue;
+		}
+		const uint32_t need = (t.kind == TOP_H || t.kind == TOP_X || t.kind == TOP_U1) ? (1u << t.j0) : 0u;
+		if(open && (unsigned)__builtin_popcount(mask | need) > W) close();
+		if(!open) {
+			cur = TileStage{};
+			cur.op0 = (uint32_t)o;
+			mask = 0;
+			open = true;
+		}
+		mask |= need;
+		cur.nops++;
+	}
+	close();
+	return out;
 }
 
 void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_t hi, const GateTarget *tgt) {
@@ -741,13 +840,22 @@ void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_
 		cur.ops.push_back(i);
 	}
 	if(!cur.ops.empty()) passes.push_back(cur);
+	// a whole QFT: eight-target block passes and the tail fused with the reversal
+	const bool qft_fused = ladders && plan_qft(ops, map, n, passes);
 
 	// 3. emit each pass as a tile kernel.
-	for(const PassPlan &P : passes) {
+	for(size_t pi = 0; pi < passes.size(); ++pi) {
+		const PassPlan &P = passes[pi];
+		if(qft_fused && pi + 1 == passes.size()) {
+			ProfScope ps(h, "k_qft_tail_reverse", pass_bytes(h));
+			HIPC(launch_qft_tail_reverse(h->L(), A, n, h->d_cin));
+			continue;
+		}
 		unsigned pbot = 0;
-		if(ladders && n <= 40 && match_qft_block(ops, P.ops, qft_block_coalesce_bits(h->prec), pbot)) {
+		const unsigned qc = qft_block_coalesce_bits(h->prec, h->cfg.coalesce_qubits);
+		if(ladders && n <= 40 && match_qft_block(ops, P.ops, qc, pbot)) {
 			ProfScope ps(h, "k_qft_block", pass_bytes(h));
-			HIPC(launch_qft_block(h->L(), A, n, pbot, h->d_phase, h->d_cin));
+			HIPC(launch_qft_block(h->L(), A, n, pbot, h->d_phase, h->d_cin, qc));
 			continue;
 		}
 		unsigned Cp = T;
@@ -910,6 +1018,7 @@ void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_
 		tp.nh = (unsigned)high.size();
 		for(size_t i = 0; i < high.size(); ++i) tp.hbits[i] = high[i];
 		tp.ntiles = 1ull << (n - Tp);
+		const unsigned W = tile_stage_width(h->prec, Tp, dual);
 		// op lists longer than kMaxTileOps (or with more than kMaxGradSlots
 		// gradient reductions) are split across launches.
 		size_t s0 = 0;
@@ -925,30 +1034,31 @@ void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_
 				++e;
 			}
 			const size_t cnt = e - s0;
+			const std::vector<TileStage> stages = plan_stages(tops.data() + s0, cnt, Tp, W);
 			const TileOp *dops = (const TileOp *)arena_upload(h, tops.data() + s0, cnt * sizeof(TileOp));
+			const TileStage *dst = (const TileStage *)arena_upload(h, stages.data(), stages.size() * sizeof(TileStage));
 			const double *dm = mats.empty() ? nullptr : (const double *)arena_upload(h, mats.data(), mats.size() * sizeof(double));
 			const TwiddleTerm *dt =
 			    allterms.empty() ? nullptr : (const TwiddleTerm *)arena_upload(h, allterms.data(), allterms.size() * sizeof(TwiddleTerm));
 			tp.nops = (unsigned)cnt;
 			tp.op0 = 0;
-			if(dual) {
-				{
-					ProfScope ps(h, "k_tile_dual", 2 * pass_bytes(h));
-					HIPC(launch_tile_pass_dual(h->L(), A, B, tp, dops, dm, dt, h->d_phase, tgt->gpart, (unsigned)gidx.size()));
-				}
-				if(!gidx.empty()) {
-					const uint32_t *dg = (const uint32_t *)arena_upload(h, gidx.data(), gidx.size() * sizeof(uint32_t));
-					HIPC(launch_grad_finish(h->L(), tgt->gpart, tile_dual_grid(tp), (unsigned)gidx.size(), dg, tgt->grad_scale,
-					                        tgt->dgrads));
-				}
-			} else {
-				ProfScope ps(h, "k_tile", pass_bytes(h));
-				HIPC(launch_tile_pass(h->L(), A, tp, dops, dm, dt, h->d_phase));
+			{
+				ProfScope ps(h, dual ? "k_tile_dual" : "k_tile", (dual ? 2 : 1) * pass_bytes(h));
+				bool ldsops = false;
+				for(const TileStage &st : stages) ldsops = ldsops || st.lds;
+				HIPC(launch_tile_staged(h->L(), A, B, tp, W, dops, dst, (unsigned)stages.size(), dm, dt, h->d_phase,
+				                        dual ? tgt->gpart : nullptr, (unsigned)gidx.size(), ldsops));
+			}
+			if(dual && !gidx.empty()) {
+				const uint32_t *dg = (const uint32_t *)arena_upload(h, gidx.data(), gidx.size() * sizeof(uint32_t));
+				HIPC(launch_grad_finish(h->L(), tgt->gpart, tile_grid(tp, true), (unsigned)gidx.size(), dg, tgt->grad_scale,
+				                        tgt->dgrads));
 			}
 			s0 = e;
 		}
 	}
-	// 4. restore the identity bit layout.
+	// 4. restore the identity bit layout (the fused QFT tail already did).
+	if(qft_fused) return;
 	run_permutation(h, map, A);
 	if(dual) run_permutation(h, map, B);
 }

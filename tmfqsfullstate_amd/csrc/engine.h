@@ -53,7 +53,21 @@ struct TileOp {
 	uint32_t shift;       // LADDER shift
 	uint32_t twid;        // TWIDDLE: index of the first term in the term pool
 	uint32_t cj0, cj1;    // LADDER/TWIDDLE: phase-table chunks [cj0, cj1] F can occupy
+	uint32_t sj;          // register stages: position of j0 among the stage's bits
+	uint32_t pad;
 	double cr, ci;        // CP phase
+};
+
+// A run of tile ops applied between two LDS round trips.  In a register
+// stage every thread holds the 2^W amplitudes whose local indices differ in
+// the stage's W bit positions (ascending) and applies the stage's ops in
+// registers; ops that need more bits than that (k-qubit matrices) run one by
+// one on the LDS tile (lds = 1).
+constexpr int kMaxStageBits = 5;
+struct TileStage {
+	uint32_t op0, nops;
+	uint32_t lds;
+	uint32_t bits[kMaxStageBits];
 };
 
 struct TilePass {
@@ -104,9 +118,16 @@ hipError_t launch_ew_chain(const Launch &L, void *a, unsigned n, const EwStep *d
 hipError_t launch_scatter(const Launch &L, void *a, const Scatter *dsc, unsigned count);
 // QFT block pass over targets pbot+7 .. pbot (see kernels.hip); dcin holds the
 // 15 constants e^{i pi v / 2^j}, j = 0..3, v < 2^j.
-hipError_t launch_qft_block(const Launch &L, void *a, unsigned n, unsigned pbot, const double *dphase, const double *dcin);
-// Low contiguous bits a QFT block tile keeps (4 for complex128, 5 for complex64).
-unsigned qft_block_coalesce_bits(int prec);
+// C contiguous low bits per tile: 4 or 5 (complex128), 5 or 6 (complex64).
+hipError_t launch_qft_block(const Launch &L, void *a, unsigned n, unsigned pbot, const double *dphase, const double *dcin,
+                            unsigned C);
+// Low contiguous bits a QFT block tile keeps: `requested` when supported,
+// else 5 for both precisions.
+unsigned qft_block_coalesce_bits(int prec, int requested);
+// QFT targets 5..0 (H + complete ladders, no lower bits) followed by the full
+// bit reversal, in place, for 12 <= n <= 40; dcq holds the 63 constants
+// e^{i pi v / 2^t}, t = 0..5, v < 2^t (at 2^t - 1 + v).
+hipError_t launch_qft_tail_reverse(const Launch &L, void *a, unsigned n, const double *dcq);
 // Chain of diffusions only: dm2 holds nsteps pairs (2 Re m, 2 Im m).
 hipError_t launch_diffuse_chain(const Launch &L, void *a, unsigned n, const double *dm2, unsigned nsteps);
 hipError_t launch_copy(const Launch &L, const void *src, void *dst, unsigned n);
@@ -117,12 +138,17 @@ struct CompareBlock {
 };
 hipError_t launch_compare(const Launch &L, int prec_ref, const void *ref, int prec_cand, const void *cand, unsigned n,
                           CompareBlock *dout, unsigned *blocks);
-hipError_t launch_tile_pass(const Launch &L, void *a, const TilePass &P, const TileOp *dops, const double *dmats,
-                            const TwiddleTerm *dterms, const double *dphase);
-// Grid (workgroups) of a dual pass over P; gpart must hold nslots * grid doubles.
-unsigned tile_dual_grid(const TilePass &P);
-hipError_t launch_tile_pass_dual(const Launch &L, void *a, void *b, const TilePass &P, const TileOp *dops, const double *dmats,
-                                 const TwiddleTerm *dterms, const double *dphase, double *gpart, unsigned nslots);
+// Register-stage width for a tile of T bits.
+unsigned tile_stage_width(int prec, unsigned T, bool dual);
+// Workgroups of a tiled pass over P; a dual pass's gpart holds nslots * grid doubles.
+unsigned tile_grid(const TilePass &P, bool dual);
+// Tiled pass over one state (b == nullptr) or two states stepped together
+// (b != nullptr: TOP_GRAD partials into gpart); ldsops: some stage runs on
+// the LDS tile (a separate kernel build, so the common case keeps its
+// register budget).
+hipError_t launch_tile_staged(const Launch &L, void *a, void *b, const TilePass &P, unsigned W, const TileOp *dops,
+                              const TileStage *dstages, unsigned nstages, const double *dmats, const TwiddleTerm *dterms,
+                              const double *dphase, double *gpart, unsigned nslots, bool ldsops);
 // grads[gidx[s]] = scale * sum_i gpart[s * blocks + i], s < nslots.
 hipError_t launch_grad_finish(const Launch &L, const double *gpart, unsigned blocks, unsigned nslots, const uint32_t *gidx,
                               double scale, double *grads);

@@ -676,8 +676,6 @@ struct QftBlockParams {
 	uint32_t n, pbot;
 };
 
-__device__ __forceinline__ double2 ld_phase(const double2 *__restrict__ p) { return *p; }
-
 template <typename V> __device__ __forceinline__ void hadamard_pair(V &x, V &y, double c) {
 	const double xr = x.x, xi = x.y, yr = y.x, yi = y.y;
 	x = mk<V>((xr + yr) * c, (xi + yi) * c);
@@ -688,7 +686,6 @@ template <typename R, int C>
 __global__ void __launch_bounds__(16 << C) k_qft_block(typename V2T<R>::t *__restrict__ a, QftBlockParams P,
                                                         const double2 *__restrict__ ptab, const double2 *__restrict__ cin) {
 	using V = typename V2T<R>::t;
-	constexpr unsigned TILE = 256u << C;
 	extern __shared__ char smem[];
 	V *tile = (V *)smem;
 	const unsigned t = threadIdx.x;
@@ -697,14 +694,16 @@ __global__ void __launch_bounds__(16 << C) k_qft_block(typename V2T<R>::t *__res
 	const double hc = 1.0 / sqrt(2.0);
 	const unsigned pb = P.pbot;
 	const u64 lowmask = (1ull << pb) - 1ull;
-	// in-block ladder constants c[j][v], j = 0..3 (offset 2^j - 1)
-	double2 cst[15];
-#pragma unroll
-	for(int i = 0; i < 15; ++i) cst[i] = cin[i];
-	// stage-A thread phases e^{i pi f4 / 2^j}, j = 4..7: top-byte table entries
-	double2 tpa[4];
+	// Per-thread constants, fixed for the whole launch: the stage-A thread
+	// phases e^{i pi f4 / 2^j} (j = 4..7) and the lane part e^{i pi lo / 2^(pb+j)}
+	// (j = 0..7) of the twiddle.  The in-block ladder constants
+	// c[j][v] = cin[2^j - 1 + v] are read where they are used (uniform
+	// addresses: scalar loads).
+	double2 tpa[4], plo[8];
 #pragma unroll
 	for(int j = 4; j < 8; ++j) tpa[j - 4] = ptab[(f4 << (7 - j)) & 255u];
+#pragma unroll
+	for(int j = 0; j < 8; ++j) plo[j] = phase_of_range((u64)lo << (63 - pb - j), ptab, 0, 4);
 	for(u64 tid = blockIdx.x; tid < P.ntiles; tid += gridDim.x) {
 		const u64 x = tid << C;
 		const u64 base = ((x >> pb) << (pb + 8)) | (x & lowmask);
@@ -725,8 +724,7 @@ __global__ void __launch_bounds__(16 << C) k_qft_block(typename V2T<R>::t *__res
 			for(int ra = 0; ra < 16; ++ra) {
 				if(!(ra & (1 << b))) continue;
 				const int low = ra & ((1 << b) - 1);
-				const double2 ph = cmul(cst[(1 << b) - 1 + low], tpa[b]);
-				v[ra] = rot<V>(v[ra], ph);
+				v[ra] = rot<V>(v[ra], cmul(cin[(1 << b) - 1 + low], tpa[b]));
 			}
 		}
 		__syncthreads();  // previous tile's stage-B reads are complete
@@ -753,26 +751,30 @@ __global__ void __launch_bounds__(16 << C) k_qft_block(typename V2T<R>::t *__res
 				if(!(rb & (1 << j))) continue;
 				const int low = rb & ((1 << j) - 1);
 				if(low == 0) continue;
-				v[rb] = rot<V>(v[rb], cst[(1 << j) - 1 + low]);
+				v[rb] = rot<V>(v[rb], cin[(1 << j) - 1 + low]);
 			}
 		}
-		// ---- deferred twiddle for the bits below the block
-		const u64 r = (base & lowmask) | lo;
-		if(r != 0) {
-			u64 whi = 0;
+		// ---- deferred twiddle for the bits below the block: block bit j set
+		// contributes e^{i pi r / 2^(pb+j)}, r = rhi + lo, which factors into a
+		// part uniform across the workgroup (rhi = x mod 2^pb) and the lane's plo[j].
+		const u64 rhi = x & lowmask;
+		if((rhi | lo) != 0) {
+			double2 phi = make_double2(1.0, 0.0);
 #pragma unroll
 			for(int j = 4; j < 8; ++j)
-				if((f4 >> (j - 4)) & 1u) whi += 1ull << (63 - pb - j);
-			double2 tw[16];
-			tw[0] = phase_of(r * whi, ptab);
+				if((f4 >> (j - 4)) & 1u)
+					phi = cmul(phi, cmul(phase_of_range(rhi << (63 - pb - j), ptab, 0, 4), plo[j]));
+			if(f4 != 0) {
 #pragma unroll
-			for(int j = 0; j < 4; ++j) {
-				const double2 phj = phase_of(r << (63 - pb - j), ptab);
-#pragma unroll
-				for(int m = 0; m < (1 << j); ++m) tw[m | (1 << j)] = cmul(tw[m], phj);
+				for(int rb = 0; rb < 16; ++rb) v[rb] = rot<V>(v[rb], phi);
 			}
 #pragma unroll
-			for(int rb = 0; rb < 16; ++rb) v[rb] = rot<V>(v[rb], tw[rb]);
+			for(int j = 0; j < 4; ++j) {
+				const double2 pj = cmul(phase_of_range(rhi << (63 - pb - j), ptab, 0, 4), plo[j]);
+#pragma unroll
+				for(int rb = 0; rb < 16; ++rb)
+					if(rb & (1 << j)) v[rb] = rot<V>(v[rb], pj);
+			}
 		}
 #pragma unroll
 		for(int rb = 0; rb < 16; ++rb) {
@@ -782,39 +784,187 @@ __global__ void __launch_bounds__(16 << C) k_qft_block(typename V2T<R>::t *__res
 	}
 }
 
-hipError_t launch_qft_block(const Launch &L, void *a, unsigned n, unsigned pbot, const double *dphase, const double *dcin) {
+template <typename R, int C>
+static hipError_t launch_qft_block_t(const Launch &L, void *a, unsigned n, unsigned pbot, const double *dphase, const double *dcin) {
+	using V = typename V2T<R>::t;
 	QftBlockParams P;
 	P.n = n;
 	P.pbot = pbot;
-	if(L.prec == 0) {
-		constexpr int C = 4;
-		P.ntiles = 1ull << (n - 8 - C);
-		const unsigned grid = (unsigned)(P.ntiles < (1u << 20) ? P.ntiles : (1u << 20));
-		const size_t shmem = (size_t)((256u << C) + (16u)) * 16;
-		static bool attr = false;
-		if(!attr) {
-			(void)hipFuncSetAttribute((const void *)k_qft_block<double, C>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-			attr = true;
-		}
-		hipLaunchKernelGGL((k_qft_block<double, C>), dim3(grid), dim3(16 << C), shmem, L.stream, (double2 *)a, P,
-		                   (const double2 *)dphase, (const double2 *)dcin);
-	} else {
-		constexpr int C = 5;
-		P.ntiles = 1ull << (n - 8 - C);
-		const unsigned grid = (unsigned)(P.ntiles < (1u << 20) ? P.ntiles : (1u << 20));
-		const size_t shmem = (size_t)((256u << C) + (16u)) * 8;
-		static bool attr = false;
-		if(!attr) {
-			(void)hipFuncSetAttribute((const void *)k_qft_block<float, C>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-			attr = true;
-		}
-		hipLaunchKernelGGL((k_qft_block<float, C>), dim3(grid), dim3(16 << C), shmem, L.stream, (float2 *)a, P,
-		                   (const double2 *)dphase, (const double2 *)dcin);
+	P.ntiles = 1ull << (n - 8 - C);
+	const unsigned grid = (unsigned)(P.ntiles < (1u << 20) ? P.ntiles : (1u << 20));
+	const size_t shmem = (size_t)((256u << C) + 16u) * sizeof(V);
+	static bool attr = false;
+	if(!attr) {
+		(void)hipFuncSetAttribute((const void *)k_qft_block<R, C>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+		attr = true;
 	}
+	hipLaunchKernelGGL((k_qft_block<R, C>), dim3(grid), dim3(16 << C), shmem, L.stream, (V *)a, P, (const double2 *)dphase,
+	                   (const double2 *)dcin);
 	return hipGetLastError();
 }
 
-unsigned qft_block_coalesce_bits(int prec) { return prec == 0 ? 4u : 5u; }
+hipError_t launch_qft_block(const Launch &L, void *a, unsigned n, unsigned pbot, const double *dphase, const double *dcin,
+                            unsigned C) {
+	if(pbot < C || pbot + 8 > n) return hipErrorInvalidValue;
+	if(L.prec == 0) {
+		if(C == 4) return launch_qft_block_t<double, 4>(L, a, n, pbot, dphase, dcin);
+		if(C == 5) return launch_qft_block_t<double, 5>(L, a, n, pbot, dphase, dcin);
+	} else {
+		if(C == 5) return launch_qft_block_t<float, 5>(L, a, n, pbot, dphase, dcin);
+		if(C == 6) return launch_qft_block_t<float, 6>(L, a, n, pbot, dphase, dcin);
+	}
+	return hipErrorInvalidValue;
+}
+
+unsigned qft_block_coalesce_bits(int prec, int requested) {
+	if(prec == 0) return requested == 4 ? 4u : 5u;  // 5: 512 B runs per row (measured faster)
+	return requested == 6 ? 6u : 5u;
+}
+
+// ------------------------------------------------------------------------
+// QFT tail fused with the final bit reversal (makeQft's swap network).
+// The last six targets 5..0 only mix the low six bits, so the tail is a
+// 64-point transform of every contiguous run of 64 amplitudes; the reversal
+// then sends index (H, M, Lo) -> (rev6(Lo), rev(M), rev6(H)) for the top six
+// bits H, middle bits M and low six bits Lo.  A workgroup owns the pair of
+// tiles {*, M, *} and {*, rev(M), *} (64 x 64 amplitudes each) and writes each
+// tile's transpose into the other's place, so the pair is exchanged in place.
+//
+// One wavefront holds 16 rows of a tile: a quad of lanes shares a row (lane
+// bits 1..0 = column bits 1..0) and each lane holds the 16 columns with those
+// low bits in registers.  Targets 5..2 run in registers, targets 1..0 across
+// the quad with lane shuffles, and the transposed store goes through LDS.  The
+// column swizzle keeps both the row-wise writes and the bit-reversed
+// transposed reads spread over the LDS banks.
+struct QftTailParams {
+	u64 nmid;      // 2^(n - 12)
+	uint32_t n, m;  // m = n - 12
+};
+
+__device__ __forceinline__ unsigned rev6(unsigned x) { return __brev(x) >> 26; }
+__device__ __forceinline__ unsigned tail_slot(unsigned row, unsigned col) {
+	return row * 64u + (col ^ ((((row & 3u) << 2) ^ (row >> 2)) & 15u));
+}
+
+// Butterfly of target bit `mask` of the lane index: lanes with the bit clear
+// keep (x + y) c, their partners (y - x) c.
+template <typename V> __device__ __forceinline__ V quad_h(V x, unsigned lane, unsigned mask, double c) {
+	const double yr = __shfl_xor((double)x.x, (int)mask, 64), yi = __shfl_xor((double)x.y, (int)mask, 64);
+	const double xr = x.x, xi = x.y;
+	return (lane & mask) ? mk<V>((yr - xr) * c, (yi - xi) * c) : mk<V>((xr + yr) * c, (xi + yi) * c);
+}
+
+// The 64-point tail transform of one row quarter held by a lane: registers
+// are column bits 5..2, the lane's quad position is column bits 1..0.
+// The 64-point tail transform of one row quarter held by a lane: registers
+// are column bits 5..2, the lane's quad position q is column bits 1..0.  The
+// ladder factor e^{i pi low / 2^t} of target t splits into a constant of the
+// register index (uniform: scalar loads) and the lane's qf[t - 2] =
+// e^{i pi q / 2^t}.
+template <typename V>
+__device__ __forceinline__ void tail_transform(V *v, unsigned lane, unsigned q, const double2 *qf, const double2 *__restrict__ cq,
+                                               double hc) {
+#pragma unroll
+	for(int tt = 5; tt >= 2; --tt) {
+		const int b = tt - 2;
+#pragma unroll
+		for(int ra = 0; ra < 16; ++ra)
+			if(!(ra & (1 << b))) hadamard_pair(v[ra], v[ra | (1 << b)], hc);
+#pragma unroll
+		for(int ra = 0; ra < 16; ++ra) {
+			if(!(ra & (1 << b))) continue;
+			const int hi = ra & ((1 << b) - 1);  // register bits below the target
+			const double2 ph = hi ? cmul(cq[(1 << tt) - 1 + (hi << 2)], qf[b]) : qf[b];
+			if(hi || q) v[ra] = rot<V>(v[ra], ph);
+		}
+	}
+	// target 1 (lane bit 1) with its ladder e^{i pi / 2} from bit 0, then target 0
+	const double2 ci = cq[2];
+#pragma unroll
+	for(int ra = 0; ra < 16; ++ra) {
+		V x = quad_h(v[ra], lane, 2u, hc);
+		if(q == 3u) x = rot<V>(x, ci);
+		v[ra] = quad_h(x, lane, 1u, hc);
+	}
+}
+
+// Transposed, bit-reversed store of the tile staged in LDS into tile `dst`.
+template <typename V>
+__device__ __forceinline__ void tail_store(const V *tile, V *__restrict__ a, unsigned sh, u64 dst) {
+	for(unsigned i = threadIdx.x; i < 4096u; i += blockDim.x) {
+		const unsigned orow = i >> 6, c = i & 63u;
+		nt_store(tile[tail_slot(rev6(c), rev6(orow))], &a[((u64)orow << sh) | (dst << 6) | c]);
+	}
+}
+
+// Workgroups of 256 lanes: tile M is transformed and staged in LDS while tile
+// rev(M) is read and transformed in registers, so one 64 KiB buffer serves
+// the pair and two workgroups share a CU (one loads while the other stores).
+template <typename R>
+__global__ void __launch_bounds__(256) k_qft_tail_reverse(typename V2T<R>::t *__restrict__ a, QftTailParams P,
+                                                          const double2 *__restrict__ cq) {
+	using V = typename V2T<R>::t;
+	extern __shared__ char smem[];
+	V *tile = (V *)smem;  // 64 x 64
+	const unsigned t = threadIdx.x;
+	const unsigned lane = t & 63u, q = lane & 3u;
+	const unsigned row = (t >> 6) * 16u + (lane >> 2);
+	const double hc = 1.0 / sqrt(2.0);
+	const unsigned sh = P.n - 6;
+	double2 qf[4];
+#pragma unroll
+	for(int b = 0; b < 4; ++b) qf[b] = cq[(1u << (b + 2)) - 1u + q];
+	for(u64 M = blockIdx.x; M < P.nmid; M += gridDim.x) {
+		const u64 Mr = P.m ? (__brevll(M) >> (64 - P.m)) : 0;
+		if(Mr < M) continue;  // uniform: the pair is handled at M' = Mr
+		V v[16];
+		const u64 sa = (((u64)row) << sh) | (M << 6) | q;
+#pragma unroll
+		for(int ra = 0; ra < 16; ++ra) v[ra] = nt_load(&a[sa | ((unsigned)ra << 2)]);
+		tail_transform(v, lane, q, qf, cq, hc);
+		__syncthreads();  // the previous pair's transposed reads are done
+#pragma unroll
+		for(int ra = 0; ra < 16; ++ra) tile[tail_slot(row, ((unsigned)ra << 2) | q)] = v[ra];
+		if(Mr != M) {
+			const u64 sb = (((u64)row) << sh) | (Mr << 6) | q;
+#pragma unroll
+			for(int ra = 0; ra < 16; ++ra) v[ra] = nt_load(&a[sb | ((unsigned)ra << 2)]);
+			tail_transform(v, lane, q, qf, cq, hc);
+		}
+		__syncthreads();
+		tail_store(tile, a, sh, Mr);  // tile M into the place of tile rev(M), already read
+		if(Mr != M) {
+			__syncthreads();
+#pragma unroll
+			for(int ra = 0; ra < 16; ++ra) tile[tail_slot(row, ((unsigned)ra << 2) | q)] = v[ra];
+			__syncthreads();
+			tail_store(tile, a, sh, M);
+		}
+	}
+}
+
+template <typename R>
+static hipError_t launch_qft_tail_t(const Launch &L, void *a, unsigned n, const double *dcq) {
+	using V = typename V2T<R>::t;
+	QftTailParams P;
+	P.n = n;
+	P.m = n - 12;
+	P.nmid = 1ull << P.m;
+	const size_t shmem = 4096 * sizeof(V);
+	const unsigned grid = (unsigned)(P.nmid < 8192 ? P.nmid : 8192);
+	static bool attr = false;
+	if(!attr) {
+		(void)hipFuncSetAttribute((const void *)k_qft_tail_reverse<R>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+		attr = true;
+	}
+	hipLaunchKernelGGL(k_qft_tail_reverse<R>, dim3(grid), dim3(256), shmem, L.stream, (V *)a, P, (const double2 *)dcq);
+	return hipGetLastError();
+}
+
+hipError_t launch_qft_tail_reverse(const Launch &L, void *a, unsigned n, const double *dcq) {
+	if(n < 12 || n > 40) return hipErrorInvalidValue;
+	return L.prec == 0 ? launch_qft_tail_t<double>(L, a, n, dcq) : launch_qft_tail_t<float>(L, a, n, dcq);
+}
 
 template <typename R, int K>
 __device__ __forceinline__ void tile_uk(typename V2T<R>::t *tile, u64 nloc, const TileOp &op, const double *__restrict__ mats) {
@@ -996,106 +1146,269 @@ __device__ __forceinline__ void tile_hitab(u64 *hitab, const TilePass &P) {
 	}
 }
 
-template <typename R>
-__global__ void __launch_bounds__(256) k_tile(typename V2T<R>::t *__restrict__ a, TilePass P, const TileOp *__restrict__ ops,
-                                              const double *__restrict__ mats, const TwiddleTerm *__restrict__ terms,
-                                              const double2 *__restrict__ ptab) {
-	using V = typename V2T<R>::t;
-	extern __shared__ char smem[];
-	V *tile = (V *)smem;
-	u64 *hitab = (u64 *)(smem + (sizeof(V) << P.T));
-	const u64 nloc = 1ull << P.T;
-	const unsigned C = P.C;
-	const u64 lowmask = (1ull << C) - 1ull;
-	tile_hitab<R>(hitab, P);
-	for(u64 tid = blockIdx.x; tid < P.ntiles; tid += gridDim.x) {
-		u64 base = tid << C;
-		for(unsigned b = 0; b < P.nh; ++b) base = ins0(base, P.hbits[b]);
-		__syncthreads();
-		for(u64 l = threadIdx.x; l < nloc; l += blockDim.x)
-			tile[l] = nt_load(&a[base | (l & lowmask) | hitab[l >> C]]);
-		__syncthreads();
-		for(unsigned o = 0; o < P.nops; ++o) {
-			const TileOp op = ops[P.op0 + o];
-			if((base & op.gcond) != op.gcond) continue;  // uniform across the block
-			tile_apply<R>(tile, op, nloc, base, C, hitab, mats, terms, ptab);
-			__syncthreads();
-		}
-		for(u64 l = threadIdx.x; l < nloc; l += blockDim.x)
-			nt_store(tile[l], &a[base | (l & lowmask) | hitab[l >> C]]);
+// ------------------------------------------------------------------------
+// Register-staged tiled pass.  The tile (2^T amplitudes, local index l maps to
+// global base | (l & lowmask) | hitab[l >> C]) is loaded into LDS with
+// coalesced reads; each register stage then moves 2^W amplitudes per thread
+// into registers, applies its ops there and writes them back, so a run of
+// gates costs one LDS round trip and one barrier instead of one per gate.
+template <typename V> __device__ __forceinline__ bool is_zero(V x) { return x.x == 0 && x.y == 0; }
+
+template <typename V, typename R, int W, int J>
+__device__ __forceinline__ void reg_h(V *v, R hc) {
+#pragma unroll
+	for(int r = 0; r < (1 << W); ++r) {
+		if(r & (1 << J)) continue;
+		const V x = v[r], y = v[r | (1 << J)];
+		if(is_zero(x) && is_zero(y)) continue;  // dense_state_backend.cpp:263 skips all-zero pairs
+		V o0, o1;
+		o0.x = (x.x + y.x) * hc;
+		o0.y = (x.y + y.y) * hc;
+		o1.x = (x.x - y.x) * hc;
+		o1.y = (x.y - y.y) * hc;
+		v[r] = o0;
+		v[r | (1 << J)] = o1;
 	}
 }
 
-// Two states stepped through the same op list (the adjoint's backward sweep:
-// a = psi, b = lambda).  TOP_GRAD accumulates Im sum conj(b) a over the states
-// where the op's condition holds into gradient slot op.mat: per wavefront in
-// LDS across the block's tiles, then one partial per block in
-// gpart[slot * gridDim.x + blockIdx.x] (fixed order, so results are
-// reproducible run to run).
-template <typename R>
-__global__ void __launch_bounds__(256) k_tile_dual(typename V2T<R>::t *__restrict__ a, typename V2T<R>::t *__restrict__ b,
-                                                   TilePass P, const TileOp *__restrict__ ops, const double *__restrict__ mats,
-                                                   const TwiddleTerm *__restrict__ terms, const double2 *__restrict__ ptab,
-                                                   double *__restrict__ gpart, unsigned nslots) {
+template <typename V, int W, int J>
+__device__ __forceinline__ void reg_x(V *v, const unsigned *lidx, u64 lcond) {
+#pragma unroll
+	for(int r = 0; r < (1 << W); ++r) {
+		if(r & (1 << J)) continue;
+		if((lidx[r] & lcond) != lcond) continue;
+		const V x = v[r], y = v[r | (1 << J)];
+		if(x.x == y.x && x.y == y.y) continue;
+		v[r] = y;
+		v[r | (1 << J)] = x;
+	}
+}
+
+template <typename V, typename R, int W, int J>
+__device__ __forceinline__ void reg_u1(V *v, const double *g) {
+	const R g00r = (R)g[0], g00i = (R)g[1], g01r = (R)g[2], g01i = (R)g[3];
+	const R g10r = (R)g[4], g10i = (R)g[5], g11r = (R)g[6], g11i = (R)g[7];
+#pragma unroll
+	for(int r = 0; r < (1 << W); ++r) {
+		if(r & (1 << J)) continue;
+		const V x = v[r], y = v[r | (1 << J)];
+		if(is_zero(x) && is_zero(y)) continue;
+		V o0, o1;
+		o0.x = g00r * x.x - g00i * x.y + g01r * y.x - g01i * y.y;
+		o0.y = g00r * x.y + g00i * x.x + g01r * y.y + g01i * y.x;
+		o1.x = g10r * x.x - g10i * x.y + g11r * y.x - g11i * y.y;
+		o1.y = g10r * x.y + g10i * x.x + g11r * y.y + g11i * y.x;
+		v[r] = o0;
+		v[r | (1 << J)] = o1;
+	}
+}
+
+// Dispatch on the runtime stage position sj (< W).
+template <typename V, typename R, int W>
+__device__ __forceinline__ void reg_nondiag(V *v, const TileOp &op, const unsigned *lidx, const double *__restrict__ mats, R hc) {
+#define TMFQS_REG_CASE(J)                                                      \
+	case J:                                                                    \
+		if constexpr(J < W) {                                                  \
+			if(op.kind == TOP_H)                                               \
+				reg_h<V, R, W, J>(v, hc);                                      \
+			else if(op.kind == TOP_X)                                          \
+				reg_x<V, W, J>(v, lidx, op.lcond);                             \
+			else                                                               \
+				reg_u1<V, R, W, J>(v, mats + op.mat);                          \
+		}                                                                      \
+		break;
+	switch(op.sj) {
+		TMFQS_REG_CASE(0)
+		TMFQS_REG_CASE(1)
+		TMFQS_REG_CASE(2)
+		TMFQS_REG_CASE(3)
+		TMFQS_REG_CASE(4)
+		default: break;
+	}
+#undef TMFQS_REG_CASE
+}
+
+// Global index bit of local position b.
+__device__ __forceinline__ u64 local_to_global_bit(unsigned b, unsigned C, const u64 *hitab) {
+	return b < C ? (1ull << b) : hitab[1u << (b - C)];
+}
+
+template <typename V, typename R, int W>
+__device__ __forceinline__ void reg_diag(V *v, const TileOp &op, const unsigned *lidx, unsigned lb, const unsigned *sbits, u64 tbase,
+                                         u64 lowmask, unsigned C, const u64 *hitab, const TwiddleTerm *__restrict__ terms,
+                                         const double2 *__restrict__ ptab) {
+	if(op.kind == TOP_CP) {
+		const R cr = (R)op.cr, ci = (R)op.ci;
+#pragma unroll
+		for(int r = 0; r < (1 << W); ++r) {
+			if((lidx[r] & op.lcond) != op.lcond) continue;
+			const V x = v[r];
+			if(is_zero(x)) continue;
+			V o;
+			o.x = cr * x.x - ci * x.y;
+			o.y = cr * x.y + ci * x.x;
+			v[r] = o;
+		}
+	} else if(op.kind == TOP_LADDER) {
+		// F is linear in the stage bits (disjoint bits of g & amask), so the
+		// phase is e(F0) times e(F_j) for every set stage bit j.
+		const u64 g0 = tbase | (lb & lowmask) | hitab[lb >> C];
+		const u64 F0 = (g0 & op.amask) << op.shift;
+		if(F0) {
+			const double2 p0 = phase_of_range(F0, ptab, op.cj0, op.cj1);
+#pragma unroll
+			for(int r = 0; r < (1 << W); ++r)
+				if((lidx[r] & op.lcond) == op.lcond) v[r] = rot<V>(v[r], p0);
+		}
+#pragma unroll
+		for(int j = 0; j < W; ++j) {
+			const u64 Fj = (local_to_global_bit(sbits[j], C, hitab) & op.amask) << op.shift;
+			if(!Fj) continue;  // uniform
+			const double2 pj = phase_of_range(Fj, ptab, op.cj0, op.cj1);
+#pragma unroll
+			for(int r = 0; r < (1 << W); ++r)
+				if(((r >> j) & 1) && (lidx[r] & op.lcond) == op.lcond) v[r] = rot<V>(v[r], pj);
+		}
+	} else if(op.kind == TOP_TWIDDLE) {
+		const TwiddleTerm *tt = terms + op.twid;
+#pragma unroll
+		for(int r = 0; r < (1 << W); ++r) {
+			const u64 g = tbase | (lidx[r] & lowmask) | hitab[lidx[r] >> C];
+			u64 F = 0;
+			for(unsigned t = 0; t < op.nterms; ++t)
+				if(g & tt[t].cond) F += (g & tt[t].mask) << tt[t].shift;
+			if(F == 0) continue;
+			v[r] = rot<V>(v[r], phase_of_range(F, ptab, op.cj0, op.cj1));
+		}
+	}
+}
+
+template <typename R, int W, bool DUAL, bool LDSOPS>
+__global__ void __launch_bounds__(256) k_tile_staged(typename V2T<R>::t *__restrict__ a, typename V2T<R>::t *__restrict__ b,
+                                                     TilePass P, const TileOp *__restrict__ ops,
+                                                     const TileStage *__restrict__ stages, unsigned nstages,
+                                                     const double *__restrict__ mats, const TwiddleTerm *__restrict__ terms,
+                                                     const double2 *__restrict__ ptab, double *__restrict__ gpart,
+                                                     unsigned nslots) {
 	using V = typename V2T<R>::t;
+	constexpr int NR = 1 << W;
+	// One pad slot per 16 amplitudes keeps register-stage gathers over low
+	// bit positions (strided across lanes) off a single LDS bank; LDS-op
+	// stages index the tile directly, so their build keeps it unpadded.
+	constexpr bool PAD = !LDSOPS;
 	extern __shared__ char smem[];
+	const unsigned nloc = 1u << P.T;
+	const unsigned nlocp = PAD ? nloc + (nloc >> 4) : nloc;
 	V *ta = (V *)smem;
-	V *tb = ta + (1ull << P.T);
-	u64 *hitab = (u64 *)(tb + (1ull << P.T));
-	double *gacc = (double *)(hitab + (1u << P.nh));  // [4 waves][kMaxGradSlots]
-	const u64 nloc = 1ull << P.T;
+	V *tb = ta + nlocp;  // second tile (dual passes only)
+	u64 *hitab = (u64 *)(ta + (DUAL ? 2u : 1u) * nlocp);
+	double *gacc = (double *)(hitab + (1u << P.nh));  // dual: [4 waves][kMaxGradSlots]
 	const unsigned C = P.C;
 	const u64 lowmask = (1ull << C) - 1ull;
 	const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+	const R hc = (R)(1.0 / sqrt(2.0));
+	const unsigned ngrp = nloc >> W;
+	const unsigned iters = (ngrp + blockDim.x - 1) / blockDim.x;
 	tile_hitab<R>(hitab, P);
-	for(unsigned i = threadIdx.x; i < 4 * kMaxGradSlots; i += blockDim.x) gacc[i] = 0.0;
+	if(DUAL)
+		for(unsigned i = threadIdx.x; i < 4 * kMaxGradSlots; i += blockDim.x) gacc[i] = 0.0;
 	for(u64 tid = blockIdx.x; tid < P.ntiles; tid += gridDim.x) {
-		u64 base = tid << C;
-		for(unsigned bb = 0; bb < P.nh; ++bb) base = ins0(base, P.hbits[bb]);
+		u64 tbase = tid << C;
+		for(unsigned bb = 0; bb < P.nh; ++bb) tbase = ins0(tbase, P.hbits[bb]);
 		__syncthreads();
-		for(u64 l = threadIdx.x; l < nloc; l += blockDim.x) {
-			const u64 g = base | (l & lowmask) | hitab[l >> C];
-			ta[l] = nt_load(&a[g]);
-			tb[l] = nt_load(&b[g]);
+		for(unsigned l = threadIdx.x; l < nloc; l += blockDim.x) {
+			const u64 g = tbase | (l & lowmask) | hitab[l >> C];
+			const unsigned pl = PAD ? l + (l >> 4) : l;
+			ta[pl] = nt_load(&a[g]);
+			if(DUAL) tb[pl] = nt_load(&b[g]);
 		}
 		__syncthreads();
-		for(unsigned o = 0; o < P.nops; ++o) {
-			const TileOp op = ops[P.op0 + o];
-			if((base & op.gcond) != op.gcond) continue;  // uniform across the block
-			if(op.kind == TOP_GRAD) {
-				const unsigned nb = __popcll(op.lcond);
-				double acc = 0.0;
-				for(u64 q = threadIdx.x; q < (nloc >> nb); q += blockDim.x) {
-					u64 l = q;
-					u64 cm = op.lcond;
-					while(cm) {
-						const unsigned bit = __ffsll((long long)cm) - 1;
-						l = ins0(l, bit) | (1ull << bit);
-						cm &= cm - 1;
-					}
-					const V x = ta[l], y = tb[l];
-					acc += (double)y.x * (double)x.y - (double)y.y * (double)x.x;
+		for(unsigned s = 0; s < nstages; ++s) {
+			const TileStage st = stages[s];
+			if(LDSOPS && st.lds) {  // k-qubit matrices: their registers would cap every other stage's occupancy
+				for(unsigned o = 0; o < st.nops; ++o) {
+					const TileOp op = ops[st.op0 + o];
+					if((tbase & op.gcond) != op.gcond) continue;  // uniform across the block
+					tile_apply<R>(ta, op, nloc, tbase, C, hitab, mats, terms, ptab);
+					if(DUAL) tile_apply<R>(tb, op, nloc, tbase, C, hitab, mats, terms, ptab);
+					__syncthreads();
 				}
-				acc = wave_sum(acc);
-				if(lane == 0) gacc[wave * kMaxGradSlots + op.mat] += acc;
-				__syncthreads();  // the next op may overwrite what other waves still read
 				continue;
 			}
-			tile_apply<R>(ta, op, nloc, base, C, hitab, mats, terms, ptab);
-			tile_apply<R>(tb, op, nloc, base, C, hitab, mats, terms, ptab);
+			unsigned sbits[W];
+#pragma unroll
+			for(int j = 0; j < W; ++j) sbits[j] = st.bits[j];
+			for(unsigned it = 0; it < iters; ++it) {
+				const unsigned grp = threadIdx.x + it * blockDim.x;
+				const bool act = grp < ngrp;
+				unsigned lb = act ? grp : 0u;
+#pragma unroll
+				for(int j = 0; j < W; ++j) lb = ((lb >> sbits[j]) << (sbits[j] + 1)) | (lb & ((1u << sbits[j]) - 1u));
+				unsigned lidx[NR];
+				V va[NR], vb[NR];
+#pragma unroll
+				for(int r = 0; r < NR; ++r) {
+					unsigned off = 0;
+#pragma unroll
+					for(int j = 0; j < W; ++j)
+						if((r >> j) & 1) off |= 1u << sbits[j];
+					lidx[r] = lb | off;
+					if(act) {
+						const unsigned pl = PAD ? lidx[r] + (lidx[r] >> 4) : lidx[r];
+						va[r] = ta[pl];
+						if(DUAL) vb[r] = tb[pl];
+					}
+				}
+				for(unsigned o = 0; o < st.nops; ++o) {
+					const TileOp op = ops[st.op0 + o];
+					if((tbase & op.gcond) != op.gcond) continue;  // uniform across the block
+					if(op.kind == TOP_GRAD) {
+						if(DUAL) {
+							double acc = 0.0;
+							if(act) {
+#pragma unroll
+								for(int r = 0; r < NR; ++r)
+									if((lidx[r] & op.lcond) == op.lcond)
+										acc += (double)vb[r].x * (double)va[r].y - (double)vb[r].y * (double)va[r].x;
+							}
+							acc = wave_sum(acc);
+							if(lane == 0) gacc[wave * kMaxGradSlots + op.mat] += acc;
+						}
+						continue;
+					}
+					if(!act) continue;
+					if(op.kind == TOP_H || op.kind == TOP_X || op.kind == TOP_U1) {
+						reg_nondiag<V, R, W>(va, op, lidx, mats, hc);
+						if(DUAL) reg_nondiag<V, R, W>(vb, op, lidx, mats, hc);
+					} else {
+						reg_diag<V, R, W>(va, op, lidx, lb, sbits, tbase, lowmask, C, hitab, terms, ptab);
+						if(DUAL) reg_diag<V, R, W>(vb, op, lidx, lb, sbits, tbase, lowmask, C, hitab, terms, ptab);
+					}
+				}
+				if(act) {
+#pragma unroll
+					for(int r = 0; r < NR; ++r) {
+						const unsigned pl = PAD ? lidx[r] + (lidx[r] >> 4) : lidx[r];
+						ta[pl] = va[r];
+						if(DUAL) tb[pl] = vb[r];
+					}
+				}
+			}
 			__syncthreads();
 		}
-		__syncthreads();
-		for(u64 l = threadIdx.x; l < nloc; l += blockDim.x) {
-			const u64 g = base | (l & lowmask) | hitab[l >> C];
-			nt_store(ta[l], &a[g]);
-			nt_store(tb[l], &b[g]);
+		for(unsigned l = threadIdx.x; l < nloc; l += blockDim.x) {
+			const u64 g = tbase | (l & lowmask) | hitab[l >> C];
+			const unsigned pl = PAD ? l + (l >> 4) : l;
+			nt_store(ta[pl], &a[g]);
+			if(DUAL) nt_store(tb[pl], &b[g]);
 		}
 	}
-	__syncthreads();
-	for(unsigned s = threadIdx.x; s < nslots; s += blockDim.x)
-		gpart[(u64)s * gridDim.x + blockIdx.x] =
-		    ((gacc[s] + gacc[kMaxGradSlots + s]) + gacc[2 * kMaxGradSlots + s]) + gacc[3 * kMaxGradSlots + s];
+	if(DUAL) {
+		__syncthreads();
+		for(unsigned s = threadIdx.x; s < nslots; s += blockDim.x)
+			gpart[(u64)s * gridDim.x + blockIdx.x] =
+			    ((gacc[s] + gacc[kMaxGradSlots + s]) + gacc[2 * kMaxGradSlots + s]) + gacc[3 * kMaxGradSlots + s];
+	}
 }
 
 // grads[gidx[s]] = scale * sum over the blocks of gpart[s][.] (one block per slot).
@@ -1320,55 +1633,70 @@ hipError_t launch_chunk_ticks(const Launch &L, const void *a, unsigned n, const 
 	return hipGetLastError();
 }
 
-hipError_t launch_tile_pass(const Launch &L, void *a, const TilePass &P, const TileOp *dops, const double *dmats,
-                            const TwiddleTerm *dterms, const double *dphase) {
-	const size_t esz = L.prec == 0 ? 16 : 8;
-	const size_t shmem = (esz << P.T) + (8ull << P.nh);
-	const unsigned grid = (unsigned)(P.ntiles < 65536 ? P.ntiles : 65536);
-	if(L.prec == 0) {
-		static bool attr = false;
-		if(!attr) {
-			(void)hipFuncSetAttribute((const void *)k_tile<double>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-			attr = true;
-		}
-		TMFQS_LAUNCH(k_tile<double>, grid, 256, shmem, L.stream, (double2 *)a, P, dops, dmats, dterms, (const double2 *)dphase);
-	} else {
-		static bool attr = false;
-		if(!attr) {
-			(void)hipFuncSetAttribute((const void *)k_tile<float>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-			attr = true;
-		}
-		TMFQS_LAUNCH(k_tile<float>, grid, 256, shmem, L.stream, (float2 *)a, P, dops, dmats, dterms, (const double2 *)dphase);
+unsigned tile_stage_width(int prec, unsigned T, bool dual) {
+	(void)prec;
+	const unsigned w = dual ? 3u : 4u;  // 16 (8 per state when dual) amplitudes per thread
+	return T < w ? T : w;
+}
+
+unsigned tile_grid(const TilePass &P, bool dual) {
+	const u64 cap = dual ? 4096 : 65536;
+	return (unsigned)(P.ntiles < cap ? P.ntiles : cap);
+}
+
+template <typename R, int W, bool DUAL, bool LDSOPS>
+static hipError_t launch_staged_t(const Launch &L, void *a, void *b, const TilePass &P, const TileOp *dops, const TileStage *dstages,
+                                  unsigned nstages, const double *dmats, const TwiddleTerm *dterms, const double *dphase,
+                                  double *gpart, unsigned nslots) {
+	using V = typename V2T<R>::t;
+	const size_t nlocp = LDSOPS ? (1ull << P.T) : (1ull << P.T) + ((1ull << P.T) >> 4);
+	const size_t shmem = (DUAL ? 2 : 1) * sizeof(V) * nlocp + (8ull << P.nh) + (DUAL ? 4 * kMaxGradSlots * sizeof(double) : 0);
+	if(shmem > 160 * 1024) return hipErrorInvalidValue;
+	static bool attr = false;
+	if(!attr) {
+		(void)hipFuncSetAttribute((const void *)k_tile_staged<R, W, DUAL, LDSOPS>, hipFuncAttributeMaxDynamicSharedMemorySize,
+		                          160 * 1024);
+		attr = true;
 	}
+	TMFQS_LAUNCH((k_tile_staged<R, W, DUAL, LDSOPS>), tile_grid(P, DUAL), 256, shmem, L.stream, (V *)a, (V *)b, P, dops, dstages, nstages,
+	             dmats, dterms, (const double2 *)dphase, gpart, nslots);
 	return hipGetLastError();
 }
 
-unsigned tile_dual_grid(const TilePass &P) { return (unsigned)(P.ntiles < 4096 ? P.ntiles : 4096); }
-
-hipError_t launch_tile_pass_dual(const Launch &L, void *a, void *b, const TilePass &P, const TileOp *dops, const double *dmats,
-                                 const TwiddleTerm *dterms, const double *dphase, double *gpart, unsigned nslots) {
-	const size_t esz = L.prec == 0 ? 16 : 8;
-	const size_t shmem = 2 * (esz << P.T) + (8ull << P.nh) + 4 * kMaxGradSlots * sizeof(double);
-	if(shmem > 160 * 1024 || nslots > kMaxGradSlots) return hipErrorInvalidValue;
-	const unsigned grid = tile_dual_grid(P);
-	if(L.prec == 0) {
-		static bool attr = false;
-		if(!attr) {
-			(void)hipFuncSetAttribute((const void *)k_tile_dual<double>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-			attr = true;
-		}
-		TMFQS_LAUNCH(k_tile_dual<double>, grid, 256, shmem, L.stream, (double2 *)a, (double2 *)b, P, dops, dmats, dterms,
-		             (const double2 *)dphase, gpart, nslots);
-	} else {
-		static bool attr = false;
-		if(!attr) {
-			(void)hipFuncSetAttribute((const void *)k_tile_dual<float>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-			attr = true;
-		}
-		TMFQS_LAUNCH(k_tile_dual<float>, grid, 256, shmem, L.stream, (float2 *)a, (float2 *)b, P, dops, dmats, dterms,
-		             (const double2 *)dphase, gpart, nslots);
+template <typename R, bool DUAL, bool LDSOPS>
+static hipError_t launch_staged_w(const Launch &L, unsigned W, void *a, void *b, const TilePass &P, const TileOp *dops,
+                                  const TileStage *dstages, unsigned nstages, const double *dmats, const TwiddleTerm *dterms,
+                                  const double *dphase, double *gpart, unsigned nslots) {
+#define TMFQS_STAGED(WW) \
+	launch_staged_t<R, WW, DUAL, LDSOPS>(L, a, b, P, dops, dstages, nstages, dmats, dterms, dphase, gpart, nslots)
+	switch(W) {
+		case 1: return TMFQS_STAGED(1);
+		case 2: return TMFQS_STAGED(2);
+		case 3: return TMFQS_STAGED(3);
+		case 4: return TMFQS_STAGED(4);
+		default: return hipErrorInvalidValue;
 	}
-	return hipGetLastError();
+#undef TMFQS_STAGED
+}
+
+template <typename R>
+static hipError_t launch_staged_r(const Launch &L, unsigned W, void *a, void *b, const TilePass &P, const TileOp *dops,
+                                  const TileStage *dstages, unsigned nstages, const double *dmats, const TwiddleTerm *dterms,
+                                  const double *dphase, double *gpart, unsigned nslots, bool ldsops) {
+	if(b)
+		return ldsops ? launch_staged_w<R, true, true>(L, W, a, b, P, dops, dstages, nstages, dmats, dterms, dphase, gpart, nslots)
+		              : launch_staged_w<R, true, false>(L, W, a, b, P, dops, dstages, nstages, dmats, dterms, dphase, gpart, nslots);
+	return ldsops ? launch_staged_w<R, false, true>(L, W, a, b, P, dops, dstages, nstages, dmats, dterms, dphase, gpart, nslots)
+	              : launch_staged_w<R, false, false>(L, W, a, b, P, dops, dstages, nstages, dmats, dterms, dphase, gpart, nslots);
+}
+
+hipError_t launch_tile_staged(const Launch &L, void *a, void *b, const TilePass &P, unsigned W, const TileOp *dops,
+                              const TileStage *dstages, unsigned nstages, const double *dmats, const TwiddleTerm *dterms,
+                              const double *dphase, double *gpart, unsigned nslots, bool ldsops) {
+	if(W > P.T || nslots > kMaxGradSlots) return hipErrorInvalidValue;
+	if(L.prec == 0)
+		return launch_staged_r<double>(L, W, a, b, P, dops, dstages, nstages, dmats, dterms, dphase, gpart, nslots, ldsops);
+	return launch_staged_r<float>(L, W, a, b, P, dops, dstages, nstages, dmats, dterms, dphase, gpart, nslots, ldsops);
 }
 
 hipError_t launch_grad_finish(const Launch &L, const double *gpart, unsigned blocks, unsigned nslots, const uint32_t *gidx,
