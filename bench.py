@@ -13,6 +13,9 @@ Also measured in the same run and reported beside the headline:
     probability and the Mt19937(123456) measurement;
   * complex64: QFT-30 time and fidelity against the complex128 result
     (device comparison);
+  * batched circuits: the committed seed-42 workload (256 circuits of 10-22
+    qubits, 50-200 gates) as one autotuned batch graph launch, against the same
+    circuits run one at a time;
   * adjoint gradients: <Z_0> + |0><0| after 8 ring layers on 24 qubits (192
     controlled-phase angles from mt19937(3)), one forward + one backward sweep,
     against 384 forward circuits of plain central differences on the same GPU
@@ -151,6 +154,49 @@ def time_adjoint(T, S, precision, n=24, layers=8, seed=3, steps=3):
     return out, g
 
 
+def time_batch(T, precision, steps=3):
+    """The committed seed-42 workload (256 circuits, 10-22 qubits) as one
+    batch (autotuned tile width, one graph launch per run) against the same
+    circuits run one at a time through the engine."""
+    from tmfqsfullstate_amd.batch import Batch, load_circuit_list
+
+    circuits = load_circuit_list(os.path.join(ROOT, "tests", "golden", "batch_circuits_seed42.txt"))
+    gates = sum(c[2] for c in circuits)
+    with Batch(circuits, precision=precision) as b:
+        tuned = b.tune()
+        b.run()
+        runs = [b.run() for _ in range(steps)]
+        info = b.info()
+    batch_s = sorted(runs)[len(runs) // 2]
+    with T.HipState(precision=precision) as s:
+        for n, ops, cnt in circuits[:8]:  # warm up the single-circuit path
+            s.init_basis(n, 0)
+            s.run(ops, cnt)
+        s.synchronize()
+        t0 = time.perf_counter()
+        for n, ops, cnt in circuits:
+            s.init_basis(n, 0)
+            s.run(ops, cnt)
+            s.synchronize()
+        seq_s = time.perf_counter() - t0
+    return {
+        "circuits": len(circuits),
+        "gates": gates,
+        "precision": "complex128" if precision == T.C128 else "complex64",
+        "batch_s": batch_s,
+        "batch_runs_s": runs,
+        "circuits_per_s": len(circuits) / batch_s,
+        "gates_per_s": gates / batch_s,
+        "sequential_s": seq_s,
+        "speedup_vs_sequential": seq_s / batch_s,
+        "tuned_tile_qubits": tuned,
+        "rounds": info["rounds"],
+        "work_items": info["items"],
+        "graph": bool(info["graph"]),
+        "state_GiB": info["state_bytes"] / 2 ** 30,
+    }
+
+
 def cpu_baseline(n):
     """The reference's dense backend on this host: time each QFT-30 operation
     type on a 2^n state and extrapolate to the full op mix."""
@@ -207,6 +253,7 @@ def main():
     ap.add_argument("--skip-grover", action="store_true")
     ap.add_argument("--skip-c64", action="store_true")
     ap.add_argument("--skip-adjoint", action="store_true")
+    ap.add_argument("--skip-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); the state-vector engine runs on "
@@ -329,6 +376,16 @@ def main():
             adj64["speedup_vs_complex128_adjoint"] = adj["adjoint_s"] / adj64["adjoint_s"]
             adj["complex64"] = adj64
         result["adjoint"] = adj
+
+    # ---- batched uneven circuits: the seed-42 workload
+    if not args.skip_batch:
+        log("batched circuits: 256 circuits of 10-22 qubits (seed 42), complex128")
+        bt = time_batch(T, T.C128)
+        log(f"batch: {bt['batch_s'] * 1e3:.1f} ms per batch ({bt['circuits_per_s']:.0f} circuits/s), "
+            f"sequential {bt['sequential_s'] * 1e3:.1f} ms, tile {bt['tuned_tile_qubits']}")
+        if not args.skip_c64:
+            bt["complex64"] = time_batch(T, T.C64)
+        result["batch"] = bt
 
     cpu = None
     if not args.skip_cpu:

@@ -257,6 +257,44 @@ int tmfqs_hip_adjoint(tmfqs_hip_state *h, const tmfqs_hip_op *ops, size_t count,
                       const tmfqs_hip_diag_term *terms, size_t nterms, unsigned checkpoint_gates, double *expval, double *grads,
                       size_t grad_count);
 
+/* ---- batched circuits -------------------------------------------------- */
+/* No counterpart in the reference (its campaigns run circuits one
+ * Simulator::run at a time).  A batch owns the states of many independent
+ * circuits of uneven qubit counts (4..34), each starting from a basis state.
+ * Every circuit is planned like a register's gate run; round r of a batch run
+ * applies pass r of every circuit in one persistent work-list kernel, and the
+ * initialisation, the rounds and the final bit permutations are captured in
+ * one HIP graph, so tmfqs_hip_batch_run is a single graph launch.  Gate kinds:
+ * H, X, CX, CPHASE, PHASE, SWAP.  complex128 results are bit-identical to
+ * running each circuit through the reference's dense backend. */
+typedef struct tmfqs_hip_batch tmfqs_hip_batch;
+
+typedef struct tmfqs_hip_circuit {
+	unsigned qubits;
+	uint32_t reserved;
+	uint64_t init_state;     /* basis state the circuit starts from */
+	const tmfqs_hip_op *ops; /* borrowed for the duration of tmfqs_hip_batch_create */
+	size_t count;
+} tmfqs_hip_circuit;
+
+typedef struct tmfqs_hip_batch_stats {
+	size_t circuits, rounds, passes, items;
+	int tile_qubits; /* tile width of the current plan (config tile_qubits, or tuned) */
+	int graph;       /* 1 when runs are one graph launch */
+	double state_bytes;
+} tmfqs_hip_batch_stats;
+
+int tmfqs_hip_batch_create(const tmfqs_hip_config *cfg, const tmfqs_hip_circuit *circuits, size_t count, tmfqs_hip_batch **out);
+/* Autotuner: times the batch at each candidate tile width (fusion width of
+ * the tiled passes) and keeps the fastest; a config tile_qubits > 0 pins it. */
+int tmfqs_hip_batch_tune(tmfqs_hip_batch *b, int *tile_qubits);
+/* Initialises every state and runs every circuit; *seconds is the device time. */
+int tmfqs_hip_batch_run(tmfqs_hip_batch *b, double *seconds);
+/* Interleaved FP64 amplitudes (2 * 2^qubits doubles) of circuit `index`. */
+int tmfqs_hip_batch_export(tmfqs_hip_batch *b, size_t index, double *out);
+int tmfqs_hip_batch_info(tmfqs_hip_batch *b, tmfqs_hip_batch_stats *out);
+int tmfqs_hip_batch_destroy(tmfqs_hip_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

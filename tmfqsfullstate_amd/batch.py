@@ -1,0 +1,144 @@
+"""Batched circuits: ctypes binding of the tmfqs_hip_batch_* entry points of
+include/tmfqs_hip.h, and the text circuit-list format the batched workload is
+committed in.
+
+Circuit-list format: '#' comment lines; 'C <qubits> <gates>' opens a circuit;
+then one gate per line: 'H q', 'X q', 'CX c t', 'CP c t theta', 'PH q theta',
+'SW a b' (qubit numbering as everywhere in the API: qubit 0 is the most
+significant bit of the basis index).
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import (C128, Config, Op, OP_CPHASE, OP_CX, OP_H, OP_PHASE, OP_SWAP, OP_X, check, default_config, lib)
+
+_KINDS = {"H": OP_H, "X": OP_X, "CX": OP_CX, "CP": OP_CPHASE, "PH": OP_PHASE, "SW": OP_SWAP}
+
+
+class Circuit(ctypes.Structure):
+    _fields_ = [
+        ("qubits", ctypes.c_uint),
+        ("reserved", ctypes.c_uint32),
+        ("init_state", ctypes.c_uint64),
+        ("ops", ctypes.POINTER(Op)),
+        ("count", ctypes.c_size_t),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("circuits", ctypes.c_size_t),
+        ("rounds", ctypes.c_size_t),
+        ("passes", ctypes.c_size_t),
+        ("items", ctypes.c_size_t),
+        ("tile_qubits", ctypes.c_int),
+        ("graph", ctypes.c_int),
+        ("state_bytes", ctypes.c_double),
+    ]
+
+
+def load_circuit_list(path):
+    """Parses a circuit-list file into [(qubits, ops_array, count), ...]."""
+    out = []
+    cur = None
+    with open(path) as f:
+        for line in f:
+            line = line.strip()
+            if not line or line.startswith("#"):
+                continue
+            t = line.split()
+            if t[0] == "C":
+                cur = [int(t[1]), int(t[2]), []]
+                out.append(cur)
+                continue
+            kind = _KINDS[t[0]]
+            if t[0] == "CP":
+                cur[2].append((kind, int(t[1]), int(t[2]), float(t[3])))
+            elif t[0] == "PH":
+                cur[2].append((kind, int(t[1]), 0, float(t[2])))
+            elif t[0] in ("CX", "SW"):
+                cur[2].append((kind, int(t[1]), int(t[2])))
+            else:
+                cur[2].append((kind, int(t[1])))
+    circuits = []
+    for n, count, seq in out:
+        if count != len(seq):
+            raise ValueError(f"circuit header says {count} gates, found {len(seq)}")
+        arr = (Op * max(1, len(seq)))()
+        for i, g in enumerate(seq):
+            arr[i].kind = g[0]
+            arr[i].q0 = g[1]
+            arr[i].q1 = g[2] if len(g) > 2 else 0
+            arr[i].theta = g[3] if len(g) > 3 else 0.0
+        circuits.append((n, arr, len(seq)))
+    return circuits
+
+
+def _bind():
+    L = lib()
+    if not getattr(L, "_batch_bound", False):
+        vp = ctypes.c_void_p
+        L.tmfqs_hip_batch_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(Circuit), ctypes.c_size_t, ctypes.POINTER(vp)]
+        L.tmfqs_hip_batch_tune.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+        L.tmfqs_hip_batch_run.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+        L.tmfqs_hip_batch_export.argtypes = [vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_double)]
+        L.tmfqs_hip_batch_info.argtypes = [vp, ctypes.POINTER(Stats)]
+        L.tmfqs_hip_batch_destroy.argtypes = [vp]
+        L._batch_bound = True
+    return L
+
+
+class Batch:
+    """Many circuits run as one graph launch (tmfqs_hip_batch)."""
+
+    def __init__(self, circuits, precision=C128, init_states=None, **cfg):
+        L = _bind()
+        self.n = [c[0] for c in circuits]
+        arr = (Circuit * max(1, len(circuits)))()
+        for i, (n, ops, count) in enumerate(circuits):
+            arr[i].qubits = n
+            arr[i].init_state = 0 if init_states is None else int(init_states[i])
+            arr[i].ops = ctypes.cast(ops, ctypes.POINTER(Op))
+            arr[i].count = count
+        c = default_config(precision=precision, **cfg)
+        self._h = ctypes.c_void_p()
+        check(L.tmfqs_hip_batch_create(ctypes.byref(c), arr, len(circuits), ctypes.byref(self._h)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _bind().tmfqs_hip_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def tune(self):
+        t = ctypes.c_int()
+        check(_bind().tmfqs_hip_batch_tune(self._h, ctypes.byref(t)))
+        return t.value
+
+    def run(self):
+        s = ctypes.c_double()
+        check(_bind().tmfqs_hip_batch_run(self._h, ctypes.byref(s)))
+        return s.value
+
+    def export(self, i):
+        out = np.empty(2 << self.n[i], dtype=np.float64)
+        check(_bind().tmfqs_hip_batch_export(self._h, i, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def info(self):
+        s = Stats()
+        check(_bind().tmfqs_hip_batch_info(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in Stats._fields_}

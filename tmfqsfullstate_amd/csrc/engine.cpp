@@ -800,12 +800,38 @@ std::vector<TileStage> plan_stages(TileOp *ops, size_t cnt, unsigned T, unsigned
 	return out;
 }
 
-void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_t hi, const GateTarget *tgt) {
-	const unsigned n = h->n;
-	void *const A = tgt ? tgt->a : h->amp;
-	void *const B = tgt ? tgt->b : nullptr;
-	const bool dual = B != nullptr;
-	const bool ladders = !h->cfg.exact_phases && !dual;
+// One launch of a planned gate run.
+struct PassLaunch {
+	enum Kind { TILE, QFT_BLOCK, QFT_TAIL } kind = TILE;
+	unsigned pbot = 0, qc = 0;  // QFT_BLOCK
+	TilePass tp{};              // TILE
+	unsigned W = 0;
+	std::vector<TileOp> ops;
+	std::vector<TileStage> stages;
+	std::vector<double> mats;
+	std::vector<TwiddleTerm> terms;
+	std::vector<uint32_t> gidx;  // dual: global gradient slot of each TOP_GRAD slot
+	bool ldsops = false;
+};
+
+struct GatePlan {
+	std::vector<PassLaunch> launches;
+	std::vector<unsigned> map;  // data of nominal bit b sits at physical map[b] after the launches
+	bool permute = false;
+};
+
+struct PlanCfg {
+	int prec = 0;
+	bool dual = false;
+	bool ladders = true;  // fuse QFT ladders (not bit-identical to one product per gate)
+	unsigned T = 12, C = 4, qft_c = 5;
+};
+
+GatePlan plan_gates(const PlanCfg &cfg, unsigned n, const std::vector<QOp> &in, size_t lo, size_t hi,
+                    const std::vector<double> &qmats) {
+	GatePlan plan;
+	const bool dual = cfg.dual;
+	const bool ladders = cfg.ladders && !dual;
 	// 1. relabel swaps into a bit map; op bits become physical positions.
 	std::vector<unsigned> map(n);
 	for(unsigned b = 0; b < n; ++b) map[b] = b;
@@ -823,9 +849,8 @@ void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_
 		ops.push_back(op);
 	}
 	// 2. greedy grouping into passes.
-	unsigned T = tile_bits(h);
-	if(tgt && tgt->tile_delta) T = (unsigned)std::max<int>(1, std::min<int>((int)n, (int)T + tgt->tile_delta));
-	const unsigned C = coalesce_bits(h, T);
+	const unsigned T = std::max(1u, std::min(n, cfg.T));
+	const unsigned C = std::min(cfg.C, T);
 	const u64 lowC = (C >= 64) ? ~0ull : ((1ull << C) - 1ull);
 	std::vector<PassPlan> passes;
 	PassPlan cur;
@@ -847,15 +872,19 @@ void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_
 	for(size_t pi = 0; pi < passes.size(); ++pi) {
 		const PassPlan &P = passes[pi];
 		if(qft_fused && pi + 1 == passes.size()) {
-			ProfScope ps(h, "k_qft_tail_reverse", pass_bytes(h));
-			HIPC(launch_qft_tail_reverse(h->L(), A, n, h->d_cin));
+			PassLaunch L;
+			L.kind = PassLaunch::QFT_TAIL;
+			plan.launches.push_back(std::move(L));
 			continue;
 		}
 		unsigned pbot = 0;
-		const unsigned qc = qft_block_coalesce_bits(h->prec, h->cfg.coalesce_qubits);
+		const unsigned qc = cfg.qft_c;
 		if(ladders && n <= 40 && match_qft_block(ops, P.ops, qc, pbot)) {
-			ProfScope ps(h, "k_qft_block", pass_bytes(h));
-			HIPC(launch_qft_block(h->L(), A, n, pbot, h->d_phase, h->d_cin, qc));
+			PassLaunch L;
+			L.kind = PassLaunch::QFT_BLOCK;
+			L.pbot = pbot;
+			L.qc = qc;
+			plan.launches.push_back(std::move(L));
 			continue;
 		}
 		unsigned Cp = T;
@@ -915,7 +944,7 @@ void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_
 					t.kind = TOP_U1;
 					t.j0 = (unsigned)local(op.b0);
 					t.mat = (unsigned)mats.size();
-					mats.insert(mats.end(), h->qmats.begin() + op.mat, h->qmats.begin() + op.mat + 8);
+					mats.insert(mats.end(), qmats.begin() + op.mat, qmats.begin() + op.mat + 8);
 					tops.push_back(t);
 					++k;
 					break;
@@ -925,7 +954,7 @@ void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_
 					for(unsigned j = 0; j < op.k; ++j) t.jl[j] = (unsigned)local(op.bits[j]);
 					t.mat = (unsigned)mats.size();
 					const size_t dim = 1ull << op.k;
-					mats.insert(mats.end(), h->qmats.begin() + op.mat, h->qmats.begin() + op.mat + 2 * dim * dim);
+					mats.insert(mats.end(), qmats.begin() + op.mat, qmats.begin() + op.mat + 2 * dim * dim);
 					tops.push_back(t);
 					++k;
 					break;
@@ -1018,49 +1047,88 @@ void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_
 		tp.nh = (unsigned)high.size();
 		for(size_t i = 0; i < high.size(); ++i) tp.hbits[i] = high[i];
 		tp.ntiles = 1ull << (n - Tp);
-		const unsigned W = tile_stage_width(h->prec, Tp, dual);
+		const unsigned W = tile_stage_width(cfg.prec, Tp, dual);
 		// op lists longer than kMaxTileOps (or with more than kMaxGradSlots
 		// gradient reductions) are split across launches.
 		size_t s0 = 0;
 		while(s0 < tops.size()) {
+			PassLaunch L;
+			L.kind = PassLaunch::TILE;
 			size_t e = s0;
-			std::vector<uint32_t> gidx;
 			while(e < tops.size() && e - s0 < (size_t)kMaxTileOps) {
 				if(tops[e].kind == TOP_GRAD) {
-					if(gidx.size() == kMaxGradSlots) break;
-					tops[e].mat = (uint32_t)gidx.size();
-					gidx.push_back(tops[e].twid);
+					if(L.gidx.size() == kMaxGradSlots) break;
+					tops[e].mat = (uint32_t)L.gidx.size();
+					L.gidx.push_back(tops[e].twid);
 				}
 				++e;
 			}
-			const size_t cnt = e - s0;
-			const std::vector<TileStage> stages = plan_stages(tops.data() + s0, cnt, Tp, W);
-			const TileOp *dops = (const TileOp *)arena_upload(h, tops.data() + s0, cnt * sizeof(TileOp));
-			const TileStage *dst = (const TileStage *)arena_upload(h, stages.data(), stages.size() * sizeof(TileStage));
-			const double *dm = mats.empty() ? nullptr : (const double *)arena_upload(h, mats.data(), mats.size() * sizeof(double));
-			const TwiddleTerm *dt =
-			    allterms.empty() ? nullptr : (const TwiddleTerm *)arena_upload(h, allterms.data(), allterms.size() * sizeof(TwiddleTerm));
-			tp.nops = (unsigned)cnt;
-			tp.op0 = 0;
-			{
-				ProfScope ps(h, dual ? "k_tile_dual" : "k_tile", (dual ? 2 : 1) * pass_bytes(h));
-				bool ldsops = false;
-				for(const TileStage &st : stages) ldsops = ldsops || st.lds;
-				HIPC(launch_tile_staged(h->L(), A, B, tp, W, dops, dst, (unsigned)stages.size(), dm, dt, h->d_phase,
-				                        dual ? tgt->gpart : nullptr, (unsigned)gidx.size(), ldsops));
-			}
-			if(dual && !gidx.empty()) {
-				const uint32_t *dg = (const uint32_t *)arena_upload(h, gidx.data(), gidx.size() * sizeof(uint32_t));
-				HIPC(launch_grad_finish(h->L(), tgt->gpart, tile_grid(tp, true), (unsigned)gidx.size(), dg, tgt->grad_scale,
-				                        tgt->dgrads));
-			}
+			L.ops.assign(tops.begin() + s0, tops.begin() + e);
+			L.stages = plan_stages(L.ops.data(), L.ops.size(), Tp, W);
+			for(const TileStage &st : L.stages) L.ldsops = L.ldsops || st.lds;
+			L.mats = mats;
+			L.terms = allterms;
+			L.tp = tp;
+			L.tp.nops = (unsigned)L.ops.size();
+			L.tp.op0 = 0;
+			L.W = W;
+			plan.launches.push_back(std::move(L));
 			s0 = e;
 		}
 	}
-	// 4. restore the identity bit layout (the fused QFT tail already did).
-	if(qft_fused) return;
-	run_permutation(h, map, A);
-	if(dual) run_permutation(h, map, B);
+	// 4. the final bit layout (the fused QFT tail already restores it).
+	plan.map = map;
+	plan.permute = false;
+	if(!qft_fused)
+		for(unsigned b = 0; b < n; ++b) plan.permute = plan.permute || map[b] != b;
+	return plan;
+}
+
+void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_t hi, const GateTarget *tgt) {
+	const unsigned n = h->n;
+	void *const A = tgt ? tgt->a : h->amp;
+	void *const B = tgt ? tgt->b : nullptr;
+	const bool dual = B != nullptr;
+	PlanCfg cfg;
+	cfg.prec = h->prec;
+	cfg.dual = dual;
+	cfg.ladders = !h->cfg.exact_phases;
+	cfg.T = tile_bits(h);
+	if(tgt && tgt->tile_delta) cfg.T = (unsigned)std::max<int>(1, std::min<int>((int)n, (int)cfg.T + tgt->tile_delta));
+	cfg.C = coalesce_bits(h, cfg.T);
+	cfg.qft_c = qft_block_coalesce_bits(h->prec, h->cfg.coalesce_qubits);
+	const GatePlan plan = plan_gates(cfg, n, in, lo, hi, h->qmats);
+	for(const PassLaunch &L : plan.launches) {
+		if(L.kind == PassLaunch::QFT_TAIL) {
+			ProfScope ps(h, "k_qft_tail_reverse", pass_bytes(h));
+			HIPC(launch_qft_tail_reverse(h->L(), A, n, h->d_cin));
+			continue;
+		}
+		if(L.kind == PassLaunch::QFT_BLOCK) {
+			ProfScope ps(h, "k_qft_block", pass_bytes(h));
+			HIPC(launch_qft_block(h->L(), A, n, L.pbot, h->d_phase, h->d_cin, L.qc));
+			continue;
+		}
+		const TileOp *dops = (const TileOp *)arena_upload(h, L.ops.data(), L.ops.size() * sizeof(TileOp));
+		const TileStage *dst = (const TileStage *)arena_upload(h, L.stages.data(), L.stages.size() * sizeof(TileStage));
+		const double *dm = L.mats.empty() ? nullptr : (const double *)arena_upload(h, L.mats.data(), L.mats.size() * sizeof(double));
+		const TwiddleTerm *dt =
+		    L.terms.empty() ? nullptr : (const TwiddleTerm *)arena_upload(h, L.terms.data(), L.terms.size() * sizeof(TwiddleTerm));
+		{
+			ProfScope ps(h, dual ? "k_tile_dual" : "k_tile", (dual ? 2 : 1) * pass_bytes(h));
+			HIPC(launch_tile_staged(h->L(), A, B, L.tp, L.W, dops, dst, (unsigned)L.stages.size(), dm, dt, h->d_phase,
+			                        dual ? tgt->gpart : nullptr, (unsigned)L.gidx.size(), L.ldsops));
+		}
+		if(dual && !L.gidx.empty()) {
+			const uint32_t *dg = (const uint32_t *)arena_upload(h, L.gidx.data(), L.gidx.size() * sizeof(uint32_t));
+			HIPC(launch_grad_finish(h->L(), tgt->gpart, tile_grid(L.tp, true), (unsigned)L.gidx.size(), dg, tgt->grad_scale,
+			                        tgt->dgrads));
+		}
+	}
+	if(plan.permute) {
+		run_permutation(h, plan.map, A);
+		if(dual) run_permutation(h, plan.map, B);
+	}
 }
 
 // ------------------------------------------------------------ op records
@@ -2219,6 +2287,353 @@ int tmfqs_hip_adjoint(tmfqs_hip_state *h, const tmfqs_hip_op *ops, size_t count,
 		throw;
 	}
 	h->qmats.clear();
+	ABI_END
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ batched circuits
+// Many independent circuits of uneven size: each is planned like a single
+// register's gate run (tiled passes, swaps relabeled), then round r launches
+// pass r of every circuit as one persistent work-list kernel.  The rounds,
+// the basis-state initialisation and the final permutations are captured into
+// one HIP graph, so a batch run is a single graph launch.
+struct tmfqs_hip_batch {
+	tmfqs_hip_config cfg{};
+	int prec = 0;
+	size_t esz = 16;
+	hipStream_t stream = nullptr;
+	std::vector<unsigned> n;
+	std::vector<u64> init;
+	std::vector<std::vector<QOp>> ops;  // physical bit positions
+	char *dmem = nullptr;                // every circuit's two state buffers
+	std::vector<size_t> offA, offB;
+	unsigned T = 0;                      // tile bits of the current plan
+	std::vector<char> in_b;              // result left in the second buffer
+	struct Round {
+		size_t pass0, item0, nitems;
+	};
+	std::vector<Round> rounds;
+	char *dtab = nullptr;  // device tables of the plan
+	BatchPass *d_passes = nullptr;
+	BatchItem *d_items = nullptr;
+	TileOp *d_ops = nullptr;
+	TileStage *d_stages = nullptr;
+	BatchChunk *d_init = nullptr, *d_perm = nullptr;
+	unsigned ninit = 0, nperm = 0;
+	size_t npasses = 0, nitems = 0;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	bool use_graph = true;
+	Launch L() const { return Launch{stream, prec}; }
+};
+
+namespace {
+
+void batch_free_plan(tmfqs_hip_batch *b) {
+	if(b->exec) hipGraphExecDestroy(b->exec);
+	if(b->graph) hipGraphDestroy(b->graph);
+	b->exec = nullptr;
+	b->graph = nullptr;
+	if(b->dtab) hipFree(b->dtab);
+	b->dtab = nullptr;
+	b->rounds.clear();
+}
+
+void batch_enqueue(tmfqs_hip_batch *b) {
+	HIPC(launch_batch_init(b->L(), b->d_init, b->ninit));
+	const unsigned W = tile_stage_width(b->prec, b->T, false);
+	for(const auto &r : b->rounds)
+		HIPC(launch_batch_tiles(b->L(), W, b->d_passes + r.pass0, b->d_items + r.item0, (unsigned)r.nitems, b->d_ops, b->d_stages,
+		                        nullptr, 2048));
+	HIPC(launch_batch_permute(b->L(), b->d_perm, b->nperm));
+}
+
+// Plans every circuit with tiles of T bits and builds the device tables and
+// the graph of one batch run.
+void batch_plan(tmfqs_hip_batch *b, unsigned T) {
+	batch_free_plan(b);
+	b->T = T;
+	const size_t nc = b->n.size();
+	std::vector<GatePlan> plans(nc);
+	size_t R = 0;
+	for(size_t c = 0; c < nc; ++c) {
+		PlanCfg pc;
+		pc.prec = b->prec;
+		pc.dual = false;
+		pc.ladders = false;  // one product per controlled phase: bit-identical to the dense backend
+		pc.T = std::min(T, b->n[c]);
+		pc.C = std::min(pc.T, b->prec == 0 ? 4u : 5u);
+		plans[c] = plan_gates(pc, b->n[c], b->ops[c], 0, b->ops[c].size(), std::vector<double>());
+		for(const PassLaunch &l : plans[c].launches)
+			if(l.kind != PassLaunch::TILE || l.ldsops || l.W != tile_stage_width(b->prec, T, false))
+				throw std::invalid_argument("batched circuit needs a pass the batch kernel does not run");
+		R = std::max(R, plans[c].launches.size());
+	}
+	std::vector<BatchPass> passes;
+	std::vector<BatchItem> items;
+	std::vector<TileOp> ops;
+	std::vector<TileStage> stages;
+	for(size_t r = 0; r < R; ++r) {
+		tmfqs_hip_batch::Round rd{passes.size(), items.size(), 0};
+		for(size_t c = 0; c < nc; ++c) {
+			if(r >= plans[c].launches.size()) continue;
+			const PassLaunch &l = plans[c].launches[r];
+			BatchPass bp{};
+			bp.state = b->dmem + b->offA[c];
+			bp.tp = l.tp;
+			bp.op0 = (uint32_t)ops.size();
+			bp.st0 = (uint32_t)stages.size();
+			bp.nst = (uint32_t)l.stages.size();
+			ops.insert(ops.end(), l.ops.begin(), l.ops.end());
+			stages.insert(stages.end(), l.stages.begin(), l.stages.end());
+			const uint32_t local = (uint32_t)(passes.size() - rd.pass0);
+			passes.push_back(bp);
+			const u64 per = 4;  // tiles per work item
+			for(u64 t0 = 0; t0 < l.tp.ntiles; t0 += per) {
+				BatchItem it{};
+				it.pass = local;
+				it.tile0 = t0;
+				it.ntiles = (uint32_t)std::min<u64>(per, l.tp.ntiles - t0);
+				items.push_back(it);
+			}
+		}
+		rd.nitems = items.size() - rd.item0;
+		b->rounds.push_back(rd);
+	}
+	std::vector<BatchChunk> init, perm;
+	const u64 chunk = 1ull << 16;
+	b->in_b.assign(nc, 0);
+	for(size_t c = 0; c < nc; ++c) {
+		const u64 N = 1ull << b->n[c];
+		for(u64 f = 0; f < N; f += chunk) {
+			BatchChunk ch{};
+			ch.dst = b->dmem + b->offA[c];
+			ch.first = f;
+			ch.count = std::min(chunk, N - f);
+			ch.init = b->init[c];
+			ch.n = b->n[c];
+			init.push_back(ch);
+		}
+		if(plans[c].permute) {
+			b->in_b[c] = 1;
+			for(u64 f = 0; f < N; f += chunk) {
+				BatchChunk ch{};
+				ch.src = b->dmem + b->offA[c];
+				ch.dst = b->dmem + b->offB[c];
+				ch.first = f;
+				ch.count = std::min(chunk, N - f);
+				ch.n = b->n[c];
+				for(unsigned q = 0; q < b->n[c]; ++q) ch.map[q] = (uint8_t)plans[c].map[q];
+				perm.push_back(ch);
+			}
+		}
+	}
+	// one device allocation for every table
+	auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+	const size_t sp = al(passes.size() * sizeof(BatchPass)), si = al(items.size() * sizeof(BatchItem)),
+	             so = al(ops.size() * sizeof(TileOp)), ss = al(stages.size() * sizeof(TileStage)),
+	             sn = al(init.size() * sizeof(BatchChunk)), sm = al(perm.size() * sizeof(BatchChunk));
+	const size_t total = std::max<size_t>(256, sp + si + so + ss + sn + sm);
+	HIPC(hipMalloc((void **)&b->dtab, total));
+	std::vector<char> host(total, 0);
+	size_t off = 0;
+	auto put = [&](const void *src, size_t bytes, size_t span) {
+		if(bytes) std::memcpy(host.data() + off, src, bytes);
+		char *d = b->dtab + off;
+		off += span;
+		return d;
+	};
+	b->d_passes = (BatchPass *)put(passes.data(), passes.size() * sizeof(BatchPass), sp);
+	b->d_items = (BatchItem *)put(items.data(), items.size() * sizeof(BatchItem), si);
+	b->d_ops = (TileOp *)put(ops.data(), ops.size() * sizeof(TileOp), so);
+	b->d_stages = (TileStage *)put(stages.data(), stages.size() * sizeof(TileStage), ss);
+	b->d_init = (BatchChunk *)put(init.data(), init.size() * sizeof(BatchChunk), sn);
+	b->d_perm = (BatchChunk *)put(perm.data(), perm.size() * sizeof(BatchChunk), sm);
+	HIPC(hipMemcpy(b->dtab, host.data(), total, hipMemcpyHostToDevice));
+	b->ninit = (unsigned)init.size();
+	b->nperm = (unsigned)perm.size();
+	b->npasses = passes.size();
+	b->nitems = items.size();
+	// one eager run (kernel attributes are set outside capture), then the graph
+	batch_enqueue(b);
+	HIPC(hipStreamSynchronize(b->stream));
+	if(b->use_graph) {
+		HIPC(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+		try {
+			batch_enqueue(b);
+		} catch(...) {
+			hipGraph_t g = nullptr;
+			(void)hipStreamEndCapture(b->stream, &g);
+			if(g) hipGraphDestroy(g);
+			throw;
+		}
+		HIPC(hipStreamEndCapture(b->stream, &b->graph));
+		HIPC(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+	}
+}
+
+double batch_run_once(tmfqs_hip_batch *b) {
+	hipEvent_t e0, e1;
+	HIPC(hipEventCreate(&e0));
+	HIPC(hipEventCreate(&e1));
+	hipError_t err = hipEventRecord(e0, b->stream);
+	if(err == hipSuccess) {
+		if(b->exec)
+			err = hipGraphLaunch(b->exec, b->stream);
+		else {
+			try {
+				batch_enqueue(b);
+			} catch(...) {
+				hipEventDestroy(e0);
+				hipEventDestroy(e1);
+				throw;
+			}
+		}
+	}
+	if(err == hipSuccess) err = hipEventRecord(e1, b->stream);
+	if(err == hipSuccess) err = hipEventSynchronize(e1);
+	float ms = 0;
+	if(err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+	hipEventDestroy(e0);
+	hipEventDestroy(e1);
+	HIPC(err);
+	return ms * 1e-3;
+}
+
+void destroy_batch(tmfqs_hip_batch *b) {
+	if(!b) return;
+	hipSetDevice(b->cfg.device);
+	if(b->stream) hipStreamSynchronize(b->stream);
+	batch_free_plan(b);
+	hipFree(b->dmem);
+	if(b->stream) hipStreamDestroy(b->stream);
+	delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tmfqs_hip_batch_create(const tmfqs_hip_config *cfg, const tmfqs_hip_circuit *circuits, size_t count, tmfqs_hip_batch **out) {
+	tmfqs_hip_batch *b = nullptr;
+	ABI_BEGIN
+	if(!out) throw std::invalid_argument("tmfqs_hip_batch_create: null output");
+	*out = nullptr;
+	if(count && !circuits) throw std::invalid_argument("tmfqs_hip_batch_create: null circuit list");
+	b = new tmfqs_hip_batch();
+	tmfqs_hip_config_default(&b->cfg);
+	if(cfg) b->cfg = *cfg;
+	b->prec = b->cfg.precision == TMFQS_HIP_COMPLEX64 ? 1 : 0;
+	b->esz = b->prec == 0 ? 16 : 8;
+	HIPC(hipSetDevice(b->cfg.device));
+	HIPC(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+	size_t total = 0;
+	for(size_t c = 0; c < count; ++c) {
+		const tmfqs_hip_circuit &ci = circuits[c];
+		if(ci.qubits < 4 || ci.qubits > 34) throw std::invalid_argument("batched circuits need 4 to 34 qubits");
+		if(ci.init_state >> ci.qubits) throw std::out_of_range("batched circuit initial state out of range");
+		if(ci.count && !ci.ops) throw std::invalid_argument("batched circuit with a null operation list");
+		tmfqs_hip_state tmp;  // validation and conversion context only
+		tmp.n = ci.qubits;
+		tmp.N = 1ull << ci.qubits;
+		std::vector<QOp> q = convert_ops(&tmp, ci.ops, ci.count, nullptr, nullptr, false, nullptr);
+		for(const QOp &o : q)
+			if(o.kind == Q_U1 || o.kind == Q_UK) throw std::invalid_argument("batched circuits take H, X, CX, CPHASE, PHASE and SWAP");
+		b->n.push_back(ci.qubits);
+		b->init.push_back(ci.init_state);
+		b->ops.push_back(std::move(q));
+		const size_t bytes = ((size_t)1 << ci.qubits) * b->esz;
+		b->offA.push_back(total);
+		total += (bytes + 255) & ~size_t(255);
+		b->offB.push_back(total);
+		total += (bytes + 255) & ~size_t(255);
+	}
+	HIPC(hipMalloc((void **)&b->dmem, std::max<size_t>(total, 256)));
+	const unsigned maxT = batch_max_tile_bits(b->prec);
+	unsigned T = b->cfg.tile_qubits > 0 ? (unsigned)b->cfg.tile_qubits : maxT;
+	T = std::max(4u, std::min(T, maxT));
+	batch_plan(b, T);
+	*out = b;
+	b = nullptr;
+	return TMFQS_HIP_OK;
+	}
+	catch(...) {
+		destroy_batch(b);
+		return map_exception();
+	}
+}
+
+int tmfqs_hip_batch_tune(tmfqs_hip_batch *b, int *tile_qubits) {
+	ABI_BEGIN
+	if(!b) throw std::invalid_argument("null tmfqs_hip_batch handle");
+	HIPC(hipSetDevice(b->cfg.device));
+	if(b->cfg.tile_qubits <= 0) {
+		const unsigned maxT = batch_max_tile_bits(b->prec);
+		unsigned best = b->T;
+		double best_s = 1e300;
+		for(unsigned T = maxT - 3; T <= maxT; ++T) {
+			batch_plan(b, T);
+			batch_run_once(b);
+			const double s = std::min(batch_run_once(b), batch_run_once(b));
+			if(s < best_s) {
+				best_s = s;
+				best = T;
+			}
+		}
+		if(best != b->T) batch_plan(b, best);
+	}
+	if(tile_qubits) *tile_qubits = (int)b->T;
+	ABI_END
+}
+
+int tmfqs_hip_batch_run(tmfqs_hip_batch *b, double *seconds) {
+	ABI_BEGIN
+	if(!b) throw std::invalid_argument("null tmfqs_hip_batch handle");
+	HIPC(hipSetDevice(b->cfg.device));
+	const double s = batch_run_once(b);
+	if(seconds) *seconds = s;
+	ABI_END
+}
+
+int tmfqs_hip_batch_export(tmfqs_hip_batch *b, size_t index, double *out) {
+	ABI_BEGIN
+	if(!b) throw std::invalid_argument("null tmfqs_hip_batch handle");
+	if(index >= b->n.size()) throw std::out_of_range("tmfqs_hip_batch_export: circuit index out of range");
+	if(!out) throw std::invalid_argument("tmfqs_hip_batch_export: null output");
+	HIPC(hipSetDevice(b->cfg.device));
+	HIPC(hipStreamSynchronize(b->stream));
+	const u64 N = 1ull << b->n[index];
+	const char *src = b->dmem + (b->in_b[index] ? b->offB[index] : b->offA[index]);
+	if(b->prec == 0) {
+		HIPC(hipMemcpy(out, src, N * 16, hipMemcpyDeviceToHost));
+	} else {
+		std::vector<float> tmp(2 * N);
+		HIPC(hipMemcpy(tmp.data(), src, N * 8, hipMemcpyDeviceToHost));
+		for(u64 i = 0; i < 2 * N; ++i) out[i] = tmp[i];
+	}
+	ABI_END
+}
+
+int tmfqs_hip_batch_info(tmfqs_hip_batch *b, tmfqs_hip_batch_stats *out) {
+	ABI_BEGIN
+	if(!b || !out) throw std::invalid_argument("tmfqs_hip_batch_info: null argument");
+	out->circuits = b->n.size();
+	out->rounds = b->rounds.size();
+	out->passes = b->npasses;
+	out->items = b->nitems;
+	out->tile_qubits = (int)b->T;
+	out->graph = b->exec ? 1 : 0;
+	double bytes = 0, pass_bytes = 0;
+	for(size_t c = 0; c < b->n.size(); ++c) bytes += (double)(1ull << b->n[c]) * (double)b->esz;
+	(void)pass_bytes;
+	out->state_bytes = bytes;
+	ABI_END
+}
+
+int tmfqs_hip_batch_destroy(tmfqs_hip_batch *b) {
+	ABI_BEGIN
+	destroy_batch(b);
 	ABI_END
 }
 

@@ -186,4 +186,37 @@ struct ChunkTicks {
 hipError_t launch_chunk_ticks(const Launch &L, const void *a, unsigned n, const double *affine4, int sel,
                               const int32_t *dulog, ChunkTicks *dout);
 
+// ------------------------------------------------------ batched circuits
+// Batched tile passes hold at most 2^batch_max_tile_bits(prec) amplitudes per
+// tile (64 KiB of amplitudes: two workgroups per CU).
+__host__ __device__ constexpr unsigned batch_max_tile_bits(int prec) { return prec == 0 ? 12u : 13u; }
+struct BatchPass {
+	void *state;      // the circuit's state buffer
+	TilePass tp;      // geometry of the pass
+	uint32_t op0;     // its ops in the batch op pool
+	uint32_t st0;     // its stages in the batch stage pool
+	uint32_t nst;
+	uint32_t pad;
+};
+struct BatchItem {
+	uint32_t pass;    // index into the round's pass table
+	uint32_t ntiles;  // tiles [tile0, tile0 + ntiles) of that pass
+	u64 tile0;
+};
+// A chunk of one circuit's state: initialisation (dst, init) or final
+// permutation (src -> dst through map).
+struct BatchChunk {
+	const void *src;
+	void *dst;
+	u64 first, count;
+	u64 init;
+	uint32_t n, pad;
+	uint8_t map[64];
+};
+hipError_t launch_batch_init(const Launch &L, const BatchChunk *dchunks, unsigned nchunks);
+hipError_t launch_batch_permute(const Launch &L, const BatchChunk *dchunks, unsigned nchunks);
+// One round: W is the register-stage width every pass of the round uses.
+hipError_t launch_batch_tiles(const Launch &L, unsigned W, const BatchPass *dpasses, const BatchItem *ditems, unsigned nitems,
+                              const TileOp *dops, const TileStage *dstages, const double *dphase, unsigned grid);
+
 }  // namespace tmfqs_hip

@@ -1283,6 +1283,123 @@ __device__ __forceinline__ void reg_diag(V *v, const TileOp &op, const unsigned 
 	}
 }
 
+// One tile of a staged pass: loads it into LDS, runs every stage, stores it.
+// ta/tb/hitab/gacc are the workgroup's LDS buffers; hitab must hold P's table.
+template <typename R, int W, bool DUAL, bool LDSOPS>
+__device__ __forceinline__ void staged_tile(typename V2T<R>::t *__restrict__ a, typename V2T<R>::t *__restrict__ b,
+                                            const TilePass &P, u64 tid, const TileOp *__restrict__ ops,
+                                            const TileStage *__restrict__ stages, unsigned nstages,
+                                            const double *__restrict__ mats, const TwiddleTerm *__restrict__ terms,
+                                            const double2 *__restrict__ ptab, typename V2T<R>::t *ta,
+                                            typename V2T<R>::t *tb, const u64 *hitab, double *gacc) {
+	using V = typename V2T<R>::t;
+	constexpr int NR = 1 << W;
+	constexpr bool PAD = !LDSOPS;
+	const unsigned nloc = 1u << P.T;
+	const unsigned C = P.C;
+	const u64 lowmask = (1ull << C) - 1ull;
+	const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+	const R hc = (R)(1.0 / sqrt(2.0));
+	const unsigned ngrp = nloc >> W;
+	const unsigned iters = (ngrp + blockDim.x - 1) / blockDim.x;
+	u64 tbase = tid << C;
+	for(unsigned bb = 0; bb < P.nh; ++bb) tbase = ins0(tbase, P.hbits[bb]);
+	__syncthreads();
+	for(unsigned l = threadIdx.x; l < nloc; l += blockDim.x) {
+		const u64 g = tbase | (l & lowmask) | hitab[l >> C];
+		const unsigned pl = PAD ? l + (l >> 4) : l;
+		ta[pl] = nt_load(&a[g]);
+		if(DUAL) tb[pl] = nt_load(&b[g]);
+	}
+	__syncthreads();
+	for(unsigned s = 0; s < nstages; ++s) {
+		const TileStage st = stages[s];
+		if(LDSOPS && st.lds) {  // k-qubit matrices: their registers would cap every other stage's occupancy
+			for(unsigned o = 0; o < st.nops; ++o) {
+				const TileOp op = ops[st.op0 + o];
+				if((tbase & op.gcond) != op.gcond) continue;  // uniform across the block
+				tile_apply<R>(ta, op, nloc, tbase, C, hitab, mats, terms, ptab);
+				if(DUAL) tile_apply<R>(tb, op, nloc, tbase, C, hitab, mats, terms, ptab);
+				__syncthreads();
+			}
+			continue;
+		}
+		unsigned sbits[W];
+#pragma unroll
+		for(int j = 0; j < W; ++j) sbits[j] = st.bits[j];
+		for(unsigned it = 0; it < iters; ++it) {
+			const unsigned grp = threadIdx.x + it * blockDim.x;
+			const bool act = grp < ngrp;
+			unsigned lb = act ? grp : 0u;
+#pragma unroll
+			for(int j = 0; j < W; ++j) lb = ((lb >> sbits[j]) << (sbits[j] + 1)) | (lb & ((1u << sbits[j]) - 1u));
+			unsigned lidx[NR];
+			V va[NR], vb[NR];
+#pragma unroll
+			for(int r = 0; r < NR; ++r) {
+				unsigned off = 0;
+#pragma unroll
+				for(int j = 0; j < W; ++j)
+					if((r >> j) & 1) off |= 1u << sbits[j];
+				lidx[r] = lb | off;
+				if(act) {
+					const unsigned pl = PAD ? lidx[r] + (lidx[r] >> 4) : lidx[r];
+					va[r] = ta[pl];
+					if(DUAL) vb[r] = tb[pl];
+				}
+			}
+			for(unsigned o = 0; o < st.nops; ++o) {
+				const TileOp op = ops[st.op0 + o];
+				if((tbase & op.gcond) != op.gcond) continue;  // uniform across the block
+				if(op.kind == TOP_GRAD) {
+					if(DUAL) {
+						double acc = 0.0;
+						if(act) {
+#pragma unroll
+							for(int r = 0; r < NR; ++r)
+								if((lidx[r] & op.lcond) == op.lcond)
+									acc += (double)vb[r].x * (double)va[r].y - (double)vb[r].y * (double)va[r].x;
+						}
+						acc = wave_sum(acc);
+						if(lane == 0) gacc[wave * kMaxGradSlots + op.mat] += acc;
+					}
+					continue;
+				}
+				if(!act) continue;
+				if(op.kind == TOP_H || op.kind == TOP_X || op.kind == TOP_U1) {
+					reg_nondiag<V, R, W>(va, op, lidx, mats, hc);
+					if(DUAL) reg_nondiag<V, R, W>(vb, op, lidx, mats, hc);
+				} else {
+					reg_diag<V, R, W>(va, op, lidx, lb, sbits, tbase, lowmask, C, hitab, terms, ptab);
+					if(DUAL) reg_diag<V, R, W>(vb, op, lidx, lb, sbits, tbase, lowmask, C, hitab, terms, ptab);
+				}
+			}
+			if(act) {
+#pragma unroll
+				for(int r = 0; r < NR; ++r) {
+					const unsigned pl = PAD ? lidx[r] + (lidx[r] >> 4) : lidx[r];
+					ta[pl] = va[r];
+					if(DUAL) tb[pl] = vb[r];
+				}
+			}
+		}
+		__syncthreads();
+	}
+	for(unsigned l = threadIdx.x; l < nloc; l += blockDim.x) {
+		const u64 g = tbase | (l & lowmask) | hitab[l >> C];
+		const unsigned pl = PAD ? l + (l >> 4) : l;
+		nt_store(ta[pl], &a[g]);
+		if(DUAL) nt_store(tb[pl], &b[g]);
+	}
+}
+
+// LDS elements of one staged tile of 2^T amplitudes (one pad slot per 16
+// amplitudes keeps register-stage gathers over low bit positions, strided
+// across lanes, off a single bank; LDS-op stages index the tile directly).
+__host__ __device__ __forceinline__ unsigned staged_tile_elems(unsigned T, bool ldsops) {
+	return ldsops ? (1u << T) : (1u << T) + ((1u << T) >> 4);
+}
+
 template <typename R, int W, bool DUAL, bool LDSOPS>
 __global__ void __launch_bounds__(256) k_tile_staged(typename V2T<R>::t *__restrict__ a, typename V2T<R>::t *__restrict__ b,
                                                      TilePass P, const TileOp *__restrict__ ops,
@@ -1291,123 +1408,74 @@ __global__ void __launch_bounds__(256) k_tile_staged(typename V2T<R>::t *__restr
                                                      const double2 *__restrict__ ptab, double *__restrict__ gpart,
                                                      unsigned nslots) {
 	using V = typename V2T<R>::t;
-	constexpr int NR = 1 << W;
-	// One pad slot per 16 amplitudes keeps register-stage gathers over low
-	// bit positions (strided across lanes) off a single LDS bank; LDS-op
-	// stages index the tile directly, so their build keeps it unpadded.
-	constexpr bool PAD = !LDSOPS;
 	extern __shared__ char smem[];
-	const unsigned nloc = 1u << P.T;
-	const unsigned nlocp = PAD ? nloc + (nloc >> 4) : nloc;
+	const unsigned nlocp = staged_tile_elems(P.T, LDSOPS);
 	V *ta = (V *)smem;
 	V *tb = ta + nlocp;  // second tile (dual passes only)
 	u64 *hitab = (u64 *)(ta + (DUAL ? 2u : 1u) * nlocp);
 	double *gacc = (double *)(hitab + (1u << P.nh));  // dual: [4 waves][kMaxGradSlots]
-	const unsigned C = P.C;
-	const u64 lowmask = (1ull << C) - 1ull;
-	const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-	const R hc = (R)(1.0 / sqrt(2.0));
-	const unsigned ngrp = nloc >> W;
-	const unsigned iters = (ngrp + blockDim.x - 1) / blockDim.x;
 	tile_hitab<R>(hitab, P);
 	if(DUAL)
 		for(unsigned i = threadIdx.x; i < 4 * kMaxGradSlots; i += blockDim.x) gacc[i] = 0.0;
-	for(u64 tid = blockIdx.x; tid < P.ntiles; tid += gridDim.x) {
-		u64 tbase = tid << C;
-		for(unsigned bb = 0; bb < P.nh; ++bb) tbase = ins0(tbase, P.hbits[bb]);
-		__syncthreads();
-		for(unsigned l = threadIdx.x; l < nloc; l += blockDim.x) {
-			const u64 g = tbase | (l & lowmask) | hitab[l >> C];
-			const unsigned pl = PAD ? l + (l >> 4) : l;
-			ta[pl] = nt_load(&a[g]);
-			if(DUAL) tb[pl] = nt_load(&b[g]);
-		}
-		__syncthreads();
-		for(unsigned s = 0; s < nstages; ++s) {
-			const TileStage st = stages[s];
-			if(LDSOPS && st.lds) {  // k-qubit matrices: their registers would cap every other stage's occupancy
-				for(unsigned o = 0; o < st.nops; ++o) {
-					const TileOp op = ops[st.op0 + o];
-					if((tbase & op.gcond) != op.gcond) continue;  // uniform across the block
-					tile_apply<R>(ta, op, nloc, tbase, C, hitab, mats, terms, ptab);
-					if(DUAL) tile_apply<R>(tb, op, nloc, tbase, C, hitab, mats, terms, ptab);
-					__syncthreads();
-				}
-				continue;
-			}
-			unsigned sbits[W];
-#pragma unroll
-			for(int j = 0; j < W; ++j) sbits[j] = st.bits[j];
-			for(unsigned it = 0; it < iters; ++it) {
-				const unsigned grp = threadIdx.x + it * blockDim.x;
-				const bool act = grp < ngrp;
-				unsigned lb = act ? grp : 0u;
-#pragma unroll
-				for(int j = 0; j < W; ++j) lb = ((lb >> sbits[j]) << (sbits[j] + 1)) | (lb & ((1u << sbits[j]) - 1u));
-				unsigned lidx[NR];
-				V va[NR], vb[NR];
-#pragma unroll
-				for(int r = 0; r < NR; ++r) {
-					unsigned off = 0;
-#pragma unroll
-					for(int j = 0; j < W; ++j)
-						if((r >> j) & 1) off |= 1u << sbits[j];
-					lidx[r] = lb | off;
-					if(act) {
-						const unsigned pl = PAD ? lidx[r] + (lidx[r] >> 4) : lidx[r];
-						va[r] = ta[pl];
-						if(DUAL) vb[r] = tb[pl];
-					}
-				}
-				for(unsigned o = 0; o < st.nops; ++o) {
-					const TileOp op = ops[st.op0 + o];
-					if((tbase & op.gcond) != op.gcond) continue;  // uniform across the block
-					if(op.kind == TOP_GRAD) {
-						if(DUAL) {
-							double acc = 0.0;
-							if(act) {
-#pragma unroll
-								for(int r = 0; r < NR; ++r)
-									if((lidx[r] & op.lcond) == op.lcond)
-										acc += (double)vb[r].x * (double)va[r].y - (double)vb[r].y * (double)va[r].x;
-							}
-							acc = wave_sum(acc);
-							if(lane == 0) gacc[wave * kMaxGradSlots + op.mat] += acc;
-						}
-						continue;
-					}
-					if(!act) continue;
-					if(op.kind == TOP_H || op.kind == TOP_X || op.kind == TOP_U1) {
-						reg_nondiag<V, R, W>(va, op, lidx, mats, hc);
-						if(DUAL) reg_nondiag<V, R, W>(vb, op, lidx, mats, hc);
-					} else {
-						reg_diag<V, R, W>(va, op, lidx, lb, sbits, tbase, lowmask, C, hitab, terms, ptab);
-						if(DUAL) reg_diag<V, R, W>(vb, op, lidx, lb, sbits, tbase, lowmask, C, hitab, terms, ptab);
-					}
-				}
-				if(act) {
-#pragma unroll
-					for(int r = 0; r < NR; ++r) {
-						const unsigned pl = PAD ? lidx[r] + (lidx[r] >> 4) : lidx[r];
-						ta[pl] = va[r];
-						if(DUAL) tb[pl] = vb[r];
-					}
-				}
-			}
-			__syncthreads();
-		}
-		for(unsigned l = threadIdx.x; l < nloc; l += blockDim.x) {
-			const u64 g = tbase | (l & lowmask) | hitab[l >> C];
-			const unsigned pl = PAD ? l + (l >> 4) : l;
-			nt_store(ta[pl], &a[g]);
-			if(DUAL) nt_store(tb[pl], &b[g]);
-		}
-	}
+	for(u64 tid = blockIdx.x; tid < P.ntiles; tid += gridDim.x)
+		staged_tile<R, W, DUAL, LDSOPS>(a, b, P, tid, ops, stages, nstages, mats, terms, ptab, ta, tb, hitab, gacc);
 	if(DUAL) {
 		__syncthreads();
 		for(unsigned s = threadIdx.x; s < nslots; s += blockDim.x)
 			gpart[(u64)s * gridDim.x + blockIdx.x] =
 			    ((gacc[s] + gacc[kMaxGradSlots + s]) + gacc[2 * kMaxGradSlots + s]) + gacc[3 * kMaxGradSlots + s];
+	}
+}
+
+// ------------------------------------------------------------------------
+// Batched circuits: one launch per round applies one pass of every circuit
+// that has one, as a work list of (pass, tile range) items that persistent
+// workgroups walk.
+template <typename R, int W>
+__global__ void __launch_bounds__(256) k_batch_tiles(const BatchPass *__restrict__ passes, const BatchItem *__restrict__ items,
+                                                     unsigned nitems, const TileOp *__restrict__ ops,
+                                                     const TileStage *__restrict__ stages, const double2 *__restrict__ ptab) {
+	using V = typename V2T<R>::t;
+	extern __shared__ char smem[];
+	V *ta = (V *)smem;
+	u64 *hitab = (u64 *)(ta + staged_tile_elems(batch_max_tile_bits(sizeof(R) == 8 ? 0 : 1), false));
+	for(unsigned it = blockIdx.x; it < nitems; it += gridDim.x) {
+		const BatchItem item = items[it];
+		const BatchPass bp = passes[item.pass];
+		__syncthreads();  // the previous item's tiles are done with hitab
+		tile_hitab<R>(hitab, bp.tp);
+		for(u64 tid = item.tile0; tid < item.tile0 + item.ntiles; ++tid)
+			staged_tile<R, W, false, false>((V *)bp.state, nullptr, bp.tp, tid, ops + bp.op0, stages + bp.st0, bp.nst, nullptr,
+			                                nullptr, ptab, ta, nullptr, hitab, nullptr);
+	}
+}
+
+// Basis-state initialisation of every circuit (items: chunks of one state).
+template <typename R>
+__global__ void __launch_bounds__(256) k_batch_init(const BatchChunk *__restrict__ chunks, unsigned nchunks) {
+	using V = typename V2T<R>::t;
+	for(unsigned c = blockIdx.x; c < nchunks; c += gridDim.x) {
+		const BatchChunk ch = chunks[c];
+		V *s = (V *)ch.dst;
+		for(u64 i = ch.first + threadIdx.x; i < ch.first + ch.count; i += blockDim.x)
+			s[i] = i == ch.init ? mk<V>(1.0, 0.0) : mk<V>(0.0, 0.0);
+	}
+}
+
+// Final bit layout of every circuit whose swaps were relabeled:
+// dst[x] = src[y], bit map[b] of y = bit b of x.
+template <typename R>
+__global__ void __launch_bounds__(256) k_batch_permute(const BatchChunk *__restrict__ chunks, unsigned nchunks) {
+	using V = typename V2T<R>::t;
+	for(unsigned c = blockIdx.x; c < nchunks; c += gridDim.x) {
+		const BatchChunk ch = chunks[c];
+		const V *src = (const V *)ch.src;
+		V *dst = (V *)ch.dst;
+		for(u64 x = ch.first + threadIdx.x; x < ch.first + ch.count; x += blockDim.x) {
+			u64 y = 0;
+			for(unsigned b = 0; b < ch.n; ++b) y |= ((x >> b) & 1ull) << ch.map[b];
+			dst[x] = src[y];
+		}
 	}
 }
 
@@ -1649,7 +1717,7 @@ static hipError_t launch_staged_t(const Launch &L, void *a, void *b, const TileP
                                   unsigned nstages, const double *dmats, const TwiddleTerm *dterms, const double *dphase,
                                   double *gpart, unsigned nslots) {
 	using V = typename V2T<R>::t;
-	const size_t nlocp = LDSOPS ? (1ull << P.T) : (1ull << P.T) + ((1ull << P.T) >> 4);
+	const size_t nlocp = staged_tile_elems(P.T, LDSOPS);
 	const size_t shmem = (DUAL ? 2 : 1) * sizeof(V) * nlocp + (8ull << P.nh) + (DUAL ? 4 * kMaxGradSlots * sizeof(double) : 0);
 	if(shmem > 160 * 1024) return hipErrorInvalidValue;
 	static bool attr = false;
@@ -1716,6 +1784,55 @@ hipError_t launch_observable(const Launch &L, const void *a, void *lam, unsigned
 		TMFQS_LAUNCH(k_observable<float>, g, 256, 0, L.stream, (const float2 *)a, (float2 *)lam, N, O, dpart);
 	TMFQS_LAUNCH(k_finish, 1, 64, 0, L.stream, dpart, g, 1u, dout);
 	return hipGetLastError();
+}
+
+hipError_t launch_batch_init(const Launch &L, const BatchChunk *dchunks, unsigned nchunks) {
+	if(!nchunks) return hipSuccess;
+	const unsigned g = nchunks < 8192 ? nchunks : 8192;
+	if(L.prec == 0)
+		TMFQS_LAUNCH(k_batch_init<double>, g, 256, 0, L.stream, dchunks, nchunks);
+	else
+		TMFQS_LAUNCH(k_batch_init<float>, g, 256, 0, L.stream, dchunks, nchunks);
+	return hipGetLastError();
+}
+
+hipError_t launch_batch_permute(const Launch &L, const BatchChunk *dchunks, unsigned nchunks) {
+	if(!nchunks) return hipSuccess;
+	const unsigned g = nchunks < 8192 ? nchunks : 8192;
+	if(L.prec == 0)
+		TMFQS_LAUNCH(k_batch_permute<double>, g, 256, 0, L.stream, dchunks, nchunks);
+	else
+		TMFQS_LAUNCH(k_batch_permute<float>, g, 256, 0, L.stream, dchunks, nchunks);
+	return hipGetLastError();
+}
+
+template <typename R, int W>
+static hipError_t launch_batch_tiles_t(const Launch &L, const BatchPass *dpasses, const BatchItem *ditems, unsigned nitems,
+                                       const TileOp *dops, const TileStage *dstages, const double *dphase, unsigned grid) {
+	using V = typename V2T<R>::t;
+	const size_t shmem = sizeof(V) * staged_tile_elems(batch_max_tile_bits(L.prec), false) + (8ull << kMaxHighBits);
+	static bool attr = false;
+	if(!attr) {
+		(void)hipFuncSetAttribute((const void *)k_batch_tiles<R, W>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+		attr = true;
+	}
+	TMFQS_LAUNCH((k_batch_tiles<R, W>), grid, 256, shmem, L.stream, dpasses, ditems, nitems, dops, dstages,
+	             (const double2 *)dphase);
+	return hipGetLastError();
+}
+
+hipError_t launch_batch_tiles(const Launch &L, unsigned W, const BatchPass *dpasses, const BatchItem *ditems, unsigned nitems,
+                              const TileOp *dops, const TileStage *dstages, const double *dphase, unsigned grid) {
+	if(!nitems) return hipSuccess;
+	if(grid > nitems) grid = nitems;
+	if(L.prec == 0) {
+		if(W == 4) return launch_batch_tiles_t<double, 4>(L, dpasses, ditems, nitems, dops, dstages, dphase, grid);
+		if(W == 3) return launch_batch_tiles_t<double, 3>(L, dpasses, ditems, nitems, dops, dstages, dphase, grid);
+	} else {
+		if(W == 4) return launch_batch_tiles_t<float, 4>(L, dpasses, ditems, nitems, dops, dstages, dphase, grid);
+		if(W == 3) return launch_batch_tiles_t<float, 3>(L, dpasses, ditems, nitems, dops, dstages, dphase, grid);
+	}
+	return hipErrorInvalidValue;
 }
 
 }  // namespace tmfqs_hip
