@@ -33,12 +33,13 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--qubits 30]
 import argparse
 import ctypes
 import json
-
-import numpy as np
+import math
 import os
 import platform
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -72,7 +73,9 @@ def read_traffic(kernel, workload):
 
 
 def time_qft(S, n, precision, steps, warmup, seed=7):
+    t0 = time.perf_counter()
     reg = S.Register.random_phase(n, seed, precision=precision)
+    log(f"randomPhase({n}, {seed}) generated and loaded in {time.perf_counter() - t0:.1f} s")
     ops, cnt = S.qft_ops(n)
     hs = reg.hip_state()
     for _ in range(warmup):
@@ -154,6 +157,72 @@ def time_adjoint(T, S, precision, n=24, layers=8, seed=3, steps=3):
     return out, g
 
 
+def time_grover(T, S, n, marked, precision):
+    """makeGrover(n, marked, materialized) through Simulator::run's timed
+    region from QuantumState::basis(n); returns (register, report)."""
+    ops, cnt = S.grover_ops(n, marked, True)
+    reg = S.Register.basis(n, 0, precision=precision)
+    hs = reg.hip_state()
+    hs.profile(True)
+    hs.profile_reset()
+    secs = reg.run(ops, cnt)
+    prof = hs.profile_read()
+    hs.profile(False)
+    p = sum(abs(reg.amplitude(m)) ** 2 for m in marked)
+    m = len(set(marked))
+    iters = (cnt - n) // (m + 1)
+    gbytes = sum(v["bytes"] for v in prof.values())
+    gms = sum(v["ms"] for v in prof.values())
+    k = math.floor(math.pi / 4 * math.sqrt(2.0 ** n / m))
+    closed = math.sin((2 * k + 1) * math.asin(math.sqrt(m / 2.0 ** n))) ** 2
+    return reg, {
+        "qubits": n,
+        "marked": list(marked),
+        "precision": "complex128" if precision == T.C128 else "complex64",
+        "iterations": iters,
+        "wall_s": secs,
+        "iterations_per_s": iters / secs,
+        "p_marked": p,
+        "p_marked_closed_form": closed,
+        "p_marked_error": abs(p - closed),
+        "measured_seed_123456": reg.measure_seed(123456),
+        "hbm_bytes": gbytes,
+        "hbm_GBps_over_kernel_time": gbytes / (gms * 1e-3) / 1e9 if gms > 0 else None,
+        "kernels": prof,
+    }
+
+
+def time_qft32(T, S, n=32, seed=11):
+    """QFT-32 in complex64 on QuantumState::randomPhase(32, 11) against the
+    complex128 result of the same engine (the FP64 oracle at this size)."""
+    ops, cnt = S.qft_ops(n)
+    t0 = time.perf_counter()
+    r64 = S.Register.random_phase(n, seed, precision=T.C64)
+    gen_s = time.perf_counter() - t0
+    h64 = r64.hip_state()
+    secs64 = [r64.run(ops, cnt)]
+    r64 = None
+    log(f"QFT-{n} complex64: {secs64[0] * 1e3:.1f} ms (input generation + load {gen_s:.1f} s)")
+    r64b = S.Register.random_phase(n, seed, precision=T.C64)
+    r128 = S.Register.random_phase(n, seed, precision=T.C128)
+    h128 = r128.hip_state()
+    secs64.append(r64b.run(ops, cnt))
+    secs128 = r128.run(ops, cnt)
+    cmp = h128.compare(r64b.hip_state())
+    del h64
+    return {
+        "qubits": n,
+        "seed": seed,
+        "complex64_s": min(secs64),
+        "complex128_s": secs128,
+        "speedup_vs_complex128": secs128 / min(secs64),
+        "fidelity": cmp["fidelity"],
+        "fidelity_error": 1.0 - cmp["fidelity"],
+        "max_abs_error": cmp["max_abs_amplitude_error"],
+        "state_GiB_complex64": (8 << n) / 2 ** 30,
+    }
+
+
 def time_batch(T, precision, steps=3):
     """The committed seed-42 workload (256 circuits, 10-22 qubits) as one
     batch (autotuned tile width, one graph launch per run) against the same
@@ -215,7 +284,6 @@ def cpu_baseline(n):
             if rc:
                 raise RuntimeError(f"orc_time_ops failed {rc}")
             return s.value
-    import math
     samples = {
         "h": [(O.OP_H, 0)],
         "cphase": [(O.OP_CPHASE, 1, 0, math.pi / 2), (O.OP_CPHASE, 29 % n, 3, math.pi / 2 ** 26)],
@@ -254,6 +322,7 @@ def main():
     ap.add_argument("--skip-c64", action="store_true")
     ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--qft32", action="store_true", help="also run QFT-32 complex64 vs complex128 (96 GiB of HBM)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); the state-vector engine runs on "
@@ -363,6 +432,30 @@ def main():
         }
         log(f"Grover-{n}: {gsecs:.3f} s, P={abs(a) ** 2:.15f}, measured {meas}")
         del greg, gh
+
+    # ---- Grover-n with three marked states, complex64 against complex128
+    if not args.skip_grover and not args.skip_c64:
+        marked3 = [1, (1 << (n - 1)) + 5, (1 << n) - 1]
+        log(f"Grover-{n} with 3 marked states, complex64 and complex128")
+        g64, rep64 = time_grover(T, S, n, marked3, T.C64)
+        g128, rep128 = time_grover(T, S, n, marked3, T.C128)
+        cmp = g128.hip_state().compare(g64.hip_state())
+        rep64["speedup_vs_complex128"] = rep128["wall_s"] / rep64["wall_s"]
+        rep64["fidelity_vs_complex128"] = cmp["fidelity"]
+        rep64["fidelity_error"] = 1.0 - cmp["fidelity"]
+        rep64["max_abs_error_vs_complex128"] = cmp["max_abs_amplitude_error"]
+        result["grover3_complex64"] = rep64
+        result["grover3_complex128"] = rep128
+        log(f"Grover-{n} x3: complex64 {rep64['wall_s']:.3f} s, complex128 {rep128['wall_s']:.3f} s, "
+            f"P {rep64['p_marked']:.12f} / {rep128['p_marked']:.12f}")
+        del g64, g128
+    if not args.skip_grover:
+        g20, rep20 = time_grover(T, S, 20, [524287], T.C128)
+        result["grover20"] = rep20
+        del g20
+
+    if args.qft32:
+        result["qft32_complex64"] = time_qft32(T, S)
 
     # ---- adjoint gradients: n = 24, 8 ring layers, 192 angles
     if not args.skip_adjoint:

@@ -134,3 +134,31 @@ def test_errors_map_to_exceptions():
         S.grover_ops(3, [8], True)
     with pytest.raises(T.InvalidArgument):
         S.Register.basis(64)
+
+
+def test_config_qft26_random_phase_vs_fft():
+    # QFT-26 on QuantumState::randomPhase(26, 7) (the QFT parity workload) against the DFT
+    n = 26
+    print("QFT-26 randomPhase(26, 7)", flush=True)
+    reg = S.Register.random_phase(n, 7)
+    ops, cnt = S.qft_ops(n)
+    reg.run(ops, cnt)
+    got = reg.amplitudes()
+    init = S.random_phase(n, 7)
+    want = np.fft.ifft(init[0::2] + 1j * init[1::2], norm="ortho")
+    assert np.abs(got - want).max() <= 1e-12
+
+
+def test_config_grover20_bitexact():
+    # Grover n=20, marked 524287, 804 iterations, materialized diffusion; measure(Mt19937(123456))
+    n, marked = 20, 524287
+    print("Grover-20 vs oracle", flush=True)
+    ops, cnt = S.grover_ops(n, [marked], True)
+    reg, _ = S.simulate(n, ops, cnt)
+    oops, ocnt = O.grover_ops(n, [marked], True)
+    ref = O.run_circuit(n, (oops, ocnt))
+    assert np.array_equal(reg.export(), ref.amplitudes())
+    p = abs(reg.amplitude(marked)) ** 2
+    assert abs(p - 0.999999756965361) < 1e-12
+    draw = O.mt_unit_draws(123456, 1)[0]
+    assert reg.measure_seed(123456) == ref.measure(draw) == marked

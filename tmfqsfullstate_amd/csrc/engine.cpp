@@ -420,10 +420,40 @@ void sync_all(tmfqs_hip_state *h) {
 	HIPC(hipStreamSynchronize(h->stream));
 }
 
+// Default most elementwise steps per flush: complex128 replays each step's
+// rounding (bit-identical to the dense backend), so a sweep carries 96 steps;
+// complex64 composes a whole run into one affine map.
+unsigned ew_block(const tmfqs_hip_state *h) {
+	if(h->cfg.grover_block > 0) return (unsigned)h->cfg.grover_block;
+	return h->prec == 0 ? 96u : 65536u;
+}
+
 void exec_elementwise(tmfqs_hip_state *h, const std::vector<QOp> &ops, size_t lo, size_t hi) {
 	std::vector<EwStep> steps;
-	const unsigned block = h->cfg.grover_block > 0 ? (unsigned)h->cfg.grover_block : (h->prec == 0 ? 96u : 48u);
+	const unsigned block = ew_block(h);
 	auto run_steps = [&]() {
+		if(h->prec == 1 && !steps.empty()) {
+			// complex64: a -> 2m - a and a -> A a + B are affine, so the run
+			// collapses (in FP64) to a -> s a + b and stored amplitudes are
+			// rounded once per sweep instead of once per step
+			double sr = 1, si = 0, br = 0, bi = 0;
+			for(const EwStep &st : steps) {
+				if(st.kind == EW_DIFFUSE) {
+					sr = -sr;
+					si = -si;
+					br = st.ar - br;
+					bi = st.ai - bi;
+				} else {
+					const double nsr = st.ar * sr - st.ai * si, nsi = st.ar * si + st.ai * sr;
+					const double nbr = st.ar * br - st.ai * bi + st.br, nbi = st.ar * bi + st.ai * br + st.bi;
+					sr = nsr;
+					si = nsi;
+					br = nbr;
+					bi = nbi;
+				}
+			}
+			steps.assign(1, EwStep{EW_AFFINE, 0, sr, si, br, bi});
+		}
 		size_t s0 = 0;
 		while(s0 < steps.size()) {
 			const size_t cnt = std::min<size_t>(block, steps.size() - s0);
@@ -523,8 +553,7 @@ void enqueue_ew(tmfqs_hip_state *h, const QOp &op) {
 	h->queue.push_back(op);
 	if(op.kind == Q_DIFF || op.kind == Q_AFFINE) {
 		++h->ew_steps_pending;
-		const size_t block = h->cfg.grover_block > 0 ? (size_t)h->cfg.grover_block : (h->prec == 0 ? 96u : 48u);
-		if(!h->cfg.fusion || h->ew_steps_pending >= block) flush(h);
+		if(!h->cfg.fusion || h->ew_steps_pending >= ew_block(h)) flush(h);
 	}
 }
 

@@ -651,9 +651,15 @@ __device__ __forceinline__ double2 phase_of_range(u64 F, const double2 *__restri
 	return p;
 }
 
+// Rotation by a phase evaluated in FP64; complex64 amplitudes take it rounded
+// to FP32 and rotate in FP32 (the bulk arithmetic of the complex64 form).
 template <typename V> __device__ __forceinline__ V rot(V x, double2 p) {
 	const double xr = x.x, xi = x.y;
 	return mk<V>(fma(xr, p.x, -xi * p.y), fma(xr, p.y, xi * p.x));
+}
+template <> __device__ __forceinline__ float2 rot<float2>(float2 x, double2 p) {
+	const float pr = (float)p.x, pi = (float)p.y;
+	return make_float2(fmaf(x.x, pr, -x.y * pi), fmaf(x.x, pi, x.y * pr));
 }
 
 // ------------------------------------------------------------------------
@@ -681,9 +687,22 @@ template <typename V> __device__ __forceinline__ void hadamard_pair(V &x, V &y, 
 	x = mk<V>((xr + yr) * c, (xi + yi) * c);
 	y = mk<V>((xr - yr) * c, (xi - yi) * c);
 }
+__device__ __forceinline__ void hadamard_pair(float2 &x, float2 &y, double c) {
+	const float cf = (float)c;
+	const float xr = x.x, xi = x.y, yr = y.x, yi = y.y;
+	x = make_float2((xr + yr) * cf, (xi + yi) * cf);
+	y = make_float2((xr - yr) * cf, (xi - yi) * cf);
+}
+
+// Waves per SIMD the QFT block kernel is compiled for: two workgroups per CU
+// wherever LDS allows (complex128 with 16-amplitude rows, complex64 with 32).
+template <typename R, int C> struct QftBlockWaves {
+	static constexpr int value = (sizeof(R) == 8 && C == 4) ? 2 : 1;
+};
 
 template <typename R, int C>
-__global__ void __launch_bounds__(16 << C) k_qft_block(typename V2T<R>::t *__restrict__ a, QftBlockParams P,
+__global__ void __launch_bounds__(16 << C) __attribute__((amdgpu_waves_per_eu(QftBlockWaves<R, C>::value)))
+k_qft_block(typename V2T<R>::t *__restrict__ a, QftBlockParams P,
                                                         const double2 *__restrict__ ptab, const double2 *__restrict__ cin) {
 	using V = typename V2T<R>::t;
 	extern __shared__ char smem[];
@@ -852,6 +871,11 @@ template <typename V> __device__ __forceinline__ V quad_h(V x, unsigned lane, un
 	const double yr = __shfl_xor((double)x.x, (int)mask, 64), yi = __shfl_xor((double)x.y, (int)mask, 64);
 	const double xr = x.x, xi = x.y;
 	return (lane & mask) ? mk<V>((yr - xr) * c, (yi - xi) * c) : mk<V>((xr + yr) * c, (xi + yi) * c);
+}
+template <> __device__ __forceinline__ float2 quad_h<float2>(float2 x, unsigned lane, unsigned mask, double c) {
+	const float cf = (float)c;
+	const float yr = __shfl_xor(x.x, (int)mask, 64), yi = __shfl_xor(x.y, (int)mask, 64);
+	return (lane & mask) ? make_float2((yr - x.x) * cf, (yi - x.y) * cf) : make_float2((x.x + yr) * cf, (x.y + yi) * cf);
 }
 
 // The 64-point tail transform of one row quarter held by a lane: registers
@@ -1441,7 +1465,7 @@ __global__ void __launch_bounds__(256) k_batch_tiles(const BatchPass *__restrict
 	u64 *hitab = (u64 *)(ta + staged_tile_elems(batch_max_tile_bits(sizeof(R) == 8 ? 0 : 1), false));
 	for(unsigned it = blockIdx.x; it < nitems; it += gridDim.x) {
 		const BatchItem item = items[it];
-		const BatchPass bp = passes[item.pass];
+		const BatchPass &bp = passes[item.pass];  // read in place: its tables are indexed at run time
 		__syncthreads();  // the previous item's tiles are done with hitab
 		tile_hitab<R>(hitab, bp.tp);
 		for(u64 tid = item.tile0; tid < item.tile0 + item.ntiles; ++tid)
