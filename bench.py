@@ -13,6 +13,8 @@ Also measured in the same run and reported beside the headline:
     probability and the Mt19937(123456) measurement;
   * complex64: QFT-30 time and fidelity against the complex128 result
     (device comparison);
+  * dense five-qubit blocks (8 random 32 x 32 unitaries) on the 30-qubit
+    state, complex64 on FP32 MFMA against complex128;
   * batched circuits: the committed seed-42 workload (256 circuits of 10-22
     qubits, 50-200 gates) as one autotuned batch graph launch, against the same
     circuits run one at a time;
@@ -221,6 +223,50 @@ def time_qft32(T, S, n=32, seed=11):
         "max_abs_error": cmp["max_abs_amplitude_error"],
         "state_GiB_complex64": (8 << n) / 2 ** 30,
     }
+
+
+def time_blocks(T, n=30, gates=8, seed=5):
+    """Dense five-qubit blocks (random 32 x 32 unitaries on random qubits) on a
+    2^n state: complex64 runs them on FP32 MFMA inside the tiled pass,
+    complex128 on FP64 VALU with the dense backend's accumulation order."""
+    rng = np.random.default_rng(seed)
+    mats, qubits = [], []
+    for _ in range(gates):
+        z = rng.standard_normal((32, 32)) + 1j * rng.standard_normal((32, 32))
+        q, r = np.linalg.qr(z)
+        mats.append(q * (np.diag(r) / np.abs(np.diag(r))))
+        qubits.append([int(x) for x in rng.choice(n, 5, replace=False)])
+    out, states = {}, []
+    for prec in (T.C64, T.C128):
+        s = T.HipState(precision=prec)
+        s.init_basis(n, 0)
+        for q in range(n):
+            s.h(q)
+        s.synchronize()
+        s.profile(True)
+        s.profile_reset()
+        t0 = time.perf_counter()
+        for u, qs in zip(mats, qubits):
+            s.gate(u, qs)
+        s.synchronize()
+        secs = time.perf_counter() - t0
+        prof = s.profile_read()
+        s.profile(False)
+        flops = gates * 8.0 * 32 * (1 << n)  # 8 real flops per complex MAC, 32 MACs per amplitude
+        out["complex64" if prec == T.C64 else "complex128"] = {
+            "seconds": secs,
+            "gate_passes": sum(v["launches"] for v in prof.values()),
+            "TFLOPs_dense_blocks": flops / secs / 1e12,
+            "kernels": prof,
+        }
+        states.append(s)
+    cmp = states[1].compare(states[0])
+    out.update({"qubits": n, "gates": gates, "fidelity_complex64_vs_complex128": cmp["fidelity"],
+                "max_abs_error": cmp["max_abs_amplitude_error"],
+                "speedup_complex64": out["complex128"]["seconds"] / out["complex64"]["seconds"]})
+    for st in states:
+        st.close()
+    return out
 
 
 def time_batch(T, precision, steps=3):
@@ -456,6 +502,14 @@ def main():
 
     if args.qft32:
         result["qft32_complex64"] = time_qft32(T, S)
+
+    # ---- dense five-qubit blocks (FP32 MFMA in complex64)
+    if not args.skip_c64:
+        log(f"five-qubit blocks on {n} qubits: complex64 (MFMA) and complex128")
+        blk = time_blocks(T, n)
+        log(f"blocks: complex64 {blk['complex64']['seconds'] * 1e3:.1f} ms, complex128 "
+            f"{blk['complex128']['seconds'] * 1e3:.1f} ms, fidelity {blk['fidelity_complex64_vs_complex128']:.12f}")
+        result["five_qubit_blocks"] = blk
 
     # ---- adjoint gradients: n = 24, 8 ring layers, 192 angles
     if not args.skip_adjoint:

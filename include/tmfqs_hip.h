@@ -294,6 +294,13 @@ int tmfqs_hip_batch_run(tmfqs_hip_batch *b, double *seconds);
 /* Interleaved FP64 amplitudes (2 * 2^qubits doubles) of circuit `index`. */
 int tmfqs_hip_batch_export(tmfqs_hip_batch *b, size_t index, double *out);
 int tmfqs_hip_batch_info(tmfqs_hip_batch *b, tmfqs_hip_batch_stats *out);
+/* Adjoint gradients of one diagonal observable (see tmfqs_hip_adjoint) for
+ * every circuit of the batch, each from its basis state: expvals[c] per
+ * circuit; grads holds the CPHASE/PHASE gradients of circuit 0 in record
+ * order, then those of circuit 1, and so on (grad_count in total).  Circuits
+ * run one after another on one device handle. */
+int tmfqs_hip_batch_adjoint(tmfqs_hip_batch *b, const tmfqs_hip_diag_term *terms, size_t nterms, double *expvals, double *grads,
+                            size_t grad_count);
 int tmfqs_hip_batch_destroy(tmfqs_hip_batch *b);
 
 #ifdef __cplusplus

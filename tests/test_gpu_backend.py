@@ -277,6 +277,25 @@ def test_complex64_fidelity():
     assert 1 - fid <= 1e-6
 
 
+@pytest.mark.parametrize("n", [9, 12, 14])
+def test_complex64_five_qubit_blocks(n):
+    # complex64 k = 5 matrices: FP32 MFMA on tiles of >= 10 bits (n = 12, 14),
+    # FP32 VALU below; against the complex128 oracle
+    rng = np.random.default_rng(n)
+    init = rand_state(n, n)
+    s = st(T.C64).load(n, init)
+    want = init.copy()
+    for _ in range(6):
+        u = R.random_unitary(rng, 32)
+        qubits = [int(q) for q in rng.choice(n, size=5, replace=False)]
+        s.gate(u, qubits)
+        want = O.apply_gate(want, n, R.interleave(u), qubits)
+    got = s.amplitudes()
+    w = cplx(want)
+    assert np.abs(got - w).max() < 2e-6
+    assert 1.0 - abs(np.vdot(w, got)) ** 2 / (np.vdot(w, w).real * np.vdot(got, got).real) <= 1e-6
+
+
 def test_qft_large_vs_fft():
     # n = 24: a size-independent property (the DFT) rather than the oracle.
     n = 24

@@ -136,3 +136,18 @@ def test_batch_errors():
     bad[0].q0 = bad[0].q1 = 1
     with pytest.raises(T.InvalidArgument):
         Batch([(6, bad, 1)])
+
+
+def test_batched_adjoint_vs_single_and_oracle():
+    circuits = random_circuits(6, 6, 5, 10)
+    terms = [(T.DIAG_Z, 1.0, 1 << 0), (T.DIAG_PROJECTOR, 1.0, 0)]
+    print("batched adjoint", flush=True)
+    with Batch(circuits) as b:
+        e, g = b.adjoint(terms)
+    for i, (n, ops, cnt) in enumerate(circuits):
+        with T.HipState() as s:
+            s.init_basis(n, 0)
+            e1, g1 = s.adjoint(ops, cnt, terms)
+        assert e[i] == e1 and np.array_equal(g[i], g1)
+        ps = O.param_shift_grad(n, oracle_ops(ops, cnt), cnt, terms)
+        assert np.max(np.abs(g[i] - ps)) <= 1e-10

@@ -12,7 +12,8 @@ import ctypes
 
 import numpy as np
 
-from . import (C128, Config, Op, OP_CPHASE, OP_CX, OP_H, OP_PHASE, OP_SWAP, OP_X, check, default_config, lib)
+from . import (C128, Config, DiagTerm, Op, OP_CPHASE, OP_CX, OP_H, OP_PHASE, OP_SWAP, OP_X, check, default_config, lib,
+               make_terms)
 
 _KINDS = {"H": OP_H, "X": OP_X, "CX": OP_CX, "CP": OP_CPHASE, "PH": OP_PHASE, "SW": OP_SWAP}
 
@@ -86,6 +87,8 @@ def _bind():
         L.tmfqs_hip_batch_export.argtypes = [vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_double)]
         L.tmfqs_hip_batch_info.argtypes = [vp, ctypes.POINTER(Stats)]
         L.tmfqs_hip_batch_destroy.argtypes = [vp]
+        L.tmfqs_hip_batch_adjoint.argtypes = [vp, ctypes.POINTER(DiagTerm), ctypes.c_size_t, ctypes.POINTER(ctypes.c_double),
+                                              ctypes.POINTER(ctypes.c_double), ctypes.c_size_t]
         L._batch_bound = True
     return L
 
@@ -96,6 +99,7 @@ class Batch:
     def __init__(self, circuits, precision=C128, init_states=None, **cfg):
         L = _bind()
         self.n = [c[0] for c in circuits]
+        self.nparams = [sum(1 for i in range(c[2]) if c[1][i].kind in (OP_CPHASE, OP_PHASE)) for c in circuits]
         arr = (Circuit * max(1, len(circuits)))()
         for i, (n, ops, count) in enumerate(circuits):
             arr[i].qubits = n
@@ -137,6 +141,20 @@ class Batch:
         out = np.empty(2 << self.n[i], dtype=np.float64)
         check(_bind().tmfqs_hip_batch_export(self._h, i, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
+
+    def adjoint(self, terms):
+        """(expectation per circuit, list of gradient arrays per circuit)."""
+        arr, cnt = make_terms(terms)
+        e = np.zeros(len(self.n))
+        g = np.zeros(max(1, sum(self.nparams)))
+        dp = ctypes.POINTER(ctypes.c_double)
+        check(_bind().tmfqs_hip_batch_adjoint(self._h, arr, cnt, e.ctypes.data_as(dp), g.ctypes.data_as(dp),
+                                              sum(self.nparams)))
+        out, off = [], 0
+        for k in self.nparams:
+            out.append(g[off:off + k].copy())
+            off += k
+        return e, out
 
     def info(self):
         s = Stats()

@@ -2660,6 +2660,46 @@ int tmfqs_hip_batch_info(tmfqs_hip_batch *b, tmfqs_hip_batch_stats *out) {
 	ABI_END
 }
 
+int tmfqs_hip_batch_adjoint(tmfqs_hip_batch *b, const tmfqs_hip_diag_term *terms, size_t nterms, double *expvals, double *grads,
+                            size_t grad_count) {
+	tmfqs_hip_state *h = nullptr;
+	ABI_BEGIN
+	if(!b) throw std::invalid_argument("null tmfqs_hip_batch handle");
+	if(!expvals) throw std::invalid_argument("tmfqs_hip_batch_adjoint: null expectation output");
+	size_t total = 0;
+	std::vector<std::vector<size_t>> params(b->n.size());
+	for(size_t c = 0; c < b->n.size(); ++c) {
+		for(size_t i = 0; i < b->ops[c].size(); ++i)
+			if(b->ops[c][i].kind == Q_CP || b->ops[c][i].kind == Q_PH) params[c].push_back(i);
+		total += params[c].size();
+	}
+	if(total != grad_count)
+		throw std::invalid_argument("tmfqs_hip_batch_adjoint: grad_count must equal the batch's CPHASE and PHASE operations (" +
+		                            std::to_string(total) + ")");
+	if(grad_count && !grads) throw std::invalid_argument("tmfqs_hip_batch_adjoint: null gradient output");
+	HIPC(hipSetDevice(b->cfg.device));
+	HIPC(hipStreamSynchronize(b->stream));
+	int rc = tmfqs_hip_create(&b->cfg, &h);
+	if(rc != TMFQS_HIP_OK) throw_status(rc);
+	size_t off = 0;
+	for(size_t c = 0; c < b->n.size(); ++c) {
+		rc = tmfqs_hip_init_basis(h, b->n[c], b->init[c], 1.0, 0.0);
+		if(rc != TMFQS_HIP_OK) throw_status(rc);
+		const ObsParams O = make_observable(h, terms, nterms);
+		std::vector<QOp> q = b->ops[c];
+		adjoint_run(h, q, params[c], O, 0, &expvals[c], grads + off);
+		off += params[c].size();
+	}
+	destroy_handle(h);
+	h = nullptr;
+	return TMFQS_HIP_OK;
+	}
+	catch(...) {
+		destroy_handle(h);
+		return map_exception();
+	}
+}
+
 int tmfqs_hip_batch_destroy(tmfqs_hip_batch *b) {
 	ABI_BEGIN
 	destroy_batch(b);

@@ -1307,6 +1307,82 @@ __device__ __forceinline__ void reg_diag(V *v, const TileOp &op, const unsigned 
 	}
 }
 
+// ------------------------------------------------------------------------
+// complex64 five-qubit matrix on an LDS tile with FP32 MFMA.  The tile's
+// 2^(T-5) blocks of 32 amplitudes (the matrix's local bits jl, jl[0] its
+// index MSB) are the columns of X, and Y = [[Mr, -Mi], [Mi, Mr]] [Xr; Xi]
+// is a real 64 x 64 by 64 x 2^(T-5) product in 32 x 32 x 2 MFMA steps with
+// FP32 accumulation.  A wavefront owns one half of the output rows (real or
+// imaginary parts) and up to four 32-block column tiles; the matrix half it
+// multiplies by stays in its registers.  Needs T >= 10 and four wavefronts.
+typedef float mfma_f16v __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ unsigned block_offset(unsigned c, const unsigned *jl) {
+	unsigned off = 0;
+#pragma unroll
+	for(int i = 0; i < 5; ++i)
+		if((c >> (4 - i)) & 1u) off |= 1u << jl[i];
+	return off;
+}
+
+__device__ void tile_mfma5(float2 *tile, const TileOp &op, unsigned T, const double *__restrict__ mats) {
+	const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+	const unsigned half = wave & 1u;  // output rows 32 half .. 32 half + 31
+	unsigned jl[5], sorted[5];
+#pragma unroll
+	for(int i = 0; i < 5; ++i) jl[i] = sorted[i] = op.jl[i];
+	for(int i = 0; i < 5; ++i)
+		for(int j = i + 1; j < 5; ++j)
+			if(sorted[j] < sorted[i]) {
+				const unsigned t = sorted[i];
+				sorted[i] = sorted[j];
+				sorted[j] = t;
+			}
+	// this lane's A operands: Mbig[32 half + lane % 32][2 s + lane / 32], s = 0..31
+	const double *g = mats + op.mat;
+	const unsigned rr = lane & 31u, kh = lane >> 5;
+	float A[32];
+#pragma unroll
+	for(int s2 = 0; s2 < 32; ++s2) {
+		const unsigned k = 2u * (unsigned)s2 + kh, kk = k & 31u;
+		const bool right = k >= 32u;
+		const double re = g[2 * (rr * 32 + kk)], im = g[2 * (rr * 32 + kk) + 1];
+		A[s2] = (float)(half ? (right ? re : im) : (right ? -im : re));
+	}
+	const unsigned ncol = 1u << (T - 10);  // column tiles of 32 blocks
+	mfma_f16v acc[4];
+#pragma unroll
+	for(int u = 0; u < 4; ++u) {
+		const unsigned ct = (wave >> 1) + 2u * (unsigned)u;
+		acc[u] = mfma_f16v{0};
+		if(ct >= ncol) continue;  // uniform per wavefront
+		unsigned base = ct * 32u + rr;  // block index of this lane's B column
+#pragma unroll
+		for(int i = 0; i < 5; ++i) base = ((base >> sorted[i]) << (sorted[i] + 1)) | (base & ((1u << sorted[i]) - 1u));
+#pragma unroll
+		for(int s2 = 0; s2 < 32; ++s2) {
+			const unsigned k = 2u * (unsigned)s2 + kh;
+			const float2 x = tile[base | block_offset(k & 31u, jl)];
+			acc[u] = __builtin_amdgcn_mfma_f32_32x32x2f32(A[s2], k >= 32u ? x.y : x.x, acc[u], 0, 0, 0);
+		}
+	}
+	__syncthreads();  // every wavefront has read its columns
+#pragma unroll
+	for(int u = 0; u < 4; ++u) {
+		const unsigned ct = (wave >> 1) + 2u * (unsigned)u;
+		if(ct >= ncol) continue;
+		unsigned base = ct * 32u + rr;  // D column = block ct * 32 + lane % 32
+#pragma unroll
+		for(int i = 0; i < 5; ++i) base = ((base >> sorted[i]) << (sorted[i] + 1)) | (base & ((1u << sorted[i]) - 1u));
+#pragma unroll
+		for(int v = 0; v < 16; ++v) {
+			const unsigned r = 8u * ((unsigned)v >> 2) + 4u * kh + ((unsigned)v & 3u);  // D row (32x32x2 layout)
+			float *dst = (float *)&tile[base | block_offset(r, jl)];
+			dst[half] = acc[u][v];
+		}
+	}
+}
+
 // One tile of a staged pass: loads it into LDS, runs every stage, stores it.
 // ta/tb/hitab/gacc are the workgroup's LDS buffers; hitab must hold P's table.
 template <typename R, int W, bool DUAL, bool LDSOPS>
@@ -1342,6 +1418,17 @@ __device__ __forceinline__ void staged_tile(typename V2T<R>::t *__restrict__ a, 
 			for(unsigned o = 0; o < st.nops; ++o) {
 				const TileOp op = ops[st.op0 + o];
 				if((tbase & op.gcond) != op.gcond) continue;  // uniform across the block
+				if constexpr(sizeof(R) == 4) {
+					if(op.kind == TOP_UK && op.k == 5 && P.T >= 10 && blockDim.x == 256) {
+						tile_mfma5((float2 *)ta, op, P.T, mats);
+						if(DUAL) {
+							__syncthreads();
+							tile_mfma5((float2 *)tb, op, P.T, mats);
+						}
+						__syncthreads();
+						continue;
+					}
+				}
 				tile_apply<R>(ta, op, nloc, tbase, C, hitab, mats, terms, ptab);
 				if(DUAL) tile_apply<R>(tb, op, nloc, tbase, C, hitab, mats, terms, ptab);
 				__syncthreads();
