@@ -861,54 +861,56 @@ struct QftTailParams {
 };
 
 __device__ __forceinline__ unsigned rev6(unsigned x) { return __brev(x) >> 26; }
+// LDS slot of (row, col): the XOR swizzle spreads both the row-wise writes
+// (8 lanes per row, consecutive rows) and the transposed reads (rows rev6(c)
+// for consecutive c) over the banks.
 __device__ __forceinline__ unsigned tail_slot(unsigned row, unsigned col) {
-	return row * 64u + (col ^ ((((row & 3u) << 2) ^ (row >> 2)) & 15u));
+	return row * 64u + (col ^ ((((row & 1u) << 3) ^ (row >> 2)) & 15u));
 }
 
-// Butterfly of target bit `mask` of the lane index: lanes with the bit clear
-// keep (x + y) c, their partners (y - x) c.
-template <typename V> __device__ __forceinline__ V quad_h(V x, unsigned lane, unsigned mask, double c) {
+// Butterfly of the target bit `mask` of the lane index: lanes with the bit
+// clear keep (x + y) c, their partners (y - x) c.
+template <typename V> __device__ __forceinline__ V lane_h(V x, unsigned lane, unsigned mask, double c) {
 	const double yr = __shfl_xor((double)x.x, (int)mask, 64), yi = __shfl_xor((double)x.y, (int)mask, 64);
 	const double xr = x.x, xi = x.y;
 	return (lane & mask) ? mk<V>((yr - xr) * c, (yi - xi) * c) : mk<V>((xr + yr) * c, (xi + yi) * c);
 }
-template <> __device__ __forceinline__ float2 quad_h<float2>(float2 x, unsigned lane, unsigned mask, double c) {
+template <> __device__ __forceinline__ float2 lane_h<float2>(float2 x, unsigned lane, unsigned mask, double c) {
 	const float cf = (float)c;
 	const float yr = __shfl_xor(x.x, (int)mask, 64), yi = __shfl_xor(x.y, (int)mask, 64);
 	return (lane & mask) ? make_float2((yr - x.x) * cf, (yi - x.y) * cf) : make_float2((x.x + yr) * cf, (x.y + yi) * cf);
 }
 
-// The 64-point tail transform of one row quarter held by a lane: registers
-// are column bits 5..2, the lane's quad position is column bits 1..0.
-// The 64-point tail transform of one row quarter held by a lane: registers
-// are column bits 5..2, the lane's quad position q is column bits 1..0.  The
-// ladder factor e^{i pi low / 2^t} of target t splits into a constant of the
-// register index (uniform: scalar loads) and the lane's qf[t - 2] =
-// e^{i pi q / 2^t}.
+// The 64-point tail transform of one row held by eight lanes: registers are
+// column bits 5..3, the lane's position q in its group of eight is column
+// bits 2..0.  The ladder factor e^{i pi low / 2^t} of targets 5..3 splits into
+// a constant of the register index (uniform: scalar loads) and the lane's
+// qf[t - 3] = e^{i pi q / 2^t}; targets 2..0 are lane butterflies.
 template <typename V>
 __device__ __forceinline__ void tail_transform(V *v, unsigned lane, unsigned q, const double2 *qf, const double2 *__restrict__ cq,
                                                double hc) {
 #pragma unroll
-	for(int tt = 5; tt >= 2; --tt) {
-		const int b = tt - 2;
+	for(int tt = 5; tt >= 3; --tt) {
+		const int b = tt - 3;
 #pragma unroll
-		for(int ra = 0; ra < 16; ++ra)
+		for(int ra = 0; ra < 8; ++ra)
 			if(!(ra & (1 << b))) hadamard_pair(v[ra], v[ra | (1 << b)], hc);
 #pragma unroll
-		for(int ra = 0; ra < 16; ++ra) {
+		for(int ra = 0; ra < 8; ++ra) {
 			if(!(ra & (1 << b))) continue;
 			const int hi = ra & ((1 << b) - 1);  // register bits below the target
-			const double2 ph = hi ? cmul(cq[(1 << tt) - 1 + (hi << 2)], qf[b]) : qf[b];
+			const double2 ph = hi ? cmul(cq[(1 << tt) - 1 + (hi << 3)], qf[b]) : qf[b];
 			if(hi || q) v[ra] = rot<V>(v[ra], ph);
 		}
 	}
-	// target 1 (lane bit 1) with its ladder e^{i pi / 2} from bit 0, then target 0
-	const double2 ci = cq[2];
+	const double2 p2 = cq[3 + (q & 3u)], p1 = cq[2];  // e^{i pi (q mod 4) / 4}, e^{i pi / 2}
 #pragma unroll
-	for(int ra = 0; ra < 16; ++ra) {
-		V x = quad_h(v[ra], lane, 2u, hc);
-		if(q == 3u) x = rot<V>(x, ci);
-		v[ra] = quad_h(x, lane, 1u, hc);
+	for(int ra = 0; ra < 8; ++ra) {
+		V x = lane_h(v[ra], lane, 4u, hc);  // target 2
+		if((q & 4u) && (q & 3u)) x = rot<V>(x, p2);
+		x = lane_h(x, lane, 2u, hc);  // target 1
+		if((q & 3u) == 3u) x = rot<V>(x, p1);
+		v[ra] = lane_h(x, lane, 1u, hc);  // target 0
 	}
 }
 
@@ -921,9 +923,29 @@ __device__ __forceinline__ void tail_store(const V *tile, V *__restrict__ a, uns
 	}
 }
 
-// Workgroups of 256 lanes: tile M is transformed and staged in LDS while tile
-// rev(M) is read and transformed in registers, so one 64 KiB buffer serves
-// the pair and two workgroups share a CU (one loads while the other stores).
+template <typename V>
+__device__ __forceinline__ void tail_stage(V *tile, const V *v, unsigned rp, unsigned q) {
+#pragma unroll
+	for(int s2 = 0; s2 < 2; ++s2)
+#pragma unroll
+		for(int ra = 0; ra < 8; ++ra) tile[tail_slot(rp + 32u * (unsigned)s2, ((unsigned)ra << 3) | q)] = v[s2 * 8 + ra];
+}
+
+template <typename V>
+__device__ __forceinline__ void tail_load(V *v, const V *__restrict__ a, unsigned rp, unsigned q, unsigned sh, u64 mm) {
+#pragma unroll
+	for(int s2 = 0; s2 < 2; ++s2) {
+		const u64 src = ((u64)(rp + 32u * (unsigned)s2) << sh) | (mm << 6) | q;
+#pragma unroll
+		for(int ra = 0; ra < 8; ++ra) v[s2 * 8 + ra] = nt_load(&a[src | ((unsigned)ra << 3)]);
+	}
+}
+
+// Workgroups of 256 lanes; a lane holds 8 columns of two rows (rp and
+// rp + 32), eight lanes share a row, so every load instruction reads whole
+// 128 B lines.  Tile M is transformed and staged in LDS while tile rev(M) is
+// read and transformed in registers: one 64 KiB buffer per workgroup, two
+// workgroups per CU.
 template <typename R>
 __global__ void __launch_bounds__(256) k_qft_tail_reverse(typename V2T<R>::t *__restrict__ a, QftTailParams P,
                                                           const double2 *__restrict__ cq) {
@@ -931,36 +953,31 @@ __global__ void __launch_bounds__(256) k_qft_tail_reverse(typename V2T<R>::t *__
 	extern __shared__ char smem[];
 	V *tile = (V *)smem;  // 64 x 64
 	const unsigned t = threadIdx.x;
-	const unsigned lane = t & 63u, q = lane & 3u;
-	const unsigned row = (t >> 6) * 16u + (lane >> 2);
+	const unsigned lane = t & 63u, q = t & 7u, rp = t >> 3;
 	const double hc = 1.0 / sqrt(2.0);
 	const unsigned sh = P.n - 6;
-	double2 qf[4];
+	double2 qf[3];
 #pragma unroll
-	for(int b = 0; b < 4; ++b) qf[b] = cq[(1u << (b + 2)) - 1u + q];
+	for(int b = 0; b < 3; ++b) qf[b] = cq[(1u << (b + 3)) - 1u + q];
 	for(u64 M = blockIdx.x; M < P.nmid; M += gridDim.x) {
 		const u64 Mr = P.m ? (__brevll(M) >> (64 - P.m)) : 0;
 		if(Mr < M) continue;  // uniform: the pair is handled at M' = Mr
 		V v[16];
-		const u64 sa = (((u64)row) << sh) | (M << 6) | q;
-#pragma unroll
-		for(int ra = 0; ra < 16; ++ra) v[ra] = nt_load(&a[sa | ((unsigned)ra << 2)]);
+		tail_load(v, a, rp, q, sh, M);
 		tail_transform(v, lane, q, qf, cq, hc);
+		tail_transform(v + 8, lane, q, qf, cq, hc);
 		__syncthreads();  // the previous pair's transposed reads are done
-#pragma unroll
-		for(int ra = 0; ra < 16; ++ra) tile[tail_slot(row, ((unsigned)ra << 2) | q)] = v[ra];
+		tail_stage(tile, v, rp, q);
 		if(Mr != M) {
-			const u64 sb = (((u64)row) << sh) | (Mr << 6) | q;
-#pragma unroll
-			for(int ra = 0; ra < 16; ++ra) v[ra] = nt_load(&a[sb | ((unsigned)ra << 2)]);
+			tail_load(v, a, rp, q, sh, Mr);
 			tail_transform(v, lane, q, qf, cq, hc);
+			tail_transform(v + 8, lane, q, qf, cq, hc);
 		}
 		__syncthreads();
 		tail_store(tile, a, sh, Mr);  // tile M into the place of tile rev(M), already read
 		if(Mr != M) {
 			__syncthreads();
-#pragma unroll
-			for(int ra = 0; ra < 16; ++ra) tile[tail_slot(row, ((unsigned)ra << 2) | q)] = v[ra];
+			tail_stage(tile, v, rp, q);
 			__syncthreads();
 			tail_store(tile, a, sh, M);
 		}
