@@ -104,6 +104,25 @@ def test_qft(mode, n):
     assert np.abs(cplx(got) - dft).max() <= TOL
 
 
+@pytest.mark.parametrize("precision", [T.C128, T.C64], ids=["c128", "c64"])
+def test_qft_tail_pair_shapes(precision):
+    # The fused tail exchanges 64 x 64 tile pairs {M, rev(M)} over the n - 12
+    # middle bits: n = 12, 13 have one or two palindromic tiles, odd n - 12 has
+    # a middle bit, and every n here also runs the eight-target block passes.
+    for n in (12, 13, 14, 15, 17, 19, 20, 21):
+        init = rand_state(n, 3)
+        s = st(precision).load(n, init)
+        ops, cnt = O.qft_ops(n)
+        Register(s, n).run(ops, cnt)
+        got = s.amplitudes()
+        want = np.fft.ifft(cplx(init), norm="ortho")
+        if precision == T.C128:
+            assert np.abs(got - want).max() <= TOL, n
+        else:
+            assert np.abs(got - want).max() < 1e-5, n
+            assert 1 - abs(np.vdot(want, got)) ** 2 <= 1e-6, n
+
+
 def test_qft_known_answers():
     # test_circuit.cpp:76-106
     for src in (0, 5):

@@ -667,32 +667,34 @@ template <> __device__ __forceinline__ float2 rot<float2>(float2 x, double2 p) {
 // as makeQft orders its target qubits); for each target t it applies H on t
 // and then every controlled phase pi/2^(t-c) from the lower bits c < t, i.e.
 // the factor e^{i pi (x mod 2^t) / 2^t} on states with bit t set.  Splitting
-// x mod 2^t into the part inside the block and the part below it, the inner
+// x mod 2^t into the part inside the block and the part r below it, the inner
 // factors are constants c[j][v] = e^{i pi v / 2^j} and the outer ones commute
 // with the remaining H gates of the block, so they are applied last as one
-// twiddle e^{2 pi i r W(y) / 2^64} per state (r = bits below the block,
-// W(y) = sum over set block bits j of 2^(63 - pbot - j)).
+// twiddle per state: block bit j set contributes e^{i pi r / 2^(pb+j)}, so a
+// state with block bits y takes w^(r rev8(y)), w = e^{2 pi i / 2^(pb+8)}.
 //
 // A workgroup owns 2^(8+C) states: the 8 block bits times the C contiguous
 // low bits (coalesced 2^C-state runs).  Each thread holds 16 states in
 // registers: first the 16 values of block bits 7..4 (stage A), then, after an
-// LDS transpose, the 16 values of block bits 3..0 (stage B).
+// LDS transpose, the 16 values of block bits 3..0 (stage B).  Butterflies are
+// unscaled; the eight Hadamard factors 1/sqrt(2) are one exact scale by 1/16,
+// folded into the twiddles.  The stage-A loads of the workgroup's next tile
+// are issued as soon as the current tile is in LDS, so they are in flight
+// during stage B and the stores.
 struct QftBlockParams {
 	u64 ntiles;  // 2^(n - 8 - C)
 	uint32_t n, pbot;
 };
 
-template <typename V> __device__ __forceinline__ void hadamard_pair(V &x, V &y, double c) {
-	const double xr = x.x, xi = x.y, yr = y.x, yi = y.y;
-	x = mk<V>((xr + yr) * c, (xi + yi) * c);
-	y = mk<V>((xr - yr) * c, (xi - yi) * c);
+// Unscaled butterfly (x + y, x - y): transforms that apply several Hadamards
+// in a row scale once at the end, by an exact power of two.
+template <typename V> __device__ __forceinline__ void butterfly(V &x, V &y) {
+	const V s = x;
+	x = mk<V>(s.x + y.x, s.y + y.y);
+	y = mk<V>(s.x - y.x, s.y - y.y);
 }
-__device__ __forceinline__ void hadamard_pair(float2 &x, float2 &y, double c) {
-	const float cf = (float)c;
-	const float xr = x.x, xi = x.y, yr = y.x, yi = y.y;
-	x = make_float2((xr + yr) * cf, (xi + yi) * cf);
-	y = make_float2((xr - yr) * cf, (xi - yi) * cf);
-}
+
+__device__ __forceinline__ double2 csqr(double2 p) { return make_double2(fma(p.x, p.x, -p.y * p.y), 2.0 * p.x * p.y); }
 
 // Waves per SIMD the QFT block kernel is compiled for: two workgroups per CU
 // wherever LDS allows (complex128 with 16-amplitude rows, complex64 with 32).
@@ -700,45 +702,47 @@ template <typename R, int C> struct QftBlockWaves {
 	static constexpr int value = (sizeof(R) == 8 && C == 4) ? 2 : 1;
 };
 
+// Stage-A loads of tile `tid`: register ra holds block bits 7..4 = ra.
+template <typename V, int C>
+__device__ __forceinline__ void qft_block_load(V *v, const V *__restrict__ a, u64 tid, unsigned pb, unsigned f4, unsigned lo) {
+	const u64 x = tid << C;
+	const V *src = a + ((((x >> pb) << (pb + 8)) | (x & ((1ull << pb) - 1ull))) + ((u64)f4 << pb) + lo);
+#pragma unroll
+	for(int ra = 0; ra < 16; ++ra) v[ra] = nt_load(src + ((u64)ra << (pb + 4)));
+}
+
 template <typename R, int C>
 __global__ void __launch_bounds__(16 << C) __attribute__((amdgpu_waves_per_eu(QftBlockWaves<R, C>::value)))
-k_qft_block(typename V2T<R>::t *__restrict__ a, QftBlockParams P,
-                                                        const double2 *__restrict__ ptab, const double2 *__restrict__ cin) {
+k_qft_block(typename V2T<R>::t *__restrict__ a, QftBlockParams P, const double2 *__restrict__ ptab,
+            const double2 *__restrict__ cin) {
 	using V = typename V2T<R>::t;
 	extern __shared__ char smem[];
 	V *tile = (V *)smem;
 	const unsigned t = threadIdx.x;
 	const unsigned lo = t & ((1u << C) - 1u);
 	const unsigned f4 = t >> C;  // stage A: block bits 3..0; stage B: block bits 7..4
-	const double hc = 1.0 / sqrt(2.0);
 	const unsigned pb = P.pbot;
 	const u64 lowmask = (1ull << pb) - 1ull;
-	// Per-thread constants, fixed for the whole launch: the stage-A thread
-	// phases e^{i pi f4 / 2^j} (j = 4..7) and the lane part e^{i pi lo / 2^(pb+j)}
-	// (j = 0..7) of the twiddle.  The in-block ladder constants
-	// c[j][v] = cin[2^j - 1 + v] are read where they are used (uniform
-	// addresses: scalar loads).
-	double2 tpa[4], plo[8];
+	// Per-thread constants: the stage-A thread phases e^{i pi f4 / 2^j}
+	// (j = 4..7) and the lane's part w^lo of the twiddle base.  The in-block
+	// ladder constants c[j][v] = cin[2^j - 1 + v] are read where they are used
+	// (uniform addresses: scalar loads).
+	double2 tpa[4];
 #pragma unroll
 	for(int j = 4; j < 8; ++j) tpa[j - 4] = ptab[(f4 << (7 - j)) & 255u];
-#pragma unroll
-	for(int j = 0; j < 8; ++j) plo[j] = phase_of_range((u64)lo << (63 - pb - j), ptab, 0, 4);
-	for(u64 tid = blockIdx.x; tid < P.ntiles; tid += gridDim.x) {
-		const u64 x = tid << C;
-		const u64 base = ((x >> pb) << (pb + 8)) | (x & lowmask);
-		V v[16];
+	const double2 zlo = phase_of_range((u64)lo << (56 - pb), ptab, 0, 4);
+	u64 tid = blockIdx.x;
+	if(tid >= P.ntiles) return;  // uniform
+	V v[16];
+	qft_block_load<V, C>(v, a, tid, pb, f4, lo);
+	for(;;) {
 		// ---- stage A: registers index block bits 7..4
-#pragma unroll
-		for(int ra = 0; ra < 16; ++ra) {
-			const u64 y = ((u64)ra << 4) | f4;
-			v[ra] = nt_load(&a[base | (y << pb) | lo]);
-		}
 #pragma unroll
 		for(int j = 7; j >= 4; --j) {
 			const int b = j - 4;
 #pragma unroll
 			for(int ra = 0; ra < 16; ++ra)
-				if(!(ra & (1 << b))) hadamard_pair(v[ra], v[ra | (1 << b)], hc);
+				if(!(ra & (1 << b))) butterfly(v[ra], v[ra | (1 << b)]);
 #pragma unroll
 			for(int ra = 0; ra < 16; ++ra) {
 				if(!(ra & (1 << b))) continue;
@@ -753,53 +757,56 @@ k_qft_block(typename V2T<R>::t *__restrict__ a, QftBlockParams P,
 			tile[l + (l >> (4 + C))] = v[ra];
 		}
 		__syncthreads();
+		const u64 nxt = tid + gridDim.x;
+		const bool more = nxt < P.ntiles;  // uniform
+		if(more) qft_block_load<V, C>(v, a, nxt, pb, f4, lo);
 		// ---- stage B: registers index block bits 3..0
+		V w[16];
 #pragma unroll
 		for(int rb = 0; rb < 16; ++rb) {
 			const unsigned l = (((f4 << 4) | (unsigned)rb) << C) | lo;
-			v[rb] = tile[l + (l >> (4 + C))];
+			w[rb] = tile[l + (l >> (4 + C))];
 		}
 #pragma unroll
 		for(int j = 3; j >= 0; --j) {
 #pragma unroll
 			for(int rb = 0; rb < 16; ++rb)
-				if(!(rb & (1 << j))) hadamard_pair(v[rb], v[rb | (1 << j)], hc);
+				if(!(rb & (1 << j))) butterfly(w[rb], w[rb | (1 << j)]);
 			if(j == 0) continue;  // c[0][0] = 1
 #pragma unroll
 			for(int rb = 0; rb < 16; ++rb) {
 				if(!(rb & (1 << j))) continue;
 				const int low = rb & ((1 << j) - 1);
 				if(low == 0) continue;
-				v[rb] = rot<V>(v[rb], cin[(1 << j) - 1 + low]);
+				w[rb] = rot<V>(w[rb], cin[(1 << j) - 1 + low]);
 			}
 		}
-		// ---- deferred twiddle for the bits below the block: block bit j set
-		// contributes e^{i pi r / 2^(pb+j)}, r = rhi + lo, which factors into a
-		// part uniform across the workgroup (rhi = x mod 2^pb) and the lane's plo[j].
+		// ---- twiddle w^(r rev8(y)) / 16 for y = (f4, rb), r = rhi + lo:
+		// rev8(y) = 16 rev4(rb) + rev4(f4), so register rb takes
+		// base u^rev4(rb) with base = w^(r rev4(f4)) / 16 and u = w^(16 r).
+		const u64 x = tid << C;
 		const u64 rhi = x & lowmask;
-		if((rhi | lo) != 0) {
-			double2 phi = make_double2(1.0, 0.0);
+		double2 z[4];
+		z[0] = cmul(phase_of_range(rhi << (56 - pb), ptab, 0, 4), zlo);  // w^r
 #pragma unroll
-			for(int j = 4; j < 8; ++j)
-				if((f4 >> (j - 4)) & 1u)
-					phi = cmul(phi, cmul(phase_of_range(rhi << (63 - pb - j), ptab, 0, 4), plo[j]));
-			if(f4 != 0) {
+		for(int k = 1; k < 4; ++k) z[k] = csqr(z[k - 1]);
+		const double2 u = csqr(z[3]);
+		double2 ph = make_double2(0.0625, 0.0);
 #pragma unroll
-				for(int rb = 0; rb < 16; ++rb) v[rb] = rot<V>(v[rb], phi);
-			}
+		for(int b = 0; b < 4; ++b)
+			if((f4 >> b) & 1u) ph = cmul(ph, z[3 - b]);
 #pragma unroll
-			for(int j = 0; j < 4; ++j) {
-				const double2 pj = cmul(phase_of_range(rhi << (63 - pb - j), ptab, 0, 4), plo[j]);
-#pragma unroll
-				for(int rb = 0; rb < 16; ++rb)
-					if(rb & (1 << j)) v[rb] = rot<V>(v[rb], pj);
-			}
+		for(int k = 0; k < 16; ++k) {
+			const int rb = (int)(__brev((unsigned)k) >> 28);
+			w[rb] = rot<V>(w[rb], ph);
+			if(k < 15) ph = cmul(ph, u);
 		}
+		const u64 base = ((x >> pb) << (pb + 8)) | rhi;
+		V *dst = a + (base + ((u64)f4 << (pb + 4)) + lo);
 #pragma unroll
-		for(int rb = 0; rb < 16; ++rb) {
-			const u64 y = ((u64)f4 << 4) | (unsigned)rb;
-			nt_store(v[rb], &a[base | (y << pb) | lo]);
-		}
+		for(int rb = 0; rb < 16; ++rb) nt_store(w[rb], dst + ((u64)rb << pb));
+		if(!more) break;
+		tid = nxt;
 	}
 }
 
@@ -845,142 +852,219 @@ unsigned qft_block_coalesce_bits(int prec, int requested) {
 // The last six targets 5..0 only mix the low six bits, so the tail is a
 // 64-point transform of every contiguous run of 64 amplitudes; the reversal
 // then sends index (H, M, Lo) -> (rev6(Lo), rev(M), rev6(H)) for the top six
-// bits H, middle bits M and low six bits Lo.  A workgroup owns the pair of
-// tiles {*, M, *} and {*, rev(M), *} (64 x 64 amplitudes each) and writes each
-// tile's transpose into the other's place, so the pair is exchanged in place.
+// bits H, middle bits M and low six bits Lo.  Tile M = {*, M, *} (64 rows of
+// 64 amplitudes) goes, transposed, into the place of tile rev(M) and tile
+// rev(M) into the place of tile M: the work is a list of tile pairs
+// {M, rev(M)} exchanged in place.
 //
-// One wavefront holds 16 rows of a tile: a quad of lanes shares a row (lane
-// bits 1..0 = column bits 1..0) and each lane holds the 16 columns with those
-// low bits in registers.  Targets 5..2 run in registers, targets 1..0 across
-// the quad with lane shuffles, and the transposed store goes through LDS.  The
-// column swizzle keeps both the row-wise writes and the bit-reversed
-// transposed reads spread over the LDS banks.
+// Pairs are enumerated directly, with no index skipped.  Split the m = n - 12
+// middle bits into a top half x (h = m / 2 bits), a middle bit c when m is
+// odd, and a bottom half y.  With u = x and w = rev_h(y), M = (u, c, rev_h(w))
+// and rev(M) = (w, c, rev_h(u)): the reversal swaps u and w, so the pairs are
+// the entries u <= w of a K x K triangle (K = 2^h) per value of c, u == w
+// being the palindromes.  Rows r and K-1-r of the triangle hold K+1 entries
+// together, which folds it into a (K/2) x (K+1) rectangle.
 struct QftTailParams {
-	u64 nmid;      // 2^(n - 12)
-	uint32_t n, m;  // m = n - 12
+	u64 npairs;
+	uint32_t n, m, h, K;
 };
 
-__device__ __forceinline__ unsigned rev6(unsigned x) { return __brev(x) >> 26; }
-// LDS slot of (row, col): the XOR swizzle spreads both the row-wise writes
-// (8 lanes per row, consecutive rows) and the transposed reads (rows rev6(c)
-// for consecutive c) over the banks.
-__device__ __forceinline__ unsigned tail_slot(unsigned row, unsigned col) {
-	return row * 64u + (col ^ ((((row & 1u) << 3) ^ (row >> 2)) & 15u));
+__device__ __forceinline__ void tail_pair(const QftTailParams &P, u64 pair, u64 &M, u64 &Mr) {
+	unsigned p = (unsigned)pair, c = 0;  // pair < 2^29 for n <= 40
+	if(P.m & 1u) {
+		c = p & 1u;
+		p >>= 1;
+	}
+	unsigned u = 0, w = 0;
+	if(P.K > 1) {
+		const unsigned r = p / (P.K + 1), col = p % (P.K + 1);
+		if(col < P.K - r) {
+			u = r;
+			w = r + col;
+		} else {
+			u = P.K - 1 - r;
+			w = u + (col - (P.K - r));
+		}
+	}
+	const unsigned top = P.m - P.h;
+	const u64 revu = P.h ? (__brev(u) >> (32 - P.h)) : 0u, revw = P.h ? (__brev(w) >> (32 - P.h)) : 0u;
+	M = ((u64)u << top) | ((u64)c << P.h) | revw;
+	Mr = ((u64)w << top) | ((u64)c << P.h) | revu;
 }
 
-// Butterfly of the target bit `mask` of the lane index: lanes with the bit
-// clear keep (x + y) c, their partners (y - x) c.
-template <typename V> __device__ __forceinline__ V lane_h(V x, unsigned lane, unsigned mask, double c) {
-	const double yr = __shfl_xor((double)x.x, (int)mask, 64), yi = __shfl_xor((double)x.y, (int)mask, 64);
-	const double xr = x.x, xi = x.y;
-	return (lane & mask) ? mk<V>((yr - xr) * c, (yi - xi) * c) : mk<V>((xr + yr) * c, (xi + yi) * c);
+__device__ __forceinline__ unsigned rev6(unsigned x) { return __brev(x) >> 26; }
+// LDS slot of (row, col) in a 64 x 64 tile: the column's low three bits are
+// XORed with row bits 5..3, which spreads the transposed reads (rows rev6(c)
+// for eight consecutive c) over the banks; the row-wise writes (eight lanes
+// per row) are spread by the column itself.  Column bits 5..3 are left alone,
+// so a lane's eight slots in a row are constant offsets from one address.
+__device__ __forceinline__ unsigned tail_slot(unsigned row, unsigned col) { return row * 64u + (col ^ ((row >> 3) & 7u)); }
+
+// Value of lane (lane ^ MASK) within each group of eight lanes, by DPP moves:
+// quad_perm [1,0,3,2] and [2,3,0,1] for MASK 1 and 2; for MASK 4,
+// row_half_mirror (lane ^ 7) followed by quad_perm [3,2,1,0] (lane ^ 3).
+template <unsigned MASK> __device__ __forceinline__ int dpp_xor(int x) {
+	if constexpr(MASK == 1) return __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);
+	else if constexpr(MASK == 2) return __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);
+	else {
+		static_assert(MASK == 4, "groups of eight lanes");
+		return __builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(x, 0x141, 0xF, 0xF, false), 0x1B, 0xF, 0xF, false);
+	}
 }
-template <> __device__ __forceinline__ float2 lane_h<float2>(float2 x, unsigned lane, unsigned mask, double c) {
-	const float cf = (float)c;
-	const float yr = __shfl_xor(x.x, (int)mask, 64), yi = __shfl_xor(x.y, (int)mask, 64);
-	return (lane & mask) ? make_float2((yr - x.x) * cf, (yi - x.y) * cf) : make_float2((x.x + yr) * cf, (x.y + yi) * cf);
+template <unsigned MASK> __device__ __forceinline__ double dpp_xor(double x) {
+	const long long b = __double_as_longlong(x);
+	const int lo = dpp_xor<MASK>((int)(unsigned)b), hi = dpp_xor<MASK>((int)(unsigned)(b >> 32));
+	return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+template <unsigned MASK> __device__ __forceinline__ float dpp_xor(float x) {
+	return __int_as_float(dpp_xor<MASK>(__float_as_int(x)));
+}
+
+// Unscaled butterfly over the target bit MASK of the lane index: lanes with
+// the bit clear keep x + y, their partners y - x, computed as y + (+-x), which
+// rounds identically (the sign flip is exact).
+__device__ __forceinline__ double flip_if(double x, unsigned f) {
+	return __longlong_as_double(__double_as_longlong(x) ^ ((long long)f << 63));
+}
+__device__ __forceinline__ float flip_if(float x, unsigned f) { return __int_as_float(__float_as_int(x) ^ (int)(f << 31)); }
+
+template <unsigned MASK, typename V> __device__ __forceinline__ V lane_bfly(V x, unsigned lane) {
+	const unsigned f = (lane & MASK) ? 1u : 0u;
+	return mk<V>(dpp_xor<MASK>(x.x) + flip_if(x.x, f), dpp_xor<MASK>(x.y) + flip_if(x.y, f));
 }
 
 // The 64-point tail transform of one row held by eight lanes: registers are
 // column bits 5..3, the lane's position q in its group of eight is column
 // bits 2..0.  The ladder factor e^{i pi low / 2^t} of targets 5..3 splits into
 // a constant of the register index (uniform: scalar loads) and the lane's
-// qf[t - 3] = e^{i pi q / 2^t}; targets 2..0 are lane butterflies.
+// qf[t - 3] = e^{i pi q / 2^t}; targets 2..0 are lane butterflies, with the
+// lane's ladder factors of targets 2 and 1 in qf[3] and qf[4].  Every lane
+// rotates (by 1 where the ladder does not act): no per-lane selects.  The six
+// Hadamard factors 1/sqrt(2) are applied as one exact scale by 1/8.
 template <typename V>
-__device__ __forceinline__ void tail_transform(V *v, unsigned lane, unsigned q, const double2 *qf, const double2 *__restrict__ cq,
-                                               double hc) {
+__device__ __forceinline__ void tail_transform(V *v, unsigned lane, const double2 *qf, const double2 *__restrict__ cq) {
 #pragma unroll
 	for(int tt = 5; tt >= 3; --tt) {
 		const int b = tt - 3;
 #pragma unroll
 		for(int ra = 0; ra < 8; ++ra)
-			if(!(ra & (1 << b))) hadamard_pair(v[ra], v[ra | (1 << b)], hc);
+			if(!(ra & (1 << b))) butterfly(v[ra], v[ra | (1 << b)]);
 #pragma unroll
 		for(int ra = 0; ra < 8; ++ra) {
 			if(!(ra & (1 << b))) continue;
 			const int hi = ra & ((1 << b) - 1);  // register bits below the target
-			const double2 ph = hi ? cmul(cq[(1 << tt) - 1 + (hi << 3)], qf[b]) : qf[b];
-			if(hi || q) v[ra] = rot<V>(v[ra], ph);
+			v[ra] = rot<V>(v[ra], hi ? cmul(cq[(1 << tt) - 1 + (hi << 3)], qf[b]) : qf[b]);
 		}
 	}
-	const double2 p2 = cq[3 + (q & 3u)], p1 = cq[2];  // e^{i pi (q mod 4) / 4}, e^{i pi / 2}
 #pragma unroll
 	for(int ra = 0; ra < 8; ++ra) {
-		V x = lane_h(v[ra], lane, 4u, hc);  // target 2
-		if((q & 4u) && (q & 3u)) x = rot<V>(x, p2);
-		x = lane_h(x, lane, 2u, hc);  // target 1
-		if((q & 3u) == 3u) x = rot<V>(x, p1);
-		v[ra] = lane_h(x, lane, 1u, hc);  // target 0
+		V x = lane_bfly<4>(v[ra], lane);  // target 2
+		x = rot<V>(x, qf[3]);
+		x = lane_bfly<2>(x, lane);  // target 1
+		x = rot<V>(x, qf[4]);
+		x = lane_bfly<1>(x, lane);  // target 0
+		v[ra] = mk<V>(x.x * 0.125f, x.y * 0.125f);
 	}
 }
 
-// Transposed, bit-reversed store of the tile staged in LDS into tile `dst`.
+// Transposed, bit-reversed store of the tile staged in LDS into tile `dst`
+// (workgroups of 256 lanes, 16 amplitudes per lane).  Lane t writes column
+// c = t mod 64 of output rows 4j + t/64; the output row r comes from LDS
+// column rev6(r) of row rev6(c).
 template <typename V>
 __device__ __forceinline__ void tail_store(const V *tile, V *__restrict__ a, unsigned sh, u64 dst) {
-	for(unsigned i = threadIdx.x; i < 4096u; i += blockDim.x) {
-		const unsigned orow = i >> 6, c = i & 63u;
-		nt_store(tile[tail_slot(rev6(c), rev6(orow))], &a[((u64)orow << sh) | (dst << 6) | c]);
+	const unsigned w = threadIdx.x >> 6, c = threadIdx.x & 63u, rowp = rev6(c);
+	const V *src = tile + rowp * 64u + rev6(w);          // + (rev6(4j) ^ g): LDS column rev6(4j + w) ^ g
+	const unsigned g = (rowp >> 3) & 7u;
+	V *out = a + (((u64)w << sh) + (dst << 6) + c);
+	const u64 step = (u64)4 << sh;
+#pragma unroll
+	for(unsigned j = 0; j < 16; ++j) {
+		nt_store(src[(__brev(4u * j) >> 26) ^ g], out + j * step);
+		if((j & 3u) == 3u) __builtin_amdgcn_sched_barrier(0);  // at most four reads in flight: registers
 	}
 }
 
+// Row-wise writes of a lane's two rows (rp, rp + 32) of eight columns
+// (ra << 3) | q into the LDS tile.
 template <typename V>
 __device__ __forceinline__ void tail_stage(V *tile, const V *v, unsigned rp, unsigned q) {
 #pragma unroll
-	for(int s2 = 0; s2 < 2; ++s2)
+	for(int s2 = 0; s2 < 2; ++s2) {
+		V *dst = tile + tail_slot(rp + 32u * (unsigned)s2, q);
 #pragma unroll
-		for(int ra = 0; ra < 8; ++ra) tile[tail_slot(rp + 32u * (unsigned)s2, ((unsigned)ra << 3) | q)] = v[s2 * 8 + ra];
+		for(int ra = 0; ra < 8; ++ra) dst[8 * ra] = v[s2 * 8 + ra];
+	}
 }
 
 template <typename V>
 __device__ __forceinline__ void tail_load(V *v, const V *__restrict__ a, unsigned rp, unsigned q, unsigned sh, u64 mm) {
 #pragma unroll
 	for(int s2 = 0; s2 < 2; ++s2) {
-		const u64 src = ((u64)(rp + 32u * (unsigned)s2) << sh) | (mm << 6) | q;
+		const V *src = a + ((((u64)(rp + 32u * (unsigned)s2)) << sh) + (mm << 6) + q);
 #pragma unroll
-		for(int ra = 0; ra < 8; ++ra) v[s2 * 8 + ra] = nt_load(&a[src | ((unsigned)ra << 3)]);
+		for(int ra = 0; ra < 8; ++ra) v[s2 * 8 + ra] = nt_load(src + 8 * ra);
 	}
 }
 
+// Waves per SIMD the tail kernel is compiled for (one wave of each workgroup
+// per SIMD): its LDS tile (64 KiB complex128, 32 KiB complex64) allows two
+// workgroups per CU, or more for complex64, where registers allow three.
+template <typename R> struct QftTailWaves {
+	static constexpr int value = sizeof(R) == 8 ? 2 : 3;
+};
+
 // Workgroups of 256 lanes; a lane holds 8 columns of two rows (rp and
-// rp + 32), eight lanes share a row, so every load instruction reads whole
-// 128 B lines.  Tile M is transformed and staged in LDS while tile rev(M) is
-// read and transformed in registers: one 64 KiB buffer per workgroup, two
-// workgroups per CU.
+// rp + 32) of each tile of a pair, and eight lanes share a row, so every load
+// instruction reads whole 128 B lines.  Both tiles of a pair are read before
+// either is written; each is transformed in registers and staged through one
+// 64 x 64 LDS tile for its transposed store.  The loads of the workgroup's
+// next pair are issued as soon as the registers they replace are staged, so
+// they are in flight while the current pair is stored.
 template <typename R>
-__global__ void __launch_bounds__(256) k_qft_tail_reverse(typename V2T<R>::t *__restrict__ a, QftTailParams P,
-                                                          const double2 *__restrict__ cq) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QftTailWaves<R>::value)))
+k_qft_tail_reverse(typename V2T<R>::t *__restrict__ a, QftTailParams P, const double2 *__restrict__ cq) {
 	using V = typename V2T<R>::t;
 	extern __shared__ char smem[];
 	V *tile = (V *)smem;  // 64 x 64
 	const unsigned t = threadIdx.x;
 	const unsigned lane = t & 63u, q = t & 7u, rp = t >> 3;
-	const double hc = 1.0 / sqrt(2.0);
 	const unsigned sh = P.n - 6;
-	double2 qf[3];
+	double2 qf[5];
 #pragma unroll
 	for(int b = 0; b < 3; ++b) qf[b] = cq[(1u << (b + 3)) - 1u + q];
-	for(u64 M = blockIdx.x; M < P.nmid; M += gridDim.x) {
-		const u64 Mr = P.m ? (__brevll(M) >> (64 - P.m)) : 0;
-		if(Mr < M) continue;  // uniform: the pair is handled at M' = Mr
-		V v[16];
-		tail_load(v, a, rp, q, sh, M);
-		tail_transform(v, lane, q, qf, cq, hc);
-		tail_transform(v + 8, lane, q, qf, cq, hc);
+	qf[3] = (q & 4u) ? cq[3 + (q & 3u)] : make_double2(1.0, 0.0);  // e^{i pi (q mod 4) / 4} on the target-2 partners
+	qf[4] = (q & 2u) ? cq[1 + (q & 1u)] : make_double2(1.0, 0.0);  // e^{i pi (q mod 2) / 2} on the target-1 partners
+	u64 p = blockIdx.x;
+	if(p >= P.npairs) return;  // uniform
+	u64 M, Mr;
+	tail_pair(P, p, M, Mr);
+	V va[16], vb[16];
+	tail_load(va, a, rp, q, sh, M);
+	tail_load(vb, a, rp, q, sh, Mr);
+	for(;;) {
+		const u64 pn = p + gridDim.x;
+		const bool more = pn < P.npairs;
+		u64 Mn = 0, Mnr = 0;
+		if(more) tail_pair(P, pn, Mn, Mnr);
+		tail_transform(va, lane, qf, cq);
+		tail_transform(va + 8, lane, qf, cq);
+		tail_transform(vb, lane, qf, cq);
+		tail_transform(vb + 8, lane, qf, cq);
 		__syncthreads();  // the previous pair's transposed reads are done
-		tail_stage(tile, v, rp, q);
-		if(Mr != M) {
-			tail_load(v, a, rp, q, sh, Mr);
-			tail_transform(v, lane, q, qf, cq, hc);
-			tail_transform(v + 8, lane, q, qf, cq, hc);
-		}
+		tail_stage(tile, va, rp, q);
 		__syncthreads();
-		tail_store(tile, a, sh, Mr);  // tile M into the place of tile rev(M), already read
-		if(Mr != M) {
-			__syncthreads();
-			tail_stage(tile, v, rp, q);
-			__syncthreads();
-			tail_store(tile, a, sh, M);
-		}
+		if(more) tail_load(va, a, rp, q, sh, Mn);
+		tail_store(tile, a, sh, Mr);  // tile M into the place of tile rev(M)
+		__syncthreads();
+		tail_stage(tile, vb, rp, q);
+		__syncthreads();
+		if(more) tail_load(vb, a, rp, q, sh, Mnr);
+		tail_store(tile, a, sh, M);  // palindromes (M == rev(M)) write the same values twice
+		if(!more) break;
+		p = pn;
+		M = Mn;
+		Mr = Mnr;
 	}
 }
 
@@ -990,9 +1074,12 @@ static hipError_t launch_qft_tail_t(const Launch &L, void *a, unsigned n, const 
 	QftTailParams P;
 	P.n = n;
 	P.m = n - 12;
-	P.nmid = 1ull << P.m;
+	P.h = P.m / 2;
+	P.K = 1u << P.h;
+	P.npairs = (u64)((P.m & 1u) ? 2 : 1) * ((u64)P.K * (P.K + 1) / 2);
 	const size_t shmem = 4096 * sizeof(V);
-	const unsigned grid = (unsigned)(P.nmid < 8192 ? P.nmid : 8192);
+	const u64 cap = 8192;  // 16 (complex128) to 32 pairs per workgroup at n = 30 (measured best)
+	const unsigned grid = (unsigned)(P.npairs < cap ? P.npairs : cap);
 	static bool attr = false;
 	if(!attr) {
 		(void)hipFuncSetAttribute((const void *)k_qft_tail_reverse<R>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
