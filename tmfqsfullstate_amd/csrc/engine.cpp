@@ -130,7 +130,7 @@ struct tmfqs_hip_state {
 	hipStream_t stream = nullptr;
 	double *d_red = nullptr;    // reduction partials + outputs
 	double *d_phase = nullptr;  // 5 x 256 complex phase tables
-	double *d_cin = nullptr;    // e^{i pi v / 2^j} at 2^j - 1 + v, j = 0..5 (QFT in-block constants)
+	double *d_cin = nullptr;    // e^{i pi v / 2^j} at 2^j - 1 + v, j = 0..5 (QFT in-block constants), FP64 then FP32
 	double *d_sums = nullptr;   // sampling chunk sums
 	ChunkTicks *d_ticks = nullptr;
 	int32_t *d_ulog = nullptr;
@@ -314,8 +314,12 @@ void build_phase_tables(tmfqs_hip_state *h) {
 			c.push_back(co);
 			c.push_back(s);
 		}
-	HIPC(hipMalloc((void **)&h->d_cin, c.size() * sizeof(double)));
+	// The same 63 constants rounded to FP32 follow the FP64 ones (complex64
+	// kernels use them as uniform operands without converting per use).
+	std::vector<float> cf(c.begin(), c.end());
+	HIPC(hipMalloc((void **)&h->d_cin, c.size() * sizeof(double) + cf.size() * sizeof(float)));
 	HIPC(hipMemcpy(h->d_cin, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice));
+	HIPC(hipMemcpy(h->d_cin + c.size(), cf.data(), cf.size() * sizeof(float), hipMemcpyHostToDevice));
 }
 
 double pass_bytes(const tmfqs_hip_state *h) { return 2.0 * (double)h->N * (double)h->esz; }

@@ -662,6 +662,22 @@ template <> __device__ __forceinline__ float2 rot<float2>(float2 x, double2 p) {
 	return make_float2(fmaf(x.x, pr, -x.y * pi), fmaf(x.x, pi, x.y * pr));
 }
 
+// A phase held in the amplitudes' own precision (rounded once where it is
+// computed, for phases kept in registers across a loop), and the rotation by
+// it: rotp(x, LanePhase<V>::of(p)) equals rot<V>(x, p) bit for bit.
+template <typename V> struct LanePhase {
+	using t = double2;
+	static __device__ __forceinline__ t of(double2 p) { return p; }
+};
+template <> struct LanePhase<float2> {
+	using t = float2;
+	static __device__ __forceinline__ t of(double2 p) { return make_float2((float)p.x, (float)p.y); }
+};
+__device__ __forceinline__ double2 rotp(double2 x, double2 p) { return rot<double2>(x, p); }
+__device__ __forceinline__ float2 rotp(float2 x, float2 p) {
+	return make_float2(fmaf(x.x, p.x, -x.y * p.y), fmaf(x.x, p.y, x.y * p.x));
+}
+
 // ------------------------------------------------------------------------
 // QFT block pass.  Targets are the 8 bit positions pbot+7 .. pbot (MSB first,
 // as makeQft orders its target qubits); for each target t it applies H on t
@@ -696,10 +712,15 @@ template <typename V> __device__ __forceinline__ void butterfly(V &x, V &y) {
 
 __device__ __forceinline__ double2 csqr(double2 p) { return make_double2(fma(p.x, p.x, -p.y * p.y), 2.0 * p.x * p.y); }
 
-// Waves per SIMD the QFT block kernel is compiled for: two workgroups per CU
-// wherever LDS allows (complex128 with 16-amplitude rows, complex64 with 32).
+// Waves per SIMD the QFT block kernel is compiled for, and whether a
+// workgroup prefetches its next tile during stage B.  complex128 with
+// 32-amplitude rows fills the LDS with one workgroup per CU and prefetches;
+// complex128 with 16-amplitude rows and complex64 with 32 run two workgroups
+// per CU, and complex64 keeps to 128 registers by not prefetching (the other
+// workgroup's loads overlap instead).
 template <typename R, int C> struct QftBlockWaves {
-	static constexpr int value = (sizeof(R) == 8 && C == 4) ? 2 : 1;
+	static constexpr int value = (sizeof(R) == 8 && C == 4) ? 2 : (sizeof(R) == 4 && C == 5) ? 4 : 1;
+	static constexpr bool prefetch = sizeof(R) == 8;
 };
 
 // Stage-A loads of tile `tid`: register ra holds block bits 7..4 = ra.
@@ -714,7 +735,7 @@ __device__ __forceinline__ void qft_block_load(V *v, const V *__restrict__ a, u6
 template <typename R, int C>
 __global__ void __launch_bounds__(16 << C) __attribute__((amdgpu_waves_per_eu(QftBlockWaves<R, C>::value)))
 k_qft_block(typename V2T<R>::t *__restrict__ a, QftBlockParams P, const double2 *__restrict__ ptab,
-            const double2 *__restrict__ cin) {
+            const double2 *__restrict__ cin, const typename LanePhase<typename V2T<R>::t>::t *__restrict__ cinp) {
 	using V = typename V2T<R>::t;
 	extern __shared__ char smem[];
 	V *tile = (V *)smem;
@@ -723,13 +744,19 @@ k_qft_block(typename V2T<R>::t *__restrict__ a, QftBlockParams P, const double2 
 	const unsigned f4 = t >> C;  // stage A: block bits 3..0; stage B: block bits 7..4
 	const unsigned pb = P.pbot;
 	const u64 lowmask = (1ull << pb) - 1ull;
-	// Per-thread constants: the stage-A thread phases e^{i pi f4 / 2^j}
-	// (j = 4..7) and the lane's part w^lo of the twiddle base.  The in-block
-	// ladder constants c[j][v] = cin[2^j - 1 + v] are read where they are used
-	// (uniform addresses: scalar loads).
-	double2 tpa[4];
+	// Per-thread constants: the stage-A ladder factors of target 4 + b on
+	// register bits low below it, c[b][low] e^{i pi f4 / 2^(4+b)} with the
+	// in-block constants c[b][v] = cin[2^b - 1 + v] = e^{i pi v / 2^b}, and the
+	// lane's part w^lo of the twiddle base.  Stage B's in-block constants are
+	// read where they are used (uniform addresses: scalar loads), from cinp:
+	// the same table in the amplitudes' precision.
+	typename LanePhase<V>::t pa[15];
 #pragma unroll
-	for(int j = 4; j < 8; ++j) tpa[j - 4] = ptab[(f4 << (7 - j)) & 255u];
+	for(int b = 0; b < 4; ++b) {
+		const double2 tp = ptab[(f4 << (3 - b)) & 255u];  // e^{i pi f4 / 2^(4+b)}
+#pragma unroll
+		for(int low = 0; low < (1 << b); ++low) pa[(1 << b) - 1 + low] = LanePhase<V>::of(cmul(cin[(1 << b) - 1 + low], tp));
+	}
 	const double2 zlo = phase_of_range((u64)lo << (56 - pb), ptab, 0, 4);
 	u64 tid = blockIdx.x;
 	if(tid >= P.ntiles) return;  // uniform
@@ -746,8 +773,7 @@ k_qft_block(typename V2T<R>::t *__restrict__ a, QftBlockParams P, const double2 
 #pragma unroll
 			for(int ra = 0; ra < 16; ++ra) {
 				if(!(ra & (1 << b))) continue;
-				const int low = ra & ((1 << b) - 1);
-				v[ra] = rot<V>(v[ra], cmul(cin[(1 << b) - 1 + low], tpa[b]));
+				v[ra] = rotp(v[ra], pa[(1 << b) - 1 + (ra & ((1 << b) - 1))]);
 			}
 		}
 		__syncthreads();  // previous tile's stage-B reads are complete
@@ -759,7 +785,8 @@ k_qft_block(typename V2T<R>::t *__restrict__ a, QftBlockParams P, const double2 
 		__syncthreads();
 		const u64 nxt = tid + gridDim.x;
 		const bool more = nxt < P.ntiles;  // uniform
-		if(more) qft_block_load<V, C>(v, a, nxt, pb, f4, lo);
+		if constexpr(QftBlockWaves<R, C>::prefetch)
+			if(more) qft_block_load<V, C>(v, a, nxt, pb, f4, lo);
 		// ---- stage B: registers index block bits 3..0
 		V w[16];
 #pragma unroll
@@ -778,7 +805,7 @@ k_qft_block(typename V2T<R>::t *__restrict__ a, QftBlockParams P, const double2 
 				if(!(rb & (1 << j))) continue;
 				const int low = rb & ((1 << j) - 1);
 				if(low == 0) continue;
-				w[rb] = rot<V>(w[rb], cin[(1 << j) - 1 + low]);
+				w[rb] = rotp(w[rb], cinp[(1 << j) - 1 + low]);
 			}
 		}
 		// ---- twiddle w^(r rev8(y)) / 16 for y = (f4, rb), r = rhi + lo:
@@ -806,6 +833,7 @@ k_qft_block(typename V2T<R>::t *__restrict__ a, QftBlockParams P, const double2 
 #pragma unroll
 		for(int rb = 0; rb < 16; ++rb) nt_store(w[rb], dst + ((u64)rb << pb));
 		if(!more) break;
+		if constexpr(!QftBlockWaves<R, C>::prefetch) qft_block_load<V, C>(v, a, nxt, pb, f4, lo);
 		tid = nxt;
 	}
 }
@@ -824,8 +852,10 @@ static hipError_t launch_qft_block_t(const Launch &L, void *a, unsigned n, unsig
 		(void)hipFuncSetAttribute((const void *)k_qft_block<R, C>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
 		attr = true;
 	}
+	using PT = typename LanePhase<V>::t;
+	const PT *cinp = sizeof(R) == 8 ? (const PT *)dcin : (const PT *)(dcin + 126);  // FP32 copy after the 63 FP64 constants
 	hipLaunchKernelGGL((k_qft_block<R, C>), dim3(grid), dim3(16 << C), shmem, L.stream, (V *)a, P, (const double2 *)dphase,
-	                   (const double2 *)dcin);
+	                   (const double2 *)dcin, cinp);
 	return hipGetLastError();
 }
 
@@ -893,21 +923,39 @@ __device__ __forceinline__ void tail_pair(const QftTailParams &P, u64 pair, u64 
 }
 
 __device__ __forceinline__ unsigned rev6(unsigned x) { return __brev(x) >> 26; }
-// LDS slot of (row, col) in a 64 x 64 tile: the column's low three bits are
-// XORed with row bits 5..3, which spreads the transposed reads (rows rev6(c)
-// for eight consecutive c) over the banks; the row-wise writes (eight lanes
-// per row) are spread by the column itself.  Column bits 5..3 are left alone,
-// so a lane's eight slots in a row are constant offsets from one address.
-__device__ __forceinline__ unsigned tail_slot(unsigned row, unsigned col) { return row * 64u + (col ^ ((row >> 3) & 7u)); }
+// Lane layout of a 64 x 64 tile: LB column bits in lanes (a row is one
+// 128 B line per load instruction: 8 lanes of complex128, 16 of complex64),
+// the other 6 - LB column bits in registers, and each lane holds 16 / 2^(6-LB)
+// rows spaced 256 / 2^LB apart, 16 amplitudes in all.
+template <typename V> struct TailLayout {
+	static constexpr unsigned LB = sizeof(V) == 16 ? 3u : 4u;  // lane (column) bits
+	static constexpr unsigned RB = 6u - LB;                    // register (column) bits
+	static constexpr unsigned LPR = 1u << LB;                  // lanes per row
+	static constexpr unsigned RPL = 16u >> RB;                 // rows per lane
+	static constexpr unsigned RSTEP = 256u >> LB;              // row spacing
+};
 
-// Value of lane (lane ^ MASK) within each group of eight lanes, by DPP moves:
+// LDS slot of (row, col) in a 64 x 64 tile: the column's lane bits are XORed
+// with row bits 5..6-LB, which spreads the transposed reads (rows rev6(c) for
+// 2^LB consecutive c: one 128 B bank sweep) over the banks; the row-wise
+// writes (2^LB lanes per row) are spread by the column itself.  The
+// register bits of the column are left alone, so a lane's slots in a row are
+// constant offsets from one address.
+template <typename V> __device__ __forceinline__ unsigned tail_slot(unsigned row, unsigned col) {
+	constexpr unsigned LB = TailLayout<V>::LB;
+	return row * 64u + (col ^ ((row >> (6u - LB)) & ((1u << LB) - 1u)));
+}
+
+// Value of lane (lane ^ MASK) within each row of 16 lanes, by DPP moves:
 // quad_perm [1,0,3,2] and [2,3,0,1] for MASK 1 and 2; for MASK 4,
-// row_half_mirror (lane ^ 7) followed by quad_perm [3,2,1,0] (lane ^ 3).
+// row_half_mirror (lane ^ 7) followed by quad_perm [3,2,1,0] (lane ^ 3); for
+// MASK 8, row_ror:8.
 template <unsigned MASK> __device__ __forceinline__ int dpp_xor(int x) {
 	if constexpr(MASK == 1) return __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);
 	else if constexpr(MASK == 2) return __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);
+	else if constexpr(MASK == 8) return __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);  // row_ror:8
 	else {
-		static_assert(MASK == 4, "groups of eight lanes");
+		static_assert(MASK == 4, "lane masks 1, 2, 4, 8");
 		return __builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(x, 0x141, 0xF, 0xF, false), 0x1B, 0xF, 0xF, false);
 	}
 }
@@ -933,36 +981,45 @@ template <unsigned MASK, typename V> __device__ __forceinline__ V lane_bfly(V x,
 	return mk<V>(dpp_xor<MASK>(x.x) + flip_if(x.x, f), dpp_xor<MASK>(x.y) + flip_if(x.y, f));
 }
 
-// The 64-point tail transform of one row held by eight lanes: registers are
-// column bits 5..3, the lane's position q in its group of eight is column
-// bits 2..0.  The ladder factor e^{i pi low / 2^t} of targets 5..3 splits into
-// a constant of the register index (uniform: scalar loads) and the lane's
-// qf[t - 3] = e^{i pi q / 2^t}; targets 2..0 are lane butterflies, with the
-// lane's ladder factors of targets 2 and 1 in qf[3] and qf[4].  Every lane
-// rotates (by 1 where the ladder does not act): no per-lane selects.  The six
-// Hadamard factors 1/sqrt(2) are applied as one exact scale by 1/8.
+// The 64-point tail transform of one row held by 2^LB lanes: registers are
+// column bits 5..LB, the lane's position q in its row is column bits LB-1..0.
+// The ladder factor e^{i pi low / 2^t} of a register target t splits into a
+// constant of the register index (uniform: scalar loads) and the lane's
+// qf[t - LB] = e^{i pi q / 2^t}; the lane targets LB-1..0 are DPP
+// butterflies, and target t > 0 then rotates its partners by
+// e^{i pi (q mod 2^t) / 2^t} (qf[RB + LB - 1 - t]; 1 on the other lanes, so
+// every lane rotates and nothing is selected).  The six Hadamard factors
+// 1/sqrt(2) are one exact scale by 1/8.
+template <typename V, unsigned MASK> __device__ __forceinline__ V tail_lane_step(V x, unsigned lane, const double2 *qf) {
+	constexpr unsigned LB = TailLayout<V>::LB, RB = TailLayout<V>::RB;
+	x = lane_bfly<MASK>(x, lane);
+	if constexpr(MASK > 1) x = rot<V>(x, qf[RB + LB - 1 - (31 - __builtin_clz(MASK))]);
+	return x;
+}
+
 template <typename V>
 __device__ __forceinline__ void tail_transform(V *v, unsigned lane, const double2 *qf, const double2 *__restrict__ cq) {
+	constexpr int LB = (int)TailLayout<V>::LB, RB = (int)TailLayout<V>::RB, NR = 1 << RB;
 #pragma unroll
-	for(int tt = 5; tt >= 3; --tt) {
-		const int b = tt - 3;
+	for(int tt = 5; tt >= LB; --tt) {
+		const int b = tt - LB;
 #pragma unroll
-		for(int ra = 0; ra < 8; ++ra)
+		for(int ra = 0; ra < NR; ++ra)
 			if(!(ra & (1 << b))) butterfly(v[ra], v[ra | (1 << b)]);
 #pragma unroll
-		for(int ra = 0; ra < 8; ++ra) {
+		for(int ra = 0; ra < NR; ++ra) {
 			if(!(ra & (1 << b))) continue;
 			const int hi = ra & ((1 << b) - 1);  // register bits below the target
-			v[ra] = rot<V>(v[ra], hi ? cmul(cq[(1 << tt) - 1 + (hi << 3)], qf[b]) : qf[b]);
+			v[ra] = rot<V>(v[ra], hi ? cmul(cq[(1 << tt) - 1 + (hi << LB)], qf[b]) : qf[b]);
 		}
 	}
 #pragma unroll
-	for(int ra = 0; ra < 8; ++ra) {
-		V x = lane_bfly<4>(v[ra], lane);  // target 2
-		x = rot<V>(x, qf[3]);
-		x = lane_bfly<2>(x, lane);  // target 1
-		x = rot<V>(x, qf[4]);
-		x = lane_bfly<1>(x, lane);  // target 0
+	for(int ra = 0; ra < NR; ++ra) {
+		V x = v[ra];
+		if constexpr(LB == 4) x = tail_lane_step<V, 8>(x, lane, qf);
+		x = tail_lane_step<V, 4>(x, lane, qf);
+		x = tail_lane_step<V, 2>(x, lane, qf);
+		x = tail_lane_step<V, 1>(x, lane, qf);
 		v[ra] = mk<V>(x.x * 0.125f, x.y * 0.125f);
 	}
 }
@@ -973,9 +1030,10 @@ __device__ __forceinline__ void tail_transform(V *v, unsigned lane, const double
 // column rev6(r) of row rev6(c).
 template <typename V>
 __device__ __forceinline__ void tail_store(const V *tile, V *__restrict__ a, unsigned sh, u64 dst) {
+	constexpr unsigned LB = TailLayout<V>::LB;
 	const unsigned w = threadIdx.x >> 6, c = threadIdx.x & 63u, rowp = rev6(c);
-	const V *src = tile + rowp * 64u + rev6(w);          // + (rev6(4j) ^ g): LDS column rev6(4j + w) ^ g
-	const unsigned g = (rowp >> 3) & 7u;
+	const V *src = tile + rowp * 64u + rev6(w);  // + (rev6(4j) ^ g): LDS column rev6(4j + w) ^ g
+	const unsigned g = (rowp >> (6u - LB)) & ((1u << LB) - 1u);
 	V *out = a + (((u64)w << sh) + (dst << 6) + c);
 	const u64 step = (u64)4 << sh;
 #pragma unroll
@@ -985,25 +1043,27 @@ __device__ __forceinline__ void tail_store(const V *tile, V *__restrict__ a, uns
 	}
 }
 
-// Row-wise writes of a lane's two rows (rp, rp + 32) of eight columns
-// (ra << 3) | q into the LDS tile.
+// Row-wise writes of a lane's rows rp + RSTEP s into the LDS tile: register
+// ra of row s is column (ra << LB) | q.
 template <typename V>
 __device__ __forceinline__ void tail_stage(V *tile, const V *v, unsigned rp, unsigned q) {
+	using Lay = TailLayout<V>;
 #pragma unroll
-	for(int s2 = 0; s2 < 2; ++s2) {
-		V *dst = tile + tail_slot(rp + 32u * (unsigned)s2, q);
+	for(unsigned s = 0; s < Lay::RPL; ++s) {
+		V *dst = tile + tail_slot<V>(rp + Lay::RSTEP * s, q);
 #pragma unroll
-		for(int ra = 0; ra < 8; ++ra) dst[8 * ra] = v[s2 * 8 + ra];
+		for(unsigned ra = 0; ra < (1u << Lay::RB); ++ra) dst[ra * Lay::LPR] = v[(s << Lay::RB) + ra];
 	}
 }
 
 template <typename V>
 __device__ __forceinline__ void tail_load(V *v, const V *__restrict__ a, unsigned rp, unsigned q, unsigned sh, u64 mm) {
+	using Lay = TailLayout<V>;
 #pragma unroll
-	for(int s2 = 0; s2 < 2; ++s2) {
-		const V *src = a + ((((u64)(rp + 32u * (unsigned)s2)) << sh) + (mm << 6) + q);
+	for(unsigned s = 0; s < Lay::RPL; ++s) {
+		const V *src = a + ((((u64)(rp + Lay::RSTEP * s)) << sh) + (mm << 6) + q);
 #pragma unroll
-		for(int ra = 0; ra < 8; ++ra) v[s2 * 8 + ra] = nt_load(src + 8 * ra);
+		for(unsigned ra = 0; ra < (1u << Lay::RB); ++ra) v[(s << Lay::RB) + ra] = nt_load(src + ra * Lay::LPR);
 	}
 }
 
@@ -1014,9 +1074,8 @@ template <typename R> struct QftTailWaves {
 	static constexpr int value = sizeof(R) == 8 ? 2 : 3;
 };
 
-// Workgroups of 256 lanes; a lane holds 8 columns of two rows (rp and
-// rp + 32) of each tile of a pair, and eight lanes share a row, so every load
-// instruction reads whole 128 B lines.  Both tiles of a pair are read before
+// Workgroups of 256 lanes in the layout of TailLayout: every load instruction
+// reads whole 128 B lines.  Both tiles of a pair are read before
 // either is written; each is transformed in registers and staged through one
 // 64 x 64 LDS tile for its transposed store.  The loads of the workgroup's
 // next pair are issued as soon as the registers they replace are staged, so
@@ -1028,13 +1087,20 @@ k_qft_tail_reverse(typename V2T<R>::t *__restrict__ a, QftTailParams P, const do
 	extern __shared__ char smem[];
 	V *tile = (V *)smem;  // 64 x 64
 	const unsigned t = threadIdx.x;
-	const unsigned lane = t & 63u, q = t & 7u, rp = t >> 3;
+	using Lay = TailLayout<V>;
+	constexpr unsigned LB = Lay::LB, RB = Lay::RB;
+	const unsigned lane = t & 63u, q = t & (Lay::LPR - 1u), rp = t >> LB;
 	const unsigned sh = P.n - 6;
+	// qf[b] = e^{i pi q / 2^(LB + b)} for the register targets; then, for lane
+	// targets t = LB-1 .. 1, e^{i pi (q mod 2^t) / 2^t} on lanes with bit t set.
 	double2 qf[5];
 #pragma unroll
-	for(int b = 0; b < 3; ++b) qf[b] = cq[(1u << (b + 3)) - 1u + q];
-	qf[3] = (q & 4u) ? cq[3 + (q & 3u)] : make_double2(1.0, 0.0);  // e^{i pi (q mod 4) / 4} on the target-2 partners
-	qf[4] = (q & 2u) ? cq[1 + (q & 1u)] : make_double2(1.0, 0.0);  // e^{i pi (q mod 2) / 2} on the target-1 partners
+	for(unsigned b = 0; b < RB; ++b) qf[b] = cq[(1u << (LB + b)) - 1u + q];
+#pragma unroll
+	for(unsigned tt = LB - 1; tt >= 1; --tt) {
+		const unsigned low = q & ((1u << tt) - 1u);
+		qf[RB + LB - 1 - tt] = ((q >> tt) & 1u) ? cq[(1u << tt) - 1u + low] : make_double2(1.0, 0.0);
+	}
 	u64 p = blockIdx.x;
 	if(p >= P.npairs) return;  // uniform
 	u64 M, Mr;
@@ -1047,10 +1113,11 @@ k_qft_tail_reverse(typename V2T<R>::t *__restrict__ a, QftTailParams P, const do
 		const bool more = pn < P.npairs;
 		u64 Mn = 0, Mnr = 0;
 		if(more) tail_pair(P, pn, Mn, Mnr);
-		tail_transform(va, lane, qf, cq);
-		tail_transform(va + 8, lane, qf, cq);
-		tail_transform(vb, lane, qf, cq);
-		tail_transform(vb + 8, lane, qf, cq);
+#pragma unroll
+		for(unsigned s = 0; s < Lay::RPL; ++s) {
+			tail_transform(va + (s << RB), lane, qf, cq);
+			tail_transform(vb + (s << RB), lane, qf, cq);
+		}
 		__syncthreads();  // the previous pair's transposed reads are done
 		tail_stage(tile, va, rp, q);
 		__syncthreads();
