@@ -109,18 +109,62 @@ def test_qft_tail_pair_shapes(precision):
     # The fused tail exchanges 64 x 64 tile pairs {M, rev(M)} over the n - 12
     # middle bits: n = 12, 13 have one or two palindromic tiles, odd n - 12 has
     # a middle bit, and every n here also runs the eight-target block passes.
+    # n = 13 .. 21 also start with a partial block pass of the leading
+    # (n - 6) mod 8 targets (K = 7, 1, 3, 5, 6, 7), and must run on the QFT
+    # kernels alone.
     for n in (12, 13, 14, 15, 17, 19, 20, 21):
         init = rand_state(n, 3)
         s = st(precision).load(n, init)
         ops, cnt = O.qft_ops(n)
+        s.profile(True)
         Register(s, n).run(ops, cnt)
         got = s.amplitudes()
+        used = set(s.profile_read())
+        if n >= 13:
+            assert used == {"k_qft_block", "k_qft_tail_reverse"}, (n, used)
         want = np.fft.ifft(cplx(init), norm="ortho")
         if precision == T.C128:
             assert np.abs(got - want).max() <= TOL, n
         else:
             assert np.abs(got - want).max() < 1e-5, n
             assert 1 - abs(np.vdot(want, got)) ** 2 <= 1e-6, n
+
+
+def _op_tuples(ops, cnt):
+    return [(ops[i].kind, ops[i].q0, ops[i].q1, ops[i].theta) for i in range(cnt)]
+
+
+@pytest.mark.parametrize("precision", [T.C128, T.C64], ids=["c128", "c64"])
+def test_qft_inside_longer_gate_runs(precision):
+    # Two QFTs with other gates before, between and after them, all in one
+    # flush: each QFT is still planned as a unit and runs on the QFT kernels.
+    n = 16
+    rng = np.random.default_rng(21)
+
+    def others(k):
+        seq = []
+        for _ in range(k):
+            kind = int(rng.choice([O.OP_H, O.OP_CX, O.OP_CPHASE, O.OP_SWAP]))
+            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+            seq.append((kind, a, b, float(rng.uniform(0, 2 * np.pi))))
+        return seq
+
+    qft = _op_tuples(*O.qft_ops(n))
+    seq = others(12) + qft + others(7) + qft + others(5)
+    init = rand_state(n, 9)
+    s = st(precision).load(n, init)
+    s.profile(True)
+    s.run_gates(seq)
+    got = s.amplitudes()
+    prof = s.profile_read()
+    assert prof["k_qft_tail_reverse"]["launches"] == 2
+    assert prof["k_qft_block"]["launches"] == 4  # a partial (two targets) and a full block per QFT
+    want = cplx(O.run_circuit(n, seq, init=init).amplitudes())
+    if precision == T.C128:
+        assert np.abs(got - want).max() <= TOL
+    else:
+        assert np.abs(got - want).max() < 1e-5
+        assert 1 - abs(np.vdot(want, got)) ** 2 <= 1e-6
 
 
 def test_qft_known_answers():

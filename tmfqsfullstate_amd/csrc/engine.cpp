@@ -668,11 +668,13 @@ static void phase_chunks(unsigned lo, unsigned hi, uint32_t &j0, uint32_t &j1) {
 	j1 = std::min(4u, (63u - std::min(lo, 63u)) / 8u);
 }
 
-// A pass that is exactly eight consecutive QFT targets t = pbot+7 .. pbot,
+// A pass that is exactly K <= 8 consecutive QFT targets t = top .. top-K+1,
 // each an H on t followed by the complete ladder of controlled phases
-// pi/2^(t-c) from every lower bit c (in any order), with pbot >= C.  Such a
-// pass runs as one k_qft_block launch.
-bool match_qft_block(const std::vector<QOp> &ops, const std::vector<size_t> &idx, unsigned C, unsigned &pbot) {
+// pi/2^(t-c) from every lower bit c (in any order).  Such a pass runs as one
+// k_qft_block launch over the block bits top .. top-7 (pbot = top - 7 >= C),
+// whose top K bits are its targets.
+bool match_qft_block(const std::vector<QOp> &ops, const std::vector<size_t> &idx, unsigned C, unsigned n, unsigned &pbot,
+                     unsigned &K) {
 	size_t k = 0;
 	int targets = 0;
 	unsigned prev = 0;
@@ -693,9 +695,13 @@ bool match_qft_block(const std::vector<QOp> &ops, const std::vector<size_t> &idx
 		prev = t;
 		++targets;
 	}
-	if(targets != 8) return false;
-	pbot = prev;
-	return pbot >= C;
+	if(targets < 1 || targets > 8) return false;
+	// The targets are the top `targets` bits of the block pbot+7 .. pbot.
+	const unsigned top = prev + (unsigned)targets - 1;
+	if(top < 7 + C || top >= n) return false;
+	pbot = top - 7;
+	K = (unsigned)targets;
+	return true;
 }
 
 // Recognises a complete QFT as makeQft emits it (circuit.cpp:87-106): for
@@ -836,7 +842,7 @@ std::vector<TileStage> plan_stages(TileOp *ops, size_t cnt, unsigned T, unsigned
 // One launch of a planned gate run.
 struct PassLaunch {
 	enum Kind { TILE, QFT_BLOCK, QFT_TAIL } kind = TILE;
-	unsigned pbot = 0, qc = 0;  // QFT_BLOCK
+	unsigned pbot = 0, qc = 0, qk = 0;  // QFT_BLOCK: block bits pbot+7 .. pbot, the top qk of them targets
 	TilePass tp{};              // TILE
 	unsigned W = 0;
 	std::vector<TileOp> ops;
@@ -910,13 +916,14 @@ GatePlan plan_gates(const PlanCfg &cfg, unsigned n, const std::vector<QOp> &in, 
 			plan.launches.push_back(std::move(L));
 			continue;
 		}
-		unsigned pbot = 0;
+		unsigned pbot = 0, qk = 0;
 		const unsigned qc = cfg.qft_c;
-		if(ladders && n <= 40 && match_qft_block(ops, P.ops, qc, pbot)) {
+		if(ladders && n <= 40 && match_qft_block(ops, P.ops, qc, n, pbot, qk)) {
 			PassLaunch L;
 			L.kind = PassLaunch::QFT_BLOCK;
 			L.pbot = pbot;
 			L.qc = qc;
+			L.qk = qk;
 			plan.launches.push_back(std::move(L));
 			continue;
 		}
@@ -1117,8 +1124,53 @@ GatePlan plan_gates(const PlanCfg &cfg, unsigned n, const std::vector<QOp> &in, 
 	return plan;
 }
 
+// A complete QFT in makeQft's op order (circuit.cpp:87-106) starting at
+// in[s]: for t = n-1 .. 0 an H on t and then the ladder pi/2^(t-c) from every
+// lower bit c (in any order), then floor(n/2) swaps composing to the full bit
+// reversal.  Returns the end of the unit, or s when there is none.
+static size_t qft_unit_end(const std::vector<QOp> &in, size_t s, size_t hi, unsigned n) {
+	size_t k = s;
+	for(unsigned i = 0; i < n; ++i) {
+		const unsigned t = n - 1 - i;
+		if(k >= hi || in[k].kind != Q_H || in[k].b0 != t) return s;
+		++k;
+		const u64 full = (1ull << t) - 1ull;
+		u64 seen = 0;
+		while(seen != full) {
+			unsigned hb, lb;
+			if(k >= hi || !is_qft_ladder_cp(in[k], hb, lb) || hb != t || ((seen >> lb) & 1ull)) return s;
+			seen |= 1ull << lb;
+			++k;
+		}
+	}
+	std::vector<unsigned> m(n);
+	for(unsigned b = 0; b < n; ++b) m[b] = b;
+	for(unsigned i = 0; i < n / 2; ++i, ++k) {
+		if(k >= hi || in[k].kind != Q_SWAP) return s;
+		std::swap(m[in[k].b0], m[in[k].b1]);
+	}
+	for(unsigned b = 0; b < n; ++b)
+		if(m[b] != n - 1 - b) return s;
+	return k;
+}
+
 void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_t hi, const GateTarget *tgt) {
 	const unsigned n = h->n;
+	if(!tgt && !h->cfg.exact_phases && n >= 12 && n <= 40) {
+		// A complete QFT inside a longer gate run (other gates around it, or
+		// several QFTs) is planned on its own, so it runs on the QFT kernels.
+		const size_t len = (size_t)n + (size_t)n * (n - 1) / 2 + n / 2;
+		if(hi - lo > len)
+			for(size_t s = lo; s + len <= hi; ++s) {
+				if(in[s].kind != Q_H || in[s].b0 != n - 1) continue;
+				const size_t e = qft_unit_end(in, s, hi, n);
+				if(e == s) continue;
+				if(s > lo) exec_gates(h, in, lo, s, tgt);
+				exec_gates(h, in, s, e, tgt);
+				if(e < hi) exec_gates(h, in, e, hi, tgt);
+				return;
+			}
+	}
 	void *const A = tgt ? tgt->a : h->amp;
 	void *const B = tgt ? tgt->b : nullptr;
 	const bool dual = B != nullptr;
@@ -1139,7 +1191,7 @@ void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_
 		}
 		if(L.kind == PassLaunch::QFT_BLOCK) {
 			ProfScope ps(h, "k_qft_block", pass_bytes(h));
-			HIPC(launch_qft_block(h->L(), A, n, L.pbot, h->d_phase, h->d_cin, L.qc));
+			HIPC(launch_qft_block(h->L(), A, n, L.pbot, h->d_phase, h->d_cin, L.qc, L.qk));
 			continue;
 		}
 		const TileOp *dops = (const TileOp *)arena_upload(h, L.ops.data(), L.ops.size() * sizeof(TileOp));

@@ -116,11 +116,13 @@ hipError_t launch_bitswap(const Launch &L, void *a, unsigned n, unsigned p, unsi
 hipError_t launch_involution(const Launch &L, void *a, unsigned n, const unsigned *perm, unsigned C);
 hipError_t launch_ew_chain(const Launch &L, void *a, unsigned n, const EwStep *dsteps, unsigned nsteps);
 hipError_t launch_scatter(const Launch &L, void *a, const Scatter *dsc, unsigned count);
-// QFT block pass over targets pbot+7 .. pbot (see kernels.hip); dcin holds the
-// 15 constants e^{i pi v / 2^j}, j = 0..3, v < 2^j.
+// QFT block pass over the block bits pbot+7 .. pbot (see kernels.hip): the
+// top K of them are targets (K = 8: a full block; fewer: a partial leading
+// pass), each with its whole ladder.  dcin holds the constants
+// e^{i pi v / 2^j} at 2^j - 1 + v (FP64, then the same in FP32).
 // C contiguous low bits per tile: 4 or 5 (complex128), 5 or 6 (complex64).
 hipError_t launch_qft_block(const Launch &L, void *a, unsigned n, unsigned pbot, const double *dphase, const double *dcin,
-                            unsigned C);
+                            unsigned C, unsigned K);
 // Low contiguous bits a QFT block tile keeps: `requested` when supported,
 // else 5 for both precisions.
 unsigned qft_block_coalesce_bits(int prec, int requested);

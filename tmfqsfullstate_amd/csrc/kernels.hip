@@ -700,6 +700,9 @@ __device__ __forceinline__ float2 rotp(float2 x, float2 p) {
 struct QftBlockParams {
 	u64 ntiles;  // 2^(n - 8 - C)
 	uint32_t n, pbot;
+	uint32_t K;    // targets: the top K block bits (8, or fewer in a partial pass)
+	uint32_t pad;
+	double scale;  // (1/sqrt(2))^K
 };
 
 // Unscaled butterfly (x + y, x - y): transforms that apply several Hadamards
@@ -732,7 +735,10 @@ __device__ __forceinline__ void qft_block_load(V *v, const V *__restrict__ a, u6
 	for(int ra = 0; ra < 16; ++ra) v[ra] = nt_load(src + ((u64)ra << (pb + 4)));
 }
 
-template <typename R, int C>
+// PART: a partial pass whose targets are only the top K < 8 block bits (the
+// leading (n - 6) mod 8 targets of a QFT); the other block bits are ladder
+// controls only.
+template <typename R, int C, bool PART>
 __global__ void __launch_bounds__(16 << C) __attribute__((amdgpu_waves_per_eu(QftBlockWaves<R, C>::value)))
 k_qft_block(typename V2T<R>::t *__restrict__ a, QftBlockParams P, const double2 *__restrict__ ptab,
             const double2 *__restrict__ cin, const typename LanePhase<typename V2T<R>::t>::t *__restrict__ cinp) {
@@ -767,6 +773,8 @@ k_qft_block(typename V2T<R>::t *__restrict__ a, QftBlockParams P, const double2 
 #pragma unroll
 		for(int j = 7; j >= 4; --j) {
 			const int b = j - 4;
+			if constexpr(PART)
+				if(j < 8 - (int)P.K) continue;  // uniform
 #pragma unroll
 			for(int ra = 0; ra < 16; ++ra)
 				if(!(ra & (1 << b))) butterfly(v[ra], v[ra | (1 << b)]);
@@ -796,6 +804,8 @@ k_qft_block(typename V2T<R>::t *__restrict__ a, QftBlockParams P, const double2 
 		}
 #pragma unroll
 		for(int j = 3; j >= 0; --j) {
+			if constexpr(PART)
+				if(j < 8 - (int)P.K) continue;  // uniform
 #pragma unroll
 			for(int rb = 0; rb < 16; ++rb)
 				if(!(rb & (1 << j))) butterfly(w[rb], w[rb | (1 << j)]);
@@ -818,13 +828,18 @@ k_qft_block(typename V2T<R>::t *__restrict__ a, QftBlockParams P, const double2 
 #pragma unroll
 		for(int k = 1; k < 4; ++k) z[k] = csqr(z[k - 1]);
 		const double2 u = csqr(z[3]);
-		double2 ph = make_double2(0.0625, 0.0);
+		// A partial pass keeps the target bits of rev8(y) only (its low K bits):
+		// f4 bit b counts when 3 - b < K, and register rb takes u^(rev4(rb) & m2).
+		double2 ph = make_double2(PART ? P.scale : 0.0625, 0.0);
 #pragma unroll
 		for(int b = 0; b < 4; ++b)
-			if((f4 >> b) & 1u) ph = cmul(ph, z[3 - b]);
+			if(((f4 >> b) & 1u) && (!PART || 3 - b < (int)P.K)) ph = cmul(ph, z[3 - b]);
+		const unsigned m2 = PART ? (P.K > 4 ? (1u << (P.K - 4)) - 1u : 0u) : 15u;
+		const double2 ph0 = ph;
 #pragma unroll
 		for(int k = 0; k < 16; ++k) {
 			const int rb = (int)(__brev((unsigned)k) >> 28);
+			if(PART && k && !(k & m2)) ph = ph0;
 			w[rb] = rot<V>(w[rb], ph);
 			if(k < 15) ph = cmul(ph, u);
 		}
@@ -838,36 +853,47 @@ k_qft_block(typename V2T<R>::t *__restrict__ a, QftBlockParams P, const double2 
 	}
 }
 
-template <typename R, int C>
-static hipError_t launch_qft_block_t(const Launch &L, void *a, unsigned n, unsigned pbot, const double *dphase, const double *dcin) {
+template <typename R, int C, bool PART>
+static hipError_t launch_qft_block_t(const Launch &L, void *a, unsigned n, unsigned pbot, unsigned K, const double *dphase,
+                                     const double *dcin) {
 	using V = typename V2T<R>::t;
 	QftBlockParams P;
 	P.n = n;
 	P.pbot = pbot;
+	P.K = K;
+	P.pad = 0;
+	P.scale = std::ldexp(K & 1u ? 1.0 / std::sqrt(2.0) : 1.0, -(int)(K / 2));  // (1/sqrt(2))^K, one rounding at most
 	P.ntiles = 1ull << (n - 8 - C);
 	const unsigned grid = (unsigned)(P.ntiles < (1u << 20) ? P.ntiles : (1u << 20));
 	const size_t shmem = (size_t)((256u << C) + 16u) * sizeof(V);
 	static bool attr = false;
 	if(!attr) {
-		(void)hipFuncSetAttribute((const void *)k_qft_block<R, C>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+		(void)hipFuncSetAttribute((const void *)k_qft_block<R, C, PART>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
 		attr = true;
 	}
 	using PT = typename LanePhase<V>::t;
 	const PT *cinp = sizeof(R) == 8 ? (const PT *)dcin : (const PT *)(dcin + 126);  // FP32 copy after the 63 FP64 constants
-	hipLaunchKernelGGL((k_qft_block<R, C>), dim3(grid), dim3(16 << C), shmem, L.stream, (V *)a, P, (const double2 *)dphase,
-	                   (const double2 *)dcin, cinp);
+	hipLaunchKernelGGL((k_qft_block<R, C, PART>), dim3(grid), dim3(16 << C), shmem, L.stream, (V *)a, P,
+	                   (const double2 *)dphase, (const double2 *)dcin, cinp);
 	return hipGetLastError();
 }
 
+template <typename R, int C>
+static hipError_t launch_qft_block_k(const Launch &L, void *a, unsigned n, unsigned pbot, unsigned K, const double *dphase,
+                                     const double *dcin) {
+	return K == 8 ? launch_qft_block_t<R, C, false>(L, a, n, pbot, 8, dphase, dcin)
+	              : launch_qft_block_t<R, C, true>(L, a, n, pbot, K, dphase, dcin);
+}
+
 hipError_t launch_qft_block(const Launch &L, void *a, unsigned n, unsigned pbot, const double *dphase, const double *dcin,
-                            unsigned C) {
-	if(pbot < C || pbot + 8 > n) return hipErrorInvalidValue;
+                            unsigned C, unsigned K) {
+	if(pbot < C || pbot + 8 > n || K < 1 || K > 8) return hipErrorInvalidValue;
 	if(L.prec == 0) {
-		if(C == 4) return launch_qft_block_t<double, 4>(L, a, n, pbot, dphase, dcin);
-		if(C == 5) return launch_qft_block_t<double, 5>(L, a, n, pbot, dphase, dcin);
+		if(C == 4) return launch_qft_block_k<double, 4>(L, a, n, pbot, K, dphase, dcin);
+		if(C == 5) return launch_qft_block_k<double, 5>(L, a, n, pbot, K, dphase, dcin);
 	} else {
-		if(C == 5) return launch_qft_block_t<float, 5>(L, a, n, pbot, dphase, dcin);
-		if(C == 6) return launch_qft_block_t<float, 6>(L, a, n, pbot, dphase, dcin);
+		if(C == 5) return launch_qft_block_k<float, 5>(L, a, n, pbot, K, dphase, dcin);
+		if(C == 6) return launch_qft_block_k<float, 6>(L, a, n, pbot, K, dphase, dcin);
 	}
 	return hipErrorInvalidValue;
 }
