@@ -297,8 +297,12 @@ int tmfqs_hip_batch_info(tmfqs_hip_batch *b, tmfqs_hip_batch_stats *out);
 /* Adjoint gradients of one diagonal observable (see tmfqs_hip_adjoint) for
  * every circuit of the batch, each from its basis state: expvals[c] per
  * circuit; grads holds the CPHASE/PHASE gradients of circuit 0 in record
- * order, then those of circuit 1, and so on (grad_count in total).  Circuits
- * run one after another on one device handle. */
+ * order, then those of circuit 1, and so on (grad_count in total).  One batch
+ * run gives the final states, then round r steps pass r of every circuit's
+ * backward sweep (psi and lambda = O psi in the circuit's two buffers,
+ * without checkpoints) in one work-list kernel, with per-item gradient
+ * partials summed in a fixed order.  A second batch run leaves every final
+ * state in place again, as after tmfqs_hip_batch_run. */
 int tmfqs_hip_batch_adjoint(tmfqs_hip_batch *b, const tmfqs_hip_diag_term *terms, size_t nterms, double *expvals, double *grads,
                             size_t grad_count);
 int tmfqs_hip_batch_destroy(tmfqs_hip_batch *b);

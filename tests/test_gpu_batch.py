@@ -148,6 +148,54 @@ def test_batched_adjoint_vs_single_and_oracle():
         with T.HipState() as s:
             s.init_basis(n, 0)
             e1, g1 = s.adjoint(ops, cnt, terms)
-        assert e[i] == e1 and np.array_equal(g[i], g1)
+        # the batch steps every circuit back from its final state in one dual
+        # sweep (the single path restarts from checkpoints): same values up
+        # to rounding
+        assert abs(e[i] - e1) <= 1e-14 and np.max(np.abs(g[i] - g1)) <= 1e-12
         ps = O.param_shift_grad(n, oracle_ops(ops, cnt), cnt, terms)
         assert np.max(np.abs(g[i] - ps)) <= 1e-10
+
+
+def test_batched_adjoint_many_parameters_and_states_kept():
+    # Circuits with more gradient reductions per pass than one launch holds
+    # (64), uneven sizes up to 14 qubits, and the forward states still in
+    # place after the adjoint.  complex64 follows its own rounding path (the
+    # batch steps back from the final FP32 state through every gate).
+    rng = np.random.default_rng(17)
+    circuits = []
+    for n in (14, 7, 12):
+        seq = []
+        for _ in range(240):
+            k = int(rng.integers(0, 4))
+            if k == 0:
+                seq.append((T.OP_H, int(rng.integers(n))))
+            elif k == 1:
+                seq.append((T.OP_PHASE, int(rng.integers(n)), 0, float(rng.uniform(0, 6.3))))
+            else:
+                a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+                seq.append((T.OP_CPHASE if k == 2 else T.OP_CX, a, b, float(rng.uniform(0, 6.3))))
+        arr = (T.Op * len(seq))()
+        for i, g in enumerate(seq):
+            arr[i].kind, arr[i].q0 = g[0], g[1]
+            arr[i].q1 = g[2] if len(g) > 2 else 0
+            arr[i].theta = g[3] if len(g) > 3 else 0.0
+        circuits.append((n, arr, len(seq)))
+    terms = [(T.DIAG_Z, 0.5, (1 << 0) | (1 << 3)), (T.DIAG_PROJECTOR, -1.0, 5)]
+    print("batched adjoint, many parameters", flush=True)
+    for prec in (T.C128, T.C64):
+        with Batch(circuits, precision=prec) as b:
+            b.run()
+            before = [b.export(i) for i in range(len(circuits))]
+            e, g = b.adjoint(terms)
+            for i in range(len(circuits)):
+                assert np.array_equal(b.export(i), before[i])
+        for i, (n, ops, cnt) in enumerate(circuits):
+            with T.HipState(precision=prec) as s:
+                s.init_basis(n, 0)
+                e1, g1 = s.adjoint(ops, cnt, terms)
+            tol = 1e-12 if prec == T.C128 else 1e-4
+            assert len(g[i]) == len(g1) > 64
+            assert abs(e[i] - e1) <= tol and np.max(np.abs(g[i] - g1)) <= tol, (prec, i)
+            if prec == T.C128:
+                ps = O.param_shift_grad(n, oracle_ops(ops, cnt), cnt, terms)
+                assert np.max(np.abs(g[i] - ps)) <= 1e-10

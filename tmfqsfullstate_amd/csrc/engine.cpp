@@ -125,6 +125,8 @@ struct tmfqs_hip_state {
 	bool init = false;
 	void *amp = nullptr;
 	size_t amp_bytes = 0;
+	size_t amp_cap = 0;          // allocation behind amp (>= amp_bytes)
+	bool keep_capacity = false;  // reuse a larger allocation for a smaller state (batched adjoint)
 	void *scratch = nullptr;  // lazily allocated second state buffer
 	size_t scratch_bytes = 0;
 	hipStream_t stream = nullptr;
@@ -336,14 +338,16 @@ void allocate_state(tmfqs_hip_state *h, unsigned n) {
 	h->tracked.clear();
 	h->dirty.clear();
 	h->dirty_set.clear();
-	if(h->amp_bytes != bytes) {
+	if(h->keep_capacity && h->amp && h->amp_cap >= bytes) {
+		h->amp_bytes = bytes;
+	} else if(h->amp_bytes != bytes) {
 		HIPC(hipStreamSynchronize(h->stream));
 		if(h->amp) hipFree(h->amp);
 		h->amp = nullptr;
-		h->amp_bytes = 0;
+		h->amp_bytes = h->amp_cap = 0;
 		h->init = false;
 		HIPC(hipMalloc(&h->amp, bytes));
-		h->amp_bytes = bytes;
+		h->amp_bytes = h->amp_cap = bytes;
 		if(h->scratch) {
 			hipFree(h->scratch);
 			h->scratch = nullptr;
@@ -1305,7 +1309,8 @@ std::vector<QOp> convert_ops(tmfqs_hip_state *h, const tmfqs_hip_op *ops, size_t
 }
 
 // ------------------------------------------------------------ adjoint
-ObsParams make_observable(const tmfqs_hip_state *h, const tmfqs_hip_diag_term *terms, size_t nterms) {
+// The observable's device form on n qubits (qubit q is bit n-1-q).
+ObsParams make_observable(unsigned n, const tmfqs_hip_diag_term *terms, size_t nterms) {
 	ObsParams O{};
 	if(nterms && !terms) throw std::invalid_argument("diagonal observable: null term list");
 	for(size_t i = 0; i < nterms; ++i) {
@@ -1315,14 +1320,14 @@ ObsParams make_observable(const tmfqs_hip_state *h, const tmfqs_hip_diag_term *t
 			u64 m = 0;
 			for(unsigned q = 0; q < 64; ++q)
 				if((t.bits >> q) & 1ull) {
-					validate_qubit(h, q, "diagonal observable");
-					m |= 1ull << pbit(h, q);
+					if(q >= n) throw std::out_of_range("diagonal observable qubit index out of range");
+					m |= 1ull << (n - 1u - q);
 				}
 			O.zmask[O.nz] = m;
 			O.zcoef[O.nz++] = t.coef;
 		} else if(t.kind == TMFQS_HIP_DIAG_PROJECTOR) {
 			if(O.np == kMaxObsTerms) throw std::invalid_argument("diagonal observable: too many projector terms");
-			validate_state(h, t.bits, "diagonal observable");
+			if(n < 64 && t.bits >= (1ull << n)) throw std::out_of_range("diagonal observable state index out of range");
 			O.pstate[O.np] = t.bits;
 			O.pcoef[O.np++] = t.coef;
 		} else {
@@ -1330,6 +1335,10 @@ ObsParams make_observable(const tmfqs_hip_state *h, const tmfqs_hip_diag_term *t
 		}
 	}
 	return O;
+}
+
+ObsParams make_observable(const tmfqs_hip_state *h, const tmfqs_hip_diag_term *terms, size_t nterms) {
+	return make_observable(h->n, terms, nterms);
 }
 
 double device_expectation(tmfqs_hip_state *h, const void *a, void *lam, const ObsParams &O) {
@@ -1411,8 +1420,11 @@ std::vector<size_t> checkpoint_bounds(const tmfqs_hip_state *h, const std::vecto
 	return b;
 }
 
-void adjoint_run(tmfqs_hip_state *h, std::vector<QOp> &qops, const std::vector<size_t> &params, const ObsParams &O,
-                 unsigned checkpoint_gates, double *expval, double *grads) {
+// Enqueues the whole adjoint sweep on the handle's stream without waiting
+// for it: E goes to dev_e and the gradients of the P parameters (in params
+// order) to dev_grads, both device memory.
+void adjoint_enqueue(tmfqs_hip_state *h, std::vector<QOp> &qops, const std::vector<size_t> &params, const ObsParams &O,
+                     unsigned checkpoint_gates, double *dev_e, double *dev_grads) {
 	const size_t G = qops.size();
 	size_t seg = checkpoint_gates ? checkpoint_gates : (size_t)std::ceil(std::sqrt((double)std::max<size_t>(G, 1)));
 	std::vector<size_t> bounds = checkpoint_bounds(h, qops, seg);
@@ -1427,7 +1439,7 @@ void adjoint_run(tmfqs_hip_state *h, std::vector<QOp> &qops, const std::vector<s
 	}
 	const size_t S = bounds.size() - 1;
 	const size_t nbuf = S + 1;
-	if(h->adj_buf_bytes != h->amp_bytes) {
+	if(h->adj_buf_bytes < h->amp_bytes || (!h->keep_capacity && h->adj_buf_bytes != h->amp_bytes)) {
 		HIPC(hipStreamSynchronize(h->stream));
 		for(void *p : h->adj_bufs) hipFree(p);
 		h->adj_bufs.clear();
@@ -1435,17 +1447,11 @@ void adjoint_run(tmfqs_hip_state *h, std::vector<QOp> &qops, const std::vector<s
 	}
 	while(h->adj_bufs.size() < nbuf) {
 		void *p = nullptr;
-		HIPC(hipMalloc(&p, h->amp_bytes));
+		HIPC(hipMalloc(&p, h->adj_buf_bytes));
 		h->adj_bufs.push_back(p);
 	}
 	void *lam = h->adj_bufs[S];
 	const size_t P = params.size();
-	if(h->grads_cap < std::max<size_t>(P, 1)) {
-		hipFree(h->d_grads);
-		h->d_grads = nullptr;
-		HIPC(hipMalloc((void **)&h->d_grads, std::max<size_t>(P, 1) * sizeof(double)));
-		h->grads_cap = std::max<size_t>(P, 1);
-	}
 	if(!h->d_gpart) HIPC(hipMalloc((void **)&h->d_gpart, (size_t)kMaxGradSlots * 4096 * sizeof(double)));
 	// forward sweep with a checkpoint at every segment end
 	for(size_t s = 0; s < S; ++s) {
@@ -1453,9 +1459,13 @@ void adjoint_run(tmfqs_hip_state *h, std::vector<QOp> &qops, const std::vector<s
 		ProfScope ps(h, "k_copy", pass_bytes(h));
 		HIPC(launch_copy(h->L(), h->amp, h->adj_bufs[s], h->n));
 	}
-	*expval = device_expectation(h, h->adj_bufs[S - 1], lam, O);
+	{
+		ProfScope ps(h, "k_observable", 2.0 * (double)h->N * (double)h->esz);
+		HIPC(launch_observable(h->L(), h->adj_bufs[S - 1], lam, h->n, O, h->d_red, h->d_red + 2 * kReduceBlocks));
+	}
+	HIPC(hipMemcpyAsync(dev_e, h->d_red + 2 * kReduceBlocks, sizeof(double), hipMemcpyDeviceToDevice, h->stream));
 	if(P == 0) return;
-	HIPC(hipMemsetAsync(h->d_grads, 0, P * sizeof(double), h->stream));
+	HIPC(hipMemsetAsync(dev_grads, 0, P * sizeof(double), h->stream));
 	std::vector<int64_t> slot(G, -1);
 	for(size_t p = 0; p < P; ++p) slot[params[p]] = (int64_t)p;
 	// backward sweep: psi restarts from each segment's checkpoint, lambda runs through
@@ -1463,7 +1473,7 @@ void adjoint_run(tmfqs_hip_state *h, std::vector<QOp> &qops, const std::vector<s
 	tg.b = lam;
 	tg.tile_delta = -1;
 	tg.gpart = h->d_gpart;
-	tg.dgrads = h->d_grads;
+	tg.dgrads = dev_grads;
 	tg.grad_scale = -2.0;  // dE/dtheta = -2 Im <lambda| P |psi>
 	for(size_t s = S; s-- > 0;) {
 		std::vector<QOp> rev;
@@ -1489,7 +1499,21 @@ void adjoint_run(tmfqs_hip_state *h, std::vector<QOp> &qops, const std::vector<s
 		tg.a = h->adj_bufs[s];
 		exec_gates(h, rev, 0, rev.size(), &tg);
 	}
-	HIPC(hipMemcpyAsync(grads, h->d_grads, P * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+}
+
+void adjoint_run(tmfqs_hip_state *h, std::vector<QOp> &qops, const std::vector<size_t> &params, const ObsParams &O,
+                 unsigned checkpoint_gates, double *expval, double *grads) {
+	const size_t P = params.size();
+	if(h->grads_cap < P + 1) {
+		HIPC(hipStreamSynchronize(h->stream));
+		hipFree(h->d_grads);
+		h->d_grads = nullptr;
+		HIPC(hipMalloc((void **)&h->d_grads, (P + 1) * sizeof(double)));
+		h->grads_cap = P + 1;
+	}
+	adjoint_enqueue(h, qops, params, O, checkpoint_gates, h->d_grads + P, h->d_grads);
+	if(P) HIPC(hipMemcpyAsync(grads, h->d_grads, P * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+	HIPC(hipMemcpyAsync(expval, h->d_grads + P, sizeof(double), hipMemcpyDeviceToHost, h->stream));
 	HIPC(hipStreamSynchronize(h->stream));
 }
 
@@ -2718,7 +2742,7 @@ int tmfqs_hip_batch_info(tmfqs_hip_batch *b, tmfqs_hip_batch_stats *out) {
 
 int tmfqs_hip_batch_adjoint(tmfqs_hip_batch *b, const tmfqs_hip_diag_term *terms, size_t nterms, double *expvals, double *grads,
                             size_t grad_count) {
-	tmfqs_hip_state *h = nullptr;
+	char *dblock = nullptr;
 	ABI_BEGIN
 	if(!b) throw std::invalid_argument("null tmfqs_hip_batch handle");
 	if(!expvals) throw std::invalid_argument("tmfqs_hip_batch_adjoint: null expectation output");
@@ -2735,23 +2759,160 @@ int tmfqs_hip_batch_adjoint(tmfqs_hip_batch *b, const tmfqs_hip_diag_term *terms
 	if(grad_count && !grads) throw std::invalid_argument("tmfqs_hip_batch_adjoint: null gradient output");
 	HIPC(hipSetDevice(b->cfg.device));
 	HIPC(hipStreamSynchronize(b->stream));
-	int rc = tmfqs_hip_create(&b->cfg, &h);
-	if(rc != TMFQS_HIP_OK) throw_status(rc);
-	size_t off = 0;
-	for(size_t c = 0; c < b->n.size(); ++c) {
-		rc = tmfqs_hip_init_basis(h, b->n[c], b->init[c], 1.0, 0.0);
-		if(rc != TMFQS_HIP_OK) throw_status(rc);
-		const ObsParams O = make_observable(h, terms, nterms);
-		std::vector<QOp> q = b->ops[c];
-		adjoint_run(h, q, params[c], O, 0, &expvals[c], grads + off);
-		off += params[c].size();
+	const size_t C = b->n.size();
+	std::vector<size_t> off(C + 1, 0);
+	for(size_t c = 0; c < C; ++c) off[c + 1] = off[c] + params[c].size();
+	std::vector<ObsParams> obs(C);
+	for(size_t c = 0; c < C; ++c) obs[c] = make_observable(b->n[c], terms, nterms);
+	// Backward plans: per circuit the inverse gates in reverse order, each
+	// parameter's gradient reduction (TOP_GRAD) just before its gate is
+	// undone, as dual tiled passes over (psi, lambda) one bit narrower than
+	// the forward tiles.  Round r runs pass r of every circuit.
+	const unsigned Td = std::max(3u, b->T) - 1u;
+	const unsigned Wd = tile_stage_width(b->prec, Td, true);
+	std::vector<GatePlan> plans(C);
+	size_t R = 0;
+	for(size_t c = 0; c < C; ++c) {
+		if(params[c].empty()) continue;
+		std::vector<int64_t> slot(b->ops[c].size(), -1);
+		for(size_t p = 0; p < params[c].size(); ++p) slot[params[c][p]] = (int64_t)p;
+		std::vector<QOp> rev;
+		size_t keep = 0;
+		for(size_t i = b->ops[c].size(); i-- > 0;) {
+			const QOp &op = b->ops[c][i];
+			if(slot[i] >= 0) {
+				QOp g;
+				g.kind = Q_GRAD;
+				g.b0 = op.b0;
+				g.b1 = op.b1;
+				g.k = op.kind == Q_CP ? 2u : 1u;
+				g.state = (u64)slot[i];
+				rev.push_back(g);
+				keep = rev.size();
+			}
+			QOp inv = op;
+			if(op.kind == Q_CP || op.kind == Q_PH) {
+				inv.ci = -op.ci;
+				inv.theta = -op.theta;
+			}
+			rev.push_back(inv);
+		}
+		rev.resize(keep);  // nothing before the first parameter needs undoing
+		PlanCfg pc;
+		pc.prec = b->prec;
+		pc.dual = true;
+		pc.ladders = false;
+		pc.T = std::min(Td, b->n[c]);
+		pc.C = std::min(pc.T, b->prec == 0 ? 4u : 5u);
+		plans[c] = plan_gates(pc, b->n[c], rev, 0, rev.size(), std::vector<double>());
+		for(const PassLaunch &l : plans[c].launches)
+			if(l.kind != PassLaunch::TILE || l.ldsops || l.W != Wd)
+				throw std::invalid_argument("batched adjoint needs a pass the batch kernel does not run");
+		R = std::max(R, plans[c].launches.size());
 	}
-	destroy_handle(h);
-	h = nullptr;
+	struct DualRound {
+		size_t pass0, item0, nitems, grad0, ngrads;
+	};
+	std::vector<DualRound> rounds;
+	std::vector<BatchPass> passes;
+	std::vector<BatchItem> items;
+	std::vector<BatchGrad> gents;
+	std::vector<TileOp> tops;
+	std::vector<TileStage> tstages;
+	size_t max_items = 1;
+	for(size_t r = 0; r < R; ++r) {
+		DualRound rd{passes.size(), items.size(), 0, gents.size(), 0};
+		for(size_t c = 0; c < C; ++c) {
+			if(r >= plans[c].launches.size()) continue;
+			const PassLaunch &l = plans[c].launches[r];
+			BatchPass bp{};
+			char *A = b->dmem + b->offA[c], *B = b->dmem + b->offB[c];
+			bp.state = b->in_b[c] ? B : A;  // psi: where the forward run leaves the final state
+			bp.state2 = b->in_b[c] ? A : B;
+			bp.tp = l.tp;
+			bp.op0 = (uint32_t)tops.size();
+			bp.st0 = (uint32_t)tstages.size();
+			bp.nst = (uint32_t)l.stages.size();
+			bp.nslots = (uint32_t)l.gidx.size();
+			tops.insert(tops.end(), l.ops.begin(), l.ops.end());
+			tstages.insert(tstages.end(), l.stages.begin(), l.stages.end());
+			const uint32_t local = (uint32_t)(passes.size() - rd.pass0);
+			passes.push_back(bp);
+			const uint32_t first = (uint32_t)(items.size() - rd.item0);
+			const u64 per = 4;  // tiles per work item
+			for(u64 t0 = 0; t0 < l.tp.ntiles; t0 += per) {
+				BatchItem it{};
+				it.pass = local;
+				it.tile0 = t0;
+				it.ntiles = (uint32_t)std::min<u64>(per, l.tp.ntiles - t0);
+				items.push_back(it);
+			}
+			const uint32_t cnt = (uint32_t)(items.size() - rd.item0) - first;
+			for(size_t sl = 0; sl < l.gidx.size(); ++sl) {
+				BatchGrad g{};
+				g.item0 = first;
+				g.nitems = cnt;
+				g.slot = (uint32_t)sl;
+				g.gindex = off[c] + l.gidx[sl];
+				gents.push_back(g);
+			}
+		}
+		rd.nitems = items.size() - rd.item0;
+		rd.ngrads = gents.size() - rd.grad0;
+		max_items = std::max(max_items, rd.nitems);
+		rounds.push_back(rd);
+	}
+	// one device block: outputs, reduction partials, gradient partials, tables
+	auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+	const size_t so = al((total + C + 1) * sizeof(double)), sr = al(2 * kReduceBlocks * sizeof(double)),
+	             sg = al(max_items * kMaxGradSlots * sizeof(double)), sp = al(passes.size() * sizeof(BatchPass)),
+	             si = al(items.size() * sizeof(BatchItem)), se = al(gents.size() * sizeof(BatchGrad)),
+	             sops = al(tops.size() * sizeof(TileOp)), sst = al(tstages.size() * sizeof(TileStage));
+	HIPC(hipMalloc((void **)&dblock, so + sr + sg + sp + si + se + sops + sst));
+	double *d_out = (double *)dblock, *d_red = (double *)(dblock + so), *d_gpart = (double *)(dblock + so + sr);
+	char *tab = dblock + so + sr + sg;
+	std::vector<char> host(sp + si + se + sops + sst, 0);
+	size_t hoff = 0;
+	auto put = [&](const void *src, size_t bytes, size_t span) {
+		if(bytes) std::memcpy(host.data() + hoff, src, bytes);
+		char *d = tab + hoff;
+		hoff += span;
+		return d;
+	};
+	const BatchPass *d_passes = (const BatchPass *)put(passes.data(), passes.size() * sizeof(BatchPass), sp);
+	const BatchItem *d_items = (const BatchItem *)put(items.data(), items.size() * sizeof(BatchItem), si);
+	const BatchGrad *d_gents = (const BatchGrad *)put(gents.data(), gents.size() * sizeof(BatchGrad), se);
+	const TileOp *d_ops = (const TileOp *)put(tops.data(), tops.size() * sizeof(TileOp), sops);
+	const TileStage *d_st = (const TileStage *)put(tstages.data(), tstages.size() * sizeof(TileStage), sst);
+	if(!host.empty()) HIPC(hipMemcpyAsync(tab, host.data(), host.size(), hipMemcpyHostToDevice, b->stream));
+	// forward (the batch run), then lambda = O psi and E per circuit
+	if(b->exec)
+		HIPC(hipGraphLaunch(b->exec, b->stream));
+	else
+		batch_enqueue(b);
+	for(size_t c = 0; c < C; ++c) {
+		char *A = b->dmem + b->offA[c], *B = b->dmem + b->offB[c];
+		HIPC(launch_observable(b->L(), b->in_b[c] ? B : A, b->in_b[c] ? A : B, b->n[c], obs[c], d_red, d_out + total + c));
+	}
+	for(const DualRound &rd : rounds) {
+		HIPC(launch_batch_dual(b->L(), Wd, d_passes + rd.pass0, d_items + rd.item0, (unsigned)rd.nitems, d_ops, d_st, d_gpart,
+		                       2048));
+		HIPC(launch_batch_grad_finish(b->L(), d_gents + rd.grad0, (unsigned)rd.ngrads, d_gpart, -2.0, d_out));
+	}
+	if(total) HIPC(hipMemcpyAsync(grads, d_out, total * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+	HIPC(hipMemcpyAsync(expvals, d_out + total, C * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+	// leave every circuit's final state in place again, as after a batch run
+	if(b->exec)
+		HIPC(hipGraphLaunch(b->exec, b->stream));
+	else
+		batch_enqueue(b);
+	HIPC(hipStreamSynchronize(b->stream));
+	hipFree(dblock);
 	return TMFQS_HIP_OK;
 	}
 	catch(...) {
-		destroy_handle(h);
+		if(b && b->stream) hipStreamSynchronize(b->stream);
+		hipFree(dblock);
 		return map_exception();
 	}
 }

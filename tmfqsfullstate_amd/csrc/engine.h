@@ -194,11 +194,19 @@ hipError_t launch_chunk_ticks(const Launch &L, const void *a, unsigned n, const 
 __host__ __device__ constexpr unsigned batch_max_tile_bits(int prec) { return prec == 0 ? 12u : 13u; }
 struct BatchPass {
 	void *state;      // the circuit's state buffer
+	void *state2;     // dual passes: the second state (lambda)
 	TilePass tp;      // geometry of the pass
 	uint32_t op0;     // its ops in the batch op pool
 	uint32_t st0;     // its stages in the batch stage pool
 	uint32_t nst;
-	uint32_t pad;
+	uint32_t nslots;  // dual passes: gradient slots (TOP_GRAD ops)
+};
+// One gradient of a batched dual round: the sum over the pass's work items
+// [item0, item0 + nitems) of their partials for `slot`, scaled, goes to
+// grads[gindex].
+struct BatchGrad {
+	uint32_t item0, nitems, slot, pad;
+	u64 gindex;
 };
 struct BatchItem {
 	uint32_t pass;    // index into the round's pass table
@@ -220,5 +228,12 @@ hipError_t launch_batch_permute(const Launch &L, const BatchChunk *dchunks, unsi
 // One round: W is the register-stage width every pass of the round uses.
 hipError_t launch_batch_tiles(const Launch &L, unsigned W, const BatchPass *dpasses, const BatchItem *ditems, unsigned nitems,
                               const TileOp *dops, const TileStage *dstages, const double *dphase, unsigned grid);
+// One dual round (adjoint backward): every pass applies its ops to state and
+// state2 and writes per-item partials of its TOP_GRAD slots to
+// gpart[item * kMaxGradSlots + slot]; W is the round's register-stage width.
+hipError_t launch_batch_dual(const Launch &L, unsigned W, const BatchPass *dpasses, const BatchItem *ditems, unsigned nitems,
+                             const TileOp *dops, const TileStage *dstages, double *dgpart, unsigned grid);
+hipError_t launch_batch_grad_finish(const Launch &L, const BatchGrad *dg, unsigned count, const double *dgpart, double scale,
+                                    double *dgrads);
 
 }  // namespace tmfqs_hip

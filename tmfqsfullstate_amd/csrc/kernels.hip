@@ -1758,6 +1758,47 @@ __global__ void __launch_bounds__(256) k_batch_tiles(const BatchPass *__restrict
 	}
 }
 
+// Dual form of k_batch_tiles for the batched adjoint: each pass steps psi
+// (state) and lambda (state2) back together; a work item's gradient partials
+// (per wavefront, summed in a fixed order) go to its own gpart row.
+template <typename R, int W>
+__global__ void __launch_bounds__(256) k_batch_dual(const BatchPass *__restrict__ passes, const BatchItem *__restrict__ items,
+                                                    unsigned nitems, const TileOp *__restrict__ ops,
+                                                    const TileStage *__restrict__ stages, double *__restrict__ gpart) {
+	using V = typename V2T<R>::t;
+	extern __shared__ char smem[];
+	const unsigned nlocp = staged_tile_elems(batch_max_tile_bits(sizeof(R) == 8 ? 0 : 1) - 1, false);
+	V *ta = (V *)smem;
+	V *tb = ta + nlocp;
+	u64 *hitab = (u64 *)(tb + nlocp);
+	double *gacc = (double *)(hitab + (1u << kMaxHighBits));  // [4 waves][kMaxGradSlots]
+	for(unsigned it = blockIdx.x; it < nitems; it += gridDim.x) {
+		const BatchItem item = items[it];
+		const BatchPass &bp = passes[item.pass];
+		__syncthreads();  // the previous item is done with hitab and gacc
+		tile_hitab<R>(hitab, bp.tp);
+		for(unsigned i = threadIdx.x; i < 4 * kMaxGradSlots; i += blockDim.x) gacc[i] = 0.0;
+		for(u64 tid = item.tile0; tid < item.tile0 + item.ntiles; ++tid)
+			staged_tile<R, W, true, false>((V *)bp.state, (V *)bp.state2, bp.tp, tid, ops + bp.op0, stages + bp.st0, bp.nst, nullptr,
+			                               nullptr, nullptr, ta, tb, hitab, gacc);
+		__syncthreads();
+		for(unsigned s = threadIdx.x; s < bp.nslots; s += blockDim.x)
+			gpart[(u64)it * kMaxGradSlots + s] =
+			    ((gacc[s] + gacc[kMaxGradSlots + s]) + gacc[2 * kMaxGradSlots + s]) + gacc[3 * kMaxGradSlots + s];
+	}
+}
+
+__global__ void __launch_bounds__(256) k_batch_grad_finish(const BatchGrad *__restrict__ g, unsigned count,
+                                                           const double *__restrict__ gpart, double scale,
+                                                           double *__restrict__ grads) {
+	const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= count) return;
+	const BatchGrad e = g[i];
+	double v = 0.0;
+	for(unsigned k = 0; k < e.nitems; ++k) v += gpart[(u64)(e.item0 + k) * kMaxGradSlots + e.slot];
+	grads[e.gindex] = scale * v;
+}
+
 // Basis-state initialisation of every circuit (items: chunks of one state).
 template <typename R>
 __global__ void __launch_bounds__(256) k_batch_init(const BatchChunk *__restrict__ chunks, unsigned nchunks) {
@@ -2126,6 +2167,42 @@ static hipError_t launch_batch_tiles_t(const Launch &L, const BatchPass *dpasses
 	}
 	TMFQS_LAUNCH((k_batch_tiles<R, W>), grid, 256, shmem, L.stream, dpasses, ditems, nitems, dops, dstages,
 	             (const double2 *)dphase);
+	return hipGetLastError();
+}
+
+template <typename R, int W>
+static hipError_t launch_batch_dual_t(const Launch &L, const BatchPass *dpasses, const BatchItem *ditems, unsigned nitems,
+                                      const TileOp *dops, const TileStage *dstages, double *dgpart, unsigned grid) {
+	using V = typename V2T<R>::t;
+	const size_t shmem = 2 * sizeof(V) * staged_tile_elems(batch_max_tile_bits(L.prec) - 1, false) + (8ull << kMaxHighBits) +
+	                     4 * kMaxGradSlots * sizeof(double);
+	static bool attr = false;
+	if(!attr) {
+		(void)hipFuncSetAttribute((const void *)k_batch_dual<R, W>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+		attr = true;
+	}
+	TMFQS_LAUNCH((k_batch_dual<R, W>), grid, 256, shmem, L.stream, dpasses, ditems, nitems, dops, dstages, dgpart);
+	return hipGetLastError();
+}
+
+hipError_t launch_batch_dual(const Launch &L, unsigned W, const BatchPass *dpasses, const BatchItem *ditems, unsigned nitems,
+                             const TileOp *dops, const TileStage *dstages, double *dgpart, unsigned grid) {
+	if(!nitems) return hipSuccess;
+	if(grid > nitems) grid = nitems;
+	if(L.prec == 0) {
+		if(W == 3) return launch_batch_dual_t<double, 3>(L, dpasses, ditems, nitems, dops, dstages, dgpart, grid);
+		if(W == 2) return launch_batch_dual_t<double, 2>(L, dpasses, ditems, nitems, dops, dstages, dgpart, grid);
+	} else {
+		if(W == 3) return launch_batch_dual_t<float, 3>(L, dpasses, ditems, nitems, dops, dstages, dgpart, grid);
+		if(W == 2) return launch_batch_dual_t<float, 2>(L, dpasses, ditems, nitems, dops, dstages, dgpart, grid);
+	}
+	return hipErrorInvalidValue;
+}
+
+hipError_t launch_batch_grad_finish(const Launch &L, const BatchGrad *dg, unsigned count, const double *dgpart, double scale,
+                                    double *dgrads) {
+	if(!count) return hipSuccess;
+	TMFQS_LAUNCH(k_batch_grad_finish, (count + 255) / 256, 256, 0, L.stream, dg, count, dgpart, scale, dgrads);
 	return hipGetLastError();
 }
 
