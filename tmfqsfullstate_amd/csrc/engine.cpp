@@ -125,8 +125,6 @@ struct tmfqs_hip_state {
 	bool init = false;
 	void *amp = nullptr;
 	size_t amp_bytes = 0;
-	size_t amp_cap = 0;          // allocation behind amp (>= amp_bytes)
-	bool keep_capacity = false;  // reuse a larger allocation for a smaller state (batched adjoint)
 	void *scratch = nullptr;  // lazily allocated second state buffer
 	size_t scratch_bytes = 0;
 	hipStream_t stream = nullptr;
@@ -338,16 +336,14 @@ void allocate_state(tmfqs_hip_state *h, unsigned n) {
 	h->tracked.clear();
 	h->dirty.clear();
 	h->dirty_set.clear();
-	if(h->keep_capacity && h->amp && h->amp_cap >= bytes) {
-		h->amp_bytes = bytes;
-	} else if(h->amp_bytes != bytes) {
+	if(h->amp_bytes != bytes) {
 		HIPC(hipStreamSynchronize(h->stream));
 		if(h->amp) hipFree(h->amp);
 		h->amp = nullptr;
-		h->amp_bytes = h->amp_cap = 0;
+		h->amp_bytes = 0;
 		h->init = false;
 		HIPC(hipMalloc(&h->amp, bytes));
-		h->amp_bytes = h->amp_cap = bytes;
+		h->amp_bytes = bytes;
 		if(h->scratch) {
 			hipFree(h->scratch);
 			h->scratch = nullptr;
@@ -1439,7 +1435,7 @@ void adjoint_enqueue(tmfqs_hip_state *h, std::vector<QOp> &qops, const std::vect
 	}
 	const size_t S = bounds.size() - 1;
 	const size_t nbuf = S + 1;
-	if(h->adj_buf_bytes < h->amp_bytes || (!h->keep_capacity && h->adj_buf_bytes != h->amp_bytes)) {
+	if(h->adj_buf_bytes != h->amp_bytes) {
 		HIPC(hipStreamSynchronize(h->stream));
 		for(void *p : h->adj_bufs) hipFree(p);
 		h->adj_bufs.clear();
