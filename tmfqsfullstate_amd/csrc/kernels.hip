@@ -716,13 +716,11 @@ template <typename V> __device__ __forceinline__ void butterfly(V &x, V &y) {
 __device__ __forceinline__ double2 csqr(double2 p) { return make_double2(fma(p.x, p.x, -p.y * p.y), 2.0 * p.x * p.y); }
 
 // Waves per SIMD the QFT block kernel is compiled for, and whether a
-// workgroup prefetches its next tile during stage B.  complex128 with
-// 32-amplitude rows fills the LDS with one workgroup per CU and prefetches;
-// complex128 with 16-amplitude rows and complex64 with 32 run two workgroups
-// per CU, and complex64 keeps to 128 registers by not prefetching (the other
-// workgroup's loads overlap instead).
+// workgroup prefetches its next tile during stage B.  complex128 (two waves
+// per SIMD) prefetches; complex64 keeps to 128 registers (four waves per
+// SIMD) by not prefetching, and other workgroups' loads overlap instead.
 template <typename R, int C> struct QftBlockWaves {
-	static constexpr int value = (sizeof(R) == 8 && C == 4) ? 2 : (sizeof(R) == 4 && C == 5) ? 4 : 1;
+	static constexpr int value = sizeof(R) == 8 ? 2 : 4;
 	static constexpr bool prefetch = sizeof(R) == 8;
 };
 
@@ -864,7 +862,11 @@ static hipError_t launch_qft_block_t(const Launch &L, void *a, unsigned n, unsig
 	P.pad = 0;
 	P.scale = std::ldexp(K & 1u ? 1.0 / std::sqrt(2.0) : 1.0, -(int)(K / 2));  // (1/sqrt(2))^K, one rounding at most
 	P.ntiles = 1ull << (n - 8 - C);
-	const unsigned grid = (unsigned)(P.ntiles < (1u << 20) ? P.ntiles : (1u << 20));
+	// complex128: a grid of 1024 workgroups that loop over their tiles, so each
+	// next tile's loads overlap stage B (3% faster at n = 30 than one tile per
+	// workgroup); complex64: one tile per workgroup, overlapping across them.
+	const u64 cap = QftBlockWaves<R, C>::prefetch ? 1024 : (1u << 20);
+	const unsigned grid = (unsigned)(P.ntiles < cap ? P.ntiles : cap);
 	const size_t shmem = (size_t)((256u << C) + 16u) * sizeof(V);
 	static bool attr = false;
 	if(!attr) {
