@@ -1604,11 +1604,29 @@ __device__ __forceinline__ void staged_tile(typename V2T<R>::t *__restrict__ a, 
 	u64 tbase = tid << C;
 	for(unsigned bb = 0; bb < P.nh; ++bb) tbase = ins0(tbase, P.hbits[bb]);
 	__syncthreads();
-	for(unsigned l = threadIdx.x; l < nloc; l += blockDim.x) {
-		const u64 g = tbase | (l & lowmask) | hitab[l >> C];
-		const unsigned pl = PAD ? l + (l >> 4) : l;
-		ta[pl] = nt_load(&a[g]);
-		if(DUAL) tb[pl] = nt_load(&b[g]);
+	// Loads in groups of TU per thread, all issued before the first is used,
+	// so a thread has TU (dual: 2 TU) loads in flight instead of one.
+	constexpr unsigned TU = DUAL ? 4u : 8u;
+	for(unsigned l0 = threadIdx.x; l0 < nloc; l0 += blockDim.x * TU) {
+		V xa[TU], xb[TU];
+#pragma unroll
+		for(unsigned u = 0; u < TU; ++u) {
+			const unsigned l = l0 + u * blockDim.x;
+			if(l < nloc) {
+				const u64 g = tbase | (l & lowmask) | hitab[l >> C];
+				xa[u] = nt_load(&a[g]);
+				if(DUAL) xb[u] = nt_load(&b[g]);
+			}
+		}
+#pragma unroll
+		for(unsigned u = 0; u < TU; ++u) {
+			const unsigned l = l0 + u * blockDim.x;
+			if(l < nloc) {
+				const unsigned pl = PAD ? l + (l >> 4) : l;
+				ta[pl] = xa[u];
+				if(DUAL) tb[pl] = xb[u];
+			}
+		}
 	}
 	__syncthreads();
 	for(unsigned s = 0; s < nstages; ++s) {
@@ -1695,11 +1713,17 @@ __device__ __forceinline__ void staged_tile(typename V2T<R>::t *__restrict__ a, 
 		}
 		__syncthreads();
 	}
-	for(unsigned l = threadIdx.x; l < nloc; l += blockDim.x) {
-		const u64 g = tbase | (l & lowmask) | hitab[l >> C];
-		const unsigned pl = PAD ? l + (l >> 4) : l;
-		nt_store(ta[pl], &a[g]);
-		if(DUAL) nt_store(tb[pl], &b[g]);
+	for(unsigned l0 = threadIdx.x; l0 < nloc; l0 += blockDim.x * TU) {
+#pragma unroll
+		for(unsigned u = 0; u < TU; ++u) {
+			const unsigned l = l0 + u * blockDim.x;
+			if(l < nloc) {
+				const u64 g = tbase | (l & lowmask) | hitab[l >> C];
+				const unsigned pl = PAD ? l + (l >> 4) : l;
+				nt_store(ta[pl], &a[g]);
+				if(DUAL) nt_store(tb[pl], &b[g]);
+			}
+		}
 	}
 }
 
