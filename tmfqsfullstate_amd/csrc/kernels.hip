@@ -1606,7 +1606,7 @@ __device__ __forceinline__ void staged_tile(typename V2T<R>::t *__restrict__ a, 
 	__syncthreads();
 	// Loads in groups of TU per thread, all issued before the first is used,
 	// so a thread has TU (dual: 2 TU) loads in flight instead of one.
-	constexpr unsigned TU = DUAL ? 4u : 8u;
+	constexpr unsigned TU = DUAL ? 8u : 16u;
 	for(unsigned l0 = threadIdx.x; l0 < nloc; l0 += blockDim.x * TU) {
 		V xa[TU], xb[TU];
 #pragma unroll
