@@ -1840,16 +1840,43 @@ __global__ void __launch_bounds__(256) k_batch_init(const BatchChunk *__restrict
 // Final bit layout of every circuit whose swaps were relabeled:
 // dst[x] = src[y], bit map[b] of y = bit b of x.
 template <typename R>
+// The source index is assembled from per-byte deposit tables built in LDS
+// for each chunk: y = tab[0][x & 255] | tab[1][(x >> 8) & 255] | ...
 __global__ void __launch_bounds__(256) k_batch_permute(const BatchChunk *__restrict__ chunks, unsigned nchunks) {
 	using V = typename V2T<R>::t;
+	__shared__ u64 tab[5][256];
+	__shared__ uint8_t map[64];
 	for(unsigned c = blockIdx.x; c < nchunks; c += gridDim.x) {
-		const BatchChunk ch = chunks[c];
+		const BatchChunk &ch = chunks[c];
+		const unsigned n = ch.n;
+		__syncthreads();  // the previous chunk is done with the tables
+		if(threadIdx.x < 64) map[threadIdx.x] = threadIdx.x < n ? ch.map[threadIdx.x] : 0;
+		__syncthreads();
+		for(unsigned t = threadIdx.x; t < 5 * 256; t += blockDim.x) {
+			const unsigned j = t >> 8, v = t & 255u;
+			u64 d = 0;
+			for(unsigned b = 0; b < 8; ++b)
+				if(((v >> b) & 1u) && 8 * j + b < n) d |= 1ull << map[8 * j + b];
+			tab[j][v] = d;
+		}
+		__syncthreads();
 		const V *src = (const V *)ch.src;
 		V *dst = (V *)ch.dst;
-		for(u64 x = ch.first + threadIdx.x; x < ch.first + ch.count; x += blockDim.x) {
-			u64 y = 0;
-			for(unsigned b = 0; b < ch.n; ++b) y |= ((x >> b) & 1ull) << ch.map[b];
-			dst[x] = src[y];
+		const u64 end = ch.first + ch.count;
+		auto gather = [&](u64 x) {
+			return src[tab[0][x & 255u] | tab[1][(x >> 8) & 255u] | tab[2][(x >> 16) & 255u] | tab[3][(x >> 24) & 255u] |
+			           tab[4][(x >> 32) & 255u]];
+		};
+		for(u64 x0 = ch.first + threadIdx.x; x0 < end; x0 += 8 * blockDim.x) {
+			if(x0 + 7 * blockDim.x < end) {  // eight gathers in flight
+				V v[8];
+#pragma unroll
+				for(unsigned u = 0; u < 8; ++u) v[u] = gather(x0 + u * blockDim.x);
+#pragma unroll
+				for(unsigned u = 0; u < 8; ++u) dst[x0 + u * blockDim.x] = v[u];
+			} else {
+				for(u64 x = x0; x < end; x += blockDim.x) dst[x] = gather(x);
+			}
 		}
 	}
 }
