@@ -159,10 +159,10 @@ def time_adjoint(T, S, precision, n=24, layers=8, seed=3, steps=3):
     return out, g
 
 
-def time_grover(T, S, n, marked, precision):
+def time_grover(T, S, n, marked, precision, materialized=True):
     """makeGrover(n, marked, materialized) through Simulator::run's timed
     region from QuantumState::basis(n); returns (register, report)."""
-    ops, cnt = S.grover_ops(n, marked, True)
+    ops, cnt = S.grover_ops(n, marked, materialized)
     reg = S.Register.basis(n, 0, precision=precision)
     hs = reg.hip_state()
     hs.profile(True)
@@ -181,6 +181,7 @@ def time_grover(T, S, n, marked, precision):
         "qubits": n,
         "marked": list(marked),
         "precision": "complex128" if precision == T.C128 else "complex64",
+        "diffusion": "materialized" if materialized else "lazy",
         "iterations": iters,
         "wall_s": secs,
         "iterations_per_s": iters / secs,
@@ -505,6 +506,12 @@ def main():
         g20, rep20 = time_grover(T, S, 20, [524287], T.C128)
         result["grover20"] = rep20
         del g20
+        # lazy diffusion (the affine overlay): no full-state sweep per iteration
+        glz, replz = time_grover(T, S, n, [(1 << (n - 1)) - 1], T.C128, materialized=False)
+        result["grover_lazy"] = replz
+        log(f"Grover-{n} lazy: {replz['wall_s']:.3f} s, P={replz['p_marked']:.15f}, "
+            f"measured {replz['measured_seed_123456']}")
+        del glz
 
     if args.qft32:
         result["qft32_complex64"] = time_qft32(T, S)
