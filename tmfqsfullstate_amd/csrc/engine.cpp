@@ -425,11 +425,13 @@ void sync_all(tmfqs_hip_state *h) {
 }
 
 // Default most elementwise steps per flush: complex128 replays each step's
-// rounding (bit-identical to the dense backend), so a sweep carries 96 steps;
-// complex64 composes a whole run into one affine map.
+// rounding (bit-identical to the dense backend), so a sweep carries 256 steps
+// (two FP64 adds per amplitude and step: past ~100 steps the sweep is bound by
+// FP64 issue, not HBM, and longer sweeps only save passes); complex64
+// composes a whole run into one affine map.
 unsigned ew_block(const tmfqs_hip_state *h) {
 	if(h->cfg.grover_block > 0) return (unsigned)h->cfg.grover_block;
-	return h->prec == 0 ? 96u : 65536u;
+	return h->prec == 0 ? 256u : 65536u;
 }
 
 void exec_elementwise(tmfqs_hip_state *h, const std::vector<QOp> &ops, size_t lo, size_t hi) {
