@@ -48,6 +48,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "QFT-30/Grover-30 wall s + HBM GB/s; adjoint grads/s vs plain FD at ≤1e-9 err"
 HBM_PEAK_GBPS = 8000.0
+FP64_VECTOR_PEAK_ADDS = 39.3e12  # MI355X FP64 vector 78.6 TFLOP/s counts an FMA as two flops
 
 
 def log(*a):
@@ -481,9 +482,17 @@ def main():
             "hbm_GBps_over_kernel_time": gbytes / (gms * 1e-3) / 1e9 if gms > 0 else None,
             "hbm_GBps_over_wall": gbytes / gsecs / 1e9,
             "one_sweep_per_iteration_equivalent_GBps_not_hbm": iters * 2.0 * 16 * N / gsecs / 1e9,
+            # each diffusion step is two FP64 adds per amplitude: the sweeps'
+            # bound once a sweep carries more than ~100 steps
+            "diffusion_fp64_adds_per_s": (iters * 2.0 * N / (gprof["k_diffuse_chain"]["ms"] * 1e-3)
+                                          if "k_diffuse_chain" in gprof else None),
+            "fp64_vector_peak_adds_per_s": FP64_VECTOR_PEAK_ADDS,
             "kernels": gprof,
         }
-        log(f"Grover-{n}: {gsecs:.3f} s, P={abs(a) ** 2:.15f}, measured {meas}")
+        if result["grover"]["diffusion_fp64_adds_per_s"]:
+            result["grover"]["fraction_of_fp64_vector_peak"] = result["grover"]["diffusion_fp64_adds_per_s"] / FP64_VECTOR_PEAK_ADDS
+        log(f"Grover-{n}: {gsecs:.3f} s, P={abs(a) ** 2:.15f}, measured {meas}, diffusion sweeps at "
+            f"{result['grover'].get('fraction_of_fp64_vector_peak', 0):.0%} of the FP64 add rate")
         del greg, gh
 
     # ---- Grover-n with three marked states, complex64 against complex128
