@@ -338,10 +338,15 @@ def cpu_baseline(n):
             if rc:
                 raise RuntimeError(f"orc_time_ops failed {rc}")
             return s.value
+    # about ten seconds of single-thread work at n = 30: operations of
+    # makeQft(n) on qubits spread over the register
+    cp_pairs = [(1, 0), (5, 2), (9, 4), (13, 6), (17, 8), (21, 10), (25, 12), (29, 14), (29, 28), (20, 19), (27, 1),
+                (15, 3)]
     samples = {
-        "h": [(O.OP_H, 0)],
-        "cphase": [(O.OP_CPHASE, 1, 0, math.pi / 2), (O.OP_CPHASE, 29 % n, 3, math.pi / 2 ** 26)],
-        "swap": [(O.OP_SWAP, 0, n - 1)],
+        "h": [(O.OP_H, q % n) for q in (0, 10, 20, 29)],
+        "cphase": [(O.OP_CPHASE, c % n, t % n, math.pi / 2 ** ((c % n) - (t % n))) for c, t in cp_pairs
+                   if c % n > t % n],
+        "swap": [(O.OP_SWAP, a % n, b % n) for a, b in ((0, 29), (5, 24), (14, 15)) if a % n != b % n],
     }
     per = {}
     total_s = 0.0
@@ -358,7 +363,8 @@ def cpu_baseline(n):
         "kind": kind,
         "projected_qft_seconds": projected,
         "per_op_seconds": per,
-        "sample": (f"1 H, 2 controlled-phase and 1 swap (3 CX) of makeQft({n}) timed on a 2^{n}-state "
+        "sample": (f"{len(samples['h'])} H, {len(samples['cphase'])} controlled-phase and {len(samples['swap'])} swap "
+                   f"(3 CX each) operations of makeQft({n}) timed on a 2^{n}-state "
                    f"complex128 register with uniform amplitudes ({total_s:.1f} s of CPU work), extrapolated "
                    f"to the circuit's {nh} H + {ncp} controlled-phase + {nsw} swap operations; single thread, "
                    f"{os.cpu_count()} host CPUs, {cpu_model()}"),
