@@ -997,6 +997,26 @@ GatePlan plan_gates(const PlanCfg &cfg, unsigned n, const std::vector<QOp> &in, 
 					t.mat = (unsigned)mats.size();
 					const size_t dim = 1ull << op.k;
 					mats.insert(mats.end(), qmats.begin() + op.mat, qmats.begin() + op.mat + 2 * dim * dim);
+					if(cfg.prec == 1 && op.k == 5) {
+						// complex64 five-qubit blocks run on FP32 MFMA: the A operands of
+						// every lane, already in FP32 and in MFMA order (see tile_mfma5),
+						// 16 B aligned after the FP64 matrix.
+						if(mats.size() & 1) mats.push_back(0.0);
+						t.twid = (unsigned)mats.size();
+						std::vector<float> af(2 * 64 * 32);
+						const double *g = qmats.data() + op.mat;
+						for(unsigned half = 0; half < 2; ++half)
+							for(unsigned lane = 0; lane < 64; ++lane)
+								for(unsigned s2 = 0; s2 < 32; ++s2) {
+									const unsigned rr = lane & 31u, k2 = 2u * s2 + (lane >> 5), kk = k2 & 31u;
+									const bool right = k2 >= 32u;
+									const double re = g[2 * (rr * 32 + kk)], im = g[2 * (rr * 32 + kk) + 1];
+									af[(half * 64 + lane) * 32 + s2] = (float)(half ? (right ? re : im) : (right ? -im : re));
+								}
+						const size_t at = mats.size();
+						mats.resize(at + af.size() / 2);
+						std::memcpy(mats.data() + at, af.data(), af.size() * sizeof(float));
+					}
 					tops.push_back(t);
 					++k;
 					break;

@@ -1537,16 +1537,18 @@ __device__ void tile_mfma5(float2 *tile, const TileOp &op, unsigned T, const dou
 				sorted[i] = sorted[j];
 				sorted[j] = t;
 			}
-	// this lane's A operands: Mbig[32 half + lane % 32][2 s + lane / 32], s = 0..31
-	const double *g = mats + op.mat;
+	// this lane's A operands Mbig[32 half + lane % 32][2 s + lane / 32], s = 0..31,
+	// prepared by the planner in FP32 after the op's matrix (at op.twid)
 	const unsigned rr = lane & 31u, kh = lane >> 5;
+	const float4 *a4 = (const float4 *)(mats + op.twid) + (half * 64u + lane) * 8u;
 	float A[32];
 #pragma unroll
-	for(int s2 = 0; s2 < 32; ++s2) {
-		const unsigned k = 2u * (unsigned)s2 + kh, kk = k & 31u;
-		const bool right = k >= 32u;
-		const double re = g[2 * (rr * 32 + kk)], im = g[2 * (rr * 32 + kk) + 1];
-		A[s2] = (float)(half ? (right ? re : im) : (right ? -im : re));
+	for(int i = 0; i < 8; ++i) {
+		const float4 v = a4[i];
+		A[4 * i] = v.x;
+		A[4 * i + 1] = v.y;
+		A[4 * i + 2] = v.z;
+		A[4 * i + 3] = v.w;
 	}
 	const unsigned ncol = 1u << (T - 10);  // column tiles of 32 blocks
 	mfma_f16v acc[4];
