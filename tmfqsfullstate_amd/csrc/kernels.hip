@@ -1560,11 +1560,13 @@ __device__ void tile_mfma5(float2 *tile, const TileOp &op, unsigned T, const dou
 		unsigned base = ct * 32u + rr;  // block index of this lane's B column
 #pragma unroll
 		for(int i = 0; i < 5; ++i) base = ((base >> sorted[i]) << (sorted[i] + 1)) | (base & ((1u << sorted[i]) - 1u));
+		// steps s and s + 16 take the real and the imaginary part of the same
+		// amplitude (k and k + 32): one LDS read serves both
 #pragma unroll
-		for(int s2 = 0; s2 < 32; ++s2) {
-			const unsigned k = 2u * (unsigned)s2 + kh;
-			const float2 x = tile[base | block_offset(k & 31u, jl)];
-			acc[u] = __builtin_amdgcn_mfma_f32_32x32x2f32(A[s2], k >= 32u ? x.y : x.x, acc[u], 0, 0, 0);
+		for(int s2 = 0; s2 < 16; ++s2) {
+			const float2 x = tile[base | block_offset(2u * (unsigned)s2 + kh, jl)];
+			acc[u] = __builtin_amdgcn_mfma_f32_32x32x2f32(A[s2], x.x, acc[u], 0, 0, 0);
+			acc[u] = __builtin_amdgcn_mfma_f32_32x32x2f32(A[s2 + 16], x.y, acc[u], 0, 0, 0);
 		}
 	}
 	__syncthreads();  // every wavefront has read its columns
