@@ -373,3 +373,29 @@ def test_qft_large_vs_fft():
     got = s.amplitudes()
     want = np.fft.ifft(cplx(init), norm="ortho")
     assert np.abs(got - want).max() <= TOL
+
+
+@pytest.mark.parametrize("precision,n", [(T.C128, 31), (T.C64, 32)], ids=["c128-n31", "c64-n32"])
+def test_qft_largest_sizes_basis_closed_form(precision, n):
+    # The largest QFTs a test can afford (32 GiB states), checked against the
+    # closed form QFT|s> = sum_y e^{2 pi i s y / N} |y> / sqrt(N) on three
+    # slices: n = 31 starts with a one-target partial pass, n = 32 with a
+    # two-target one, and the tail exchanges 2^(n-12) tiles.
+    from tmfqsfullstate_amd import sim as S
+
+    N = 1 << n
+    src = 0x2468ACE1 % N
+    print(f"QFT-{n} of |{src}>", flush=True)
+    with T.HipState(precision=precision) as s:
+        s.init_basis(n, src)
+        ops, cnt = S.qft_ops(n)
+        s.run(ops, cnt)
+        s.synchronize()
+        assert abs(s.total_probability() - 1.0) < (1e-12 if precision == T.C128 else 1e-5)
+        for first in (0, N // 2 - 1000, N - 4096):
+            got = cplx(s.export(first, 4096))
+            y = np.arange(first, first + 4096, dtype=np.uint64)
+            ph = (np.uint64(src) * y) % np.uint64(N)  # exact: src * y < 2^64
+            want = np.exp(2j * np.pi * (ph.astype(np.float64) / N)) / np.sqrt(N)
+            err = np.abs(got - want).max() * np.sqrt(N)  # relative to the amplitude size
+            assert err <= (1e-10 if precision == T.C128 else 1e-5), (first, err)
