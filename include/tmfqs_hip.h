@@ -301,8 +301,10 @@ int tmfqs_hip_batch_info(tmfqs_hip_batch *b, tmfqs_hip_batch_stats *out);
  * run gives the final states, then round r steps pass r of every circuit's
  * backward sweep (psi and lambda = O psi in the circuit's two buffers,
  * without checkpoints) in one work-list kernel, with per-item gradient
- * partials summed in a fixed order.  A second batch run leaves every final
- * state in place again, as after tmfqs_hip_batch_run. */
+ * partials summed in a fixed order.  Every final state is in place afterwards,
+ * as after tmfqs_hip_batch_run: the sweep steps copies of them (one more
+ * buffer per circuit while it runs) when the device has room, else the
+ * states themselves followed by a second batch run. */
 int tmfqs_hip_batch_adjoint(tmfqs_hip_batch *b, const tmfqs_hip_diag_term *terms, size_t nterms, double *expvals, double *grads,
                             size_t grad_count);
 int tmfqs_hip_batch_destroy(tmfqs_hip_batch *b);

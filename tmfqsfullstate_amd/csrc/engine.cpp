@@ -2760,7 +2760,7 @@ int tmfqs_hip_batch_info(tmfqs_hip_batch *b, tmfqs_hip_batch_stats *out) {
 
 int tmfqs_hip_batch_adjoint(tmfqs_hip_batch *b, const tmfqs_hip_diag_term *terms, size_t nterms, double *expvals, double *grads,
                             size_t grad_count) {
-	char *dblock = nullptr;
+	char *dblock = nullptr, *dpsi = nullptr;
 	ABI_BEGIN
 	if(!b) throw std::invalid_argument("null tmfqs_hip_batch handle");
 	if(!expvals) throw std::invalid_argument("tmfqs_hip_batch_adjoint: null expectation output");
@@ -2786,6 +2786,23 @@ int tmfqs_hip_batch_adjoint(tmfqs_hip_batch *b, const tmfqs_hip_diag_term *terms
 	// parameter's gradient reduction (TOP_GRAD) just before its gate is
 	// undone, as dual tiled passes over (psi, lambda) one bit narrower than
 	// the forward tiles.  Round r runs pass r of every circuit.
+	// One more buffer per circuit, when the device has room, takes a copy of
+	// the final state for the backward sweep; otherwise psi is stepped back in
+	// place and a second batch run restores the final states.
+	std::vector<size_t> poff(C, 0);
+	size_t psi_bytes = 0;
+	for(size_t c = 0; c < C; ++c) {
+		poff[c] = psi_bytes;
+		psi_bytes += ((((size_t)1 << b->n[c]) * b->esz) + 255) & ~size_t(255);
+	}
+	{
+		size_t freeb = 0, totalb = 0;
+		HIPC(hipMemGetInfo(&freeb, &totalb));
+		if(psi_bytes + (256u << 20) < freeb && hipMalloc((void **)&dpsi, std::max<size_t>(psi_bytes, 256)) != hipSuccess) {
+			(void)hipGetLastError();
+			dpsi = nullptr;
+		}
+	}
 	const unsigned Td = std::max(3u, b->T) - 1u;
 	const unsigned Wd = tile_stage_width(b->prec, Td, true);
 	std::vector<GatePlan> plans(C);
@@ -2845,7 +2862,8 @@ int tmfqs_hip_batch_adjoint(tmfqs_hip_batch *b, const tmfqs_hip_diag_term *terms
 			const PassLaunch &l = plans[c].launches[r];
 			BatchPass bp{};
 			char *A = b->dmem + b->offA[c], *B = b->dmem + b->offB[c];
-			bp.state = b->in_b[c] ? B : A;  // psi: where the forward run leaves the final state
+			// psi: the copy, or the final state itself; lambda: the circuit's other buffer
+			bp.state = dpsi ? dpsi + poff[c] : (b->in_b[c] ? B : A);
 			bp.state2 = b->in_b[c] ? A : B;
 			bp.tp = l.tp;
 			bp.op0 = (uint32_t)tops.size();
@@ -2912,6 +2930,11 @@ int tmfqs_hip_batch_adjoint(tmfqs_hip_batch *b, const tmfqs_hip_diag_term *terms
 		char *A = b->dmem + b->offA[c], *B = b->dmem + b->offB[c];
 		HIPC(launch_observable(b->L(), b->in_b[c] ? B : A, b->in_b[c] ? A : B, b->n[c], obs[c], d_red, d_out + total + c));
 	}
+	if(dpsi)  // the backward sweep steps copies of the final states; the states stay in place
+		for(size_t c = 0; c < C; ++c) {
+			const char *fin = b->dmem + (b->in_b[c] ? b->offB[c] : b->offA[c]);
+			HIPC(hipMemcpyAsync(dpsi + poff[c], fin, ((size_t)1 << b->n[c]) * b->esz, hipMemcpyDeviceToDevice, b->stream));
+		}
 	for(const DualRound &rd : rounds) {
 		HIPC(launch_batch_dual(b->L(), Wd, d_passes + rd.pass0, d_items + rd.item0, (unsigned)rd.nitems, d_ops, d_st, d_gpart,
 		                       2048));
@@ -2919,18 +2942,21 @@ int tmfqs_hip_batch_adjoint(tmfqs_hip_batch *b, const tmfqs_hip_diag_term *terms
 	}
 	if(total) HIPC(hipMemcpyAsync(grads, d_out, total * sizeof(double), hipMemcpyDeviceToHost, b->stream));
 	HIPC(hipMemcpyAsync(expvals, d_out + total, C * sizeof(double), hipMemcpyDeviceToHost, b->stream));
-	// leave every circuit's final state in place again, as after a batch run
-	if(b->exec)
-		HIPC(hipGraphLaunch(b->exec, b->stream));
-	else
-		batch_enqueue(b);
+	if(!dpsi) {  // psi was stepped back in place: leave every final state in place again
+		if(b->exec)
+			HIPC(hipGraphLaunch(b->exec, b->stream));
+		else
+			batch_enqueue(b);
+	}
 	HIPC(hipStreamSynchronize(b->stream));
 	hipFree(dblock);
+	hipFree(dpsi);
 	return TMFQS_HIP_OK;
 	}
 	catch(...) {
 		if(b && b->stream) hipStreamSynchronize(b->stream);
 		hipFree(dblock);
+		hipFree(dpsi);
 		return map_exception();
 	}
 }
