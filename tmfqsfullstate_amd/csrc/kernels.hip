@@ -426,14 +426,22 @@ __global__ void __launch_bounds__(256) k_norm(const typename V2T<R>::t *__restri
 	if(threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// Fixed-order sum of the partials (count <= kReduceBlocks), one block of 64
+// lanes: lane t sums partials t, t + 64, ... in order, then lane 0 sums the 64
+// lane results in order, so every run rounds the same way.
 __global__ void k_finish(const double *part, unsigned count, unsigned width, double *out) {
-	// one thread: fixed-order sum of the partials (count <= kReduceBlocks)
-	if(threadIdx.x == 0 && blockIdx.x == 0) {
-		for(unsigned w = 0; w < width; ++w) {
-			double s = 0;
-			for(unsigned i = 0; i < count; ++i) s += part[width * i + w];
-			out[w] = s;
+	__shared__ double sh[64];
+	for(unsigned w = 0; w < width; ++w) {
+		double s = 0;
+		for(unsigned i = threadIdx.x; i < count; i += 64) s += part[width * i + w];
+		sh[threadIdx.x] = s;
+		__syncthreads();
+		if(threadIdx.x == 0) {
+			double t = 0;
+			for(unsigned l = 0; l < 64; ++l) t += sh[l];
+			out[w] = t;
 		}
+		__syncthreads();
 	}
 }
 
