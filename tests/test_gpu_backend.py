@@ -1,6 +1,6 @@
 """GPU parity of the C-ABI backend against the CPU oracle.
 
-Tolerances (BASELINE north star): complex128 max-abs amplitude error <= 1e-12
+Tolerances: complex128 max-abs amplitude error <= 1e-12
 against the reference dense backend; per-operation sweeps and every fused
 operation except the QFT phase-ladder fusion reproduce it bit for bit
 (np.array_equal); measured indices are bit-exact.  complex64: fidelity error
