@@ -2116,8 +2116,11 @@ hipError_t launch_chunk_ticks(const Launch &L, const void *a, unsigned n, const 
 }
 
 unsigned tile_stage_width(int prec, unsigned T, bool dual) {
-	(void)prec;
-	const unsigned w = dual ? 3u : 4u;  // 16 (8 per state when dual) amplitudes per thread
+	// 16 amplitudes per thread (8 per state when dual).  Tiles one bit narrower
+	// than the default leave LDS for four workgroups per CU, which only
+	// three-bit stages (fewer registers) can use: there, 8 per thread.
+	const unsigned def = prec == 0 ? 12u : 13u;
+	const unsigned w = dual ? 3u : (T >= def ? 4u : 3u);
 	return T < w ? T : w;
 }
 
