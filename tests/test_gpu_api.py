@@ -162,3 +162,35 @@ def test_config_grover20_bitexact():
     assert abs(p - 0.999999756965361) < 1e-12
     draw = O.mt_unit_draws(123456, 1)[0]
     assert reg.measure_seed(123456) == ref.measure(draw) == marked
+
+
+def test_trace_report(tmp_path, monkeypatch):
+    # TMFQS_HIP_TRACE=<file>: each handle appends one JSON line of its kernels
+    # when it is destroyed.
+    import json
+
+    path = tmp_path / "trace.jsonl"
+    monkeypatch.setenv("TMFQS_HIP_TRACE", str(path))
+    n = 16
+    ops, cnt = S.qft_ops(n)
+    print("trace QFT-16", flush=True)
+    s = T.HipState()
+    s.init_basis(n, 5)
+    s.run(ops, cnt)
+    got = cplx(s.export())
+    s.close()
+    monkeypatch.delenv("TMFQS_HIP_TRACE")
+    want = np.fft.ifft(np.eye(1 << n)[5]) * np.sqrt(1 << n)
+    assert np.max(np.abs(got - want)) < TOL
+    lines = path.read_text().splitlines()
+    assert len(lines) == 1
+    rec = json.loads(lines[0])
+    assert rec["qubits"] == n and rec["precision"] == "complex128"
+    names = {k["name"]: k for k in rec["kernels"]}
+    assert "k_qft_tail_reverse" in names
+    amp_bytes = 16 << n
+    for k in rec["kernels"]:
+        assert k["launches"] >= 1 and k["ms"] > 0 and k["bytes"] > 0
+        assert k["fraction_of_hbm_peak"] == pytest.approx(k["gb_per_s"] / 8000, rel=1e-3, abs=1e-5)
+    assert names["k_qft_tail_reverse"]["bytes"] == 2 * amp_bytes
+    assert rec["total_ms"] == pytest.approx(sum(k["ms"] for k in rec["kernels"]), rel=1e-5)

@@ -109,6 +109,22 @@ def test_tuned_batch_stays_bitexact():
             assert np.array_equal(b.export(i), sequential(n, ops, cnt))
 
 
+def test_tune_cache_round_trip(tmp_path):
+    circuits = random_circuits(4, 10, 10, 16)
+    path = str(tmp_path / "tune.json")
+    print("tune and store", flush=True)
+    with Batch(circuits, tune_cache=path) as b:
+        assert b.cached_tile is None
+        t = b.tune()
+    print("reuse", t, flush=True)
+    with Batch(circuits, tune_cache=path) as b:
+        assert b.cached_tile == t and b.info()["tile_qubits"] == t
+        assert b.tune() == t
+        b.run()
+        for i, (n, ops, cnt) in enumerate(circuits):
+            assert np.array_equal(b.export(i), sequential(n, ops, cnt))
+
+
 def test_complex64_batch_fidelity():
     circuits = random_circuits(4, 12, 8, 16)
     print("complex64 batch", flush=True)

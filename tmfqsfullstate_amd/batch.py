@@ -9,6 +9,9 @@ significant bit of the basis index).
 """
 
 import ctypes
+import hashlib
+import json
+import os
 
 import numpy as np
 
@@ -77,6 +80,42 @@ def load_circuit_list(path):
     return circuits
 
 
+def tune_key(circuits, precision):
+    """Key of a tuned tile width: the precision and every circuit's qubit count
+    and gate-kind counts (the shape the tile choice depends on, not the
+    angles or the qubits drawn)."""
+    h = hashlib.sha1(f"precision={int(precision)}".encode())
+    for n, ops, count in circuits:
+        kinds = {}
+        for i in range(count):
+            kinds[ops[i].kind] = kinds.get(ops[i].kind, 0) + 1
+        h.update(f";{n}:{sorted(kinds.items())}".encode())
+    return h.hexdigest()
+
+
+def tune_cache_lookup(path, key):
+    """Tile width stored under `key` in the JSON file `path`, or None."""
+    if not path or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        entry = json.load(f).get(key)
+    return None if entry is None else int(entry["tile_qubits"])
+
+
+def tune_cache_store(path, key, tile_qubits, **info):
+    """Adds or replaces `key` in the JSON file `path` (written to a temporary
+    file and renamed, so a reader never sees half a file)."""
+    data = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            data = json.load(f)
+    data[key] = dict(tile_qubits=int(tile_qubits), **info)
+    tmp = f"{path}.tmp{os.getpid()}"
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
 def _bind():
     L = lib()
     if not getattr(L, "_batch_bound", False):
@@ -94,10 +133,20 @@ def _bind():
 
 
 class Batch:
-    """Many circuits run as one graph launch (tmfqs_hip_batch)."""
+    """Many circuits run as one graph launch (tmfqs_hip_batch).
 
-    def __init__(self, circuits, precision=C128, init_states=None, **cfg):
+    tune_cache: path of a JSON file of tuned tile widths.  When it holds one
+    for this workload (see tune_key) the batch is planned with it and tune()
+    returns it without timing anything; otherwise tune() times the candidates
+    and stores the winner there."""
+
+    def __init__(self, circuits, precision=C128, init_states=None, tune_cache=None, **cfg):
         L = _bind()
+        self._cache = tune_cache
+        self._key = tune_key(circuits, precision) if tune_cache else None
+        self.cached_tile = tune_cache_lookup(tune_cache, self._key) if tune_cache else None
+        if self.cached_tile is not None and not cfg.get("tile_qubits"):
+            cfg["tile_qubits"] = self.cached_tile
         self.n = [c[0] for c in circuits]
         self.nparams = [sum(1 for i in range(c[2]) if c[1][i].kind in (OP_CPHASE, OP_PHASE)) for c in circuits]
         arr = (Circuit * max(1, len(circuits)))()
@@ -130,6 +179,9 @@ class Batch:
     def tune(self):
         t = ctypes.c_int()
         check(_bind().tmfqs_hip_batch_tune(self._h, ctypes.byref(t)))
+        if self._cache and self.cached_tile is None:
+            tune_cache_store(self._cache, self._key, t.value, circuits=len(self.n), qubits=[min(self.n), max(self.n)])
+            self.cached_tile = t.value
         return t.value
 
     def run(self):

@@ -146,6 +146,7 @@ struct tmfqs_hip_state {
 	std::vector<u64> dirty;  // states whose tracked value must be scattered at the end of the run
 	std::unordered_set<u64> dirty_set;
 	bool profiling = false;
+	std::string trace_path;  // TMFQS_HIP_TRACE: where the handle's kernel report goes when it is destroyed
 	std::vector<ProfRec> recs;
 	std::vector<hipEvent_t> event_pool;
 	std::map<std::string, ProfAgg> prof;
@@ -1738,10 +1739,47 @@ tmfqs_hip_state *checked(tmfqs_hip_state *h) {
 	return h;
 }
 
+// One JSON line per handle: every kernel's launches, device time,
+// algorithmic bytes, GB/s and the fraction of the 8 TB/s HBM peak.  "1" or
+// "stderr" prints it; any other value names a file the line is appended to.
+void write_trace(tmfqs_hip_state *h) {
+	try {
+		collect_profile(h);
+	} catch(...) {
+		return;
+	}
+	std::string out = "{\"tmfqs_hip_trace\": 1, \"qubits\": " + std::to_string(h->init ? h->n : 0) + ", \"precision\": \"" +
+	                  (h->prec == 0 ? "complex128" : "complex64") + "\", \"hbm_peak_gb_per_s\": 8000, \"kernels\": [";
+	double total = 0;
+	bool first = true;
+	for(const auto &kv : h->prof) {
+		const ProfAgg &a = kv.second;
+		const double gbs = a.ms > 0 ? a.bytes / (a.ms * 1e6) : 0.0;
+		char buf[320];
+		snprintf(buf, sizeof buf,
+		         "%s{\"name\": \"%s\", \"launches\": %llu, \"ms\": %.6f, \"bytes\": %.0f, \"gb_per_s\": %.1f, "
+		         "\"fraction_of_hbm_peak\": %.6f}",
+		         first ? "" : ", ", kv.first.c_str(), (unsigned long long)a.launches, a.ms, a.bytes, gbs, gbs / 8000.0);
+		out += buf;
+		first = false;
+		total += a.ms;
+	}
+	char tail[64];
+	snprintf(tail, sizeof tail, "], \"total_ms\": %.6f}\n", total);
+	out += tail;
+	if(h->trace_path == "1" || h->trace_path == "stderr") {
+		fputs(out.c_str(), stderr);
+	} else if(FILE *f = fopen(h->trace_path.c_str(), "a")) {
+		fputs(out.c_str(), f);
+		fclose(f);
+	}
+}
+
 void destroy_handle(tmfqs_hip_state *h) {
 	if(!h) return;
 	hipSetDevice(h->cfg.device);
 	if(h->stream) hipStreamSynchronize(h->stream);
+	if(!h->trace_path.empty() && h->stream) write_trace(h);
 	for(auto &r : h->recs) {
 		hipEventDestroy(r.start);
 		hipEventDestroy(r.stop);
@@ -1836,6 +1874,11 @@ int tmfqs_hip_create(const tmfqs_hip_config *cfg, tmfqs_hip_state **out) {
 		if(h->cfg.device < 0 || h->cfg.device >= count) throw std::invalid_argument("tmfqs_hip_create: device ordinal out of range");
 		HIPC(hipSetDevice(h->cfg.device));
 		HIPC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+		if(const char *tr = getenv("TMFQS_HIP_TRACE"))
+			if(*tr && strcmp(tr, "0") != 0) {
+				h->trace_path = tr;
+				h->profiling = true;
+			}
 		HIPC(hipMalloc((void **)&h->d_red, (2 * kReduceBlocks + 16) * sizeof(double)));
 		build_phase_tables(h);
 		arena_ensure(h, 8u << 20);
