@@ -44,7 +44,48 @@ void hip_check(hipError_t e, const char *what) {
 	}
 }
 
-#define HIPC(expr) hip_check((expr), #expr)
+// Fault injection for tests of the error paths: TMFQS_HIP_FAULT=<kind>@<k>
+// with kind alloc (device and pinned allocations), launch (kernel launches)
+// or sync (stream, event and device synchronizations) makes the k-th such
+// call after each tmfqs_hip_create / tmfqs_hip_batch_create skip the HIP call
+// and report hipErrorOutOfMemory, hipErrorLaunchFailure or hipErrorUnknown.
+// Nothing reaches the device, so the state stays consistent; the handle is
+// expected to be destroyed afterwards.  Not thread-safe (a test hook).
+struct FaultPlan {
+	int kind = 0;  // 0 off, 1 alloc, 2 launch, 3 sync
+	long target = 0, count = 0;
+};
+FaultPlan g_fault;
+
+void fault_arm() {
+	g_fault = FaultPlan();
+	const char *e = getenv("TMFQS_HIP_FAULT");
+	if(!e || !*e) return;
+	const char *at = strchr(e, '@');
+	const std::string kind(e, at ? (size_t)(at - e) : strlen(e));
+	const long k = at ? strtol(at + 1, nullptr, 10) : 1;
+	if(k < 1) return;
+	g_fault.kind = kind == "alloc" ? 1 : kind == "launch" ? 2 : kind == "sync" ? 3 : 0;
+	g_fault.target = k;
+}
+
+int fault_class(const char *what) {
+	if(strstr(what, "Malloc")) return 1;
+	if(strncmp(what, "launch_", 7) == 0 || strstr(what, "GraphLaunch")) return 2;
+	if(strstr(what, "Synchronize")) return 3;
+	return 0;
+}
+
+bool fault_hit(const char *what) {
+	if(!g_fault.kind || fault_class(what) != g_fault.kind) return false;
+	return ++g_fault.count == g_fault.target;
+}
+
+hipError_t fault_error() {
+	return g_fault.kind == 1 ? hipErrorOutOfMemory : g_fault.kind == 2 ? hipErrorLaunchFailure : hipErrorUnknown;
+}
+
+#define HIPC(expr) hip_check((g_fault.kind && fault_hit(#expr)) ? fault_error() : (expr), #expr)
 
 int map_exception() {
 	try {
@@ -238,6 +279,8 @@ void arena_ensure(tmfqs_hip_state *h, size_t need) {
 		if(A.dev) hipFree(A.dev);
 		A.host = nullptr;
 		A.dev = nullptr;
+		A.cap = 0;  // stays empty if an allocation below fails
+		A.off = 0;
 		size_t cap = std::max<size_t>(need * 2, 8u << 20);
 		HIPC(hipHostMalloc((void **)&A.host, cap, hipHostMallocDefault));
 		HIPC(hipMalloc((void **)&A.dev, cap));
@@ -270,6 +313,7 @@ void ensure_stage(tmfqs_hip_state *h, size_t bytes) {
 		HIPC(hipStreamSynchronize(h->stream));
 		hipHostFree(h->h_stage);
 		h->h_stage = nullptr;
+		h->stage_bytes = 0;
 	}
 	HIPC(hipHostMalloc((void **)&h->h_stage, bytes, hipHostMallocDefault));
 	h->stage_bytes = bytes;
@@ -289,6 +333,10 @@ void ensure_chunk_buffers(tmfqs_hip_state *h, size_t nch) {
 	hipFree(h->d_sums);
 	hipFree(h->d_ticks);
 	hipFree(h->d_ulog);
+	h->d_sums = nullptr;
+	h->d_ticks = nullptr;
+	h->d_ulog = nullptr;
+	h->chunk_cap = 0;  // stays empty if an allocation below fails
 	HIPC(hipMalloc((void **)&h->d_sums, nch * sizeof(double)));
 	HIPC(hipMalloc((void **)&h->d_ticks, nch * sizeof(ChunkTicks)));
 	HIPC(hipMalloc((void **)&h->d_ulog, nch * sizeof(int32_t)));
@@ -1538,6 +1586,7 @@ void adjoint_run(tmfqs_hip_state *h, std::vector<QOp> &qops, const std::vector<s
 		HIPC(hipStreamSynchronize(h->stream));
 		hipFree(h->d_grads);
 		h->d_grads = nullptr;
+		h->grads_cap = 0;
 		HIPC(hipMalloc((void **)&h->d_grads, (P + 1) * sizeof(double)));
 		h->grads_cap = P + 1;
 	}
@@ -1855,6 +1904,7 @@ int tmfqs_hip_create(const tmfqs_hip_config *cfg, tmfqs_hip_state **out) {
 	try {
 		if(!out) throw std::invalid_argument("tmfqs_hip_create: null output pointer");
 		*out = nullptr;
+		fault_arm();
 		h = new tmfqs_hip_state();
 		if(cfg)
 			h->cfg = *cfg;
@@ -2703,6 +2753,7 @@ int tmfqs_hip_batch_create(const tmfqs_hip_config *cfg, const tmfqs_hip_circuit 
 	if(!out) throw std::invalid_argument("tmfqs_hip_batch_create: null output");
 	*out = nullptr;
 	if(count && !circuits) throw std::invalid_argument("tmfqs_hip_batch_create: null circuit list");
+	fault_arm();
 	b = new tmfqs_hip_batch();
 	tmfqs_hip_config_default(&b->cfg);
 	if(cfg) b->cfg = *cfg;
