@@ -61,7 +61,7 @@ typedef struct tmfqs_hip_config {
 	int fusion;          /* 1: queue and fuse operations into tiled passes; 0: one sweep per call */
 	int tile_qubits;     /* tiled-pass tile size in qubits (0: automatic) */
 	int coalesce_qubits; /* low qubits always inside a tile (0: automatic) */
-	int grover_block;    /* most elementwise steps folded into one sweep (0: 256 for complex128, which replays
+	int grover_block;    /* most elementwise steps folded into one sweep (0: 1024 for complex128, which replays
 	                        every step's rounding; complex64 composes up to 65536 steps into one affine map) */
 	int exact_phases;    /* 1: apply every controlled phase as its own complex product
 	                        (bit-identical to the reference); 0: fuse QFT phase ladders */

@@ -178,6 +178,41 @@ def test_qft_known_answers():
         assert np.allclose(s.amplitudes(), np.exp(2j * np.pi * src * y / 8) / np.sqrt(8), atol=1e-6)
 
 
+@pytest.mark.parametrize("case", ["real", "mixed", "complex"])
+def test_diffusion_chains_bitexact(case):
+    # Phase flips and materialized diffusions (300 steps: one sweep at the
+    # default block, three at grover_block=128) from a loaded state.  "real": every imaginary part is +0 or -0, so every
+    # wavefront takes the real-only chain.  "mixed": two imaginary parts
+    # +-0.25 cancel exactly in the amplitude sum, so Im(2m) stays 0 and the
+    # two wavefronts holding them take the full chain beside real ones.
+    # "complex": Im(2m) != 0, the full chain everywhere.  Bit for bit against
+    # the oracle, signs of zeros included.
+    n = 14
+    rng = np.random.default_rng(11)
+    inter = np.zeros(2 << n)
+    inter[0::2] = rng.uniform(-1, 1, 1 << n) / 128
+    inter[1::2][rng.random(1 << n) < 0.5] = -0.0
+    if case == "mixed":
+        inter[2 * 100 + 1], inter[2 * 9000 + 1] = 0.25, -0.25
+    if case == "complex":
+        inter[2 * 100 + 1] = 0.25
+    seq = []
+    for it in range(150):
+        seq.append((O.OP_PHASE_FLIP, 0, 0, 0.0, 77 + (it % 3)))
+        seq.append((O.OP_DIFFUSE_MAT,))
+        seq.append((O.OP_DIFFUSE_MAT,))
+    ops, cnt = O.make_ops(seq)
+    want = O.run_circuit(n, (ops, cnt), init=inter).amplitudes()
+    for block in (0, 128):
+        s = st(grover_block=block).load(n, inter)
+        reg = Register(s, n)
+        reg.run(ops, cnt)
+        got = s.export()
+        assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), block
+        if case != "complex":
+            assert np.all(got[1::2][np.r_[0:100, 101:9000, 9001:1 << n]] == 0.0)
+
+
 @pytest.mark.parametrize("materialized", [True, False])
 @pytest.mark.parametrize("mode", MODES, ids=["unfused", "fused-exact", "fused"])
 def test_grover_vs_oracle(materialized, mode):
@@ -190,8 +225,8 @@ def test_grover_vs_oracle(materialized, mode):
         oreg = O.run_circuit(n, (ops, cnt))
         want = cplx(oreg.amplitudes())
         assert np.abs(got - want).max() <= TOL
-        if materialized:
-            assert np.array_equal(s.export(), oreg.amplitudes())
+        if materialized:  # bit for bit, signs of zeros included
+            assert np.array_equal(s.export().view(np.uint64), oreg.amplitudes().view(np.uint64))
         for seed in (123456, 1, 2, 3):
             rnd = O.mt_unit_draws(seed, 1)[0]
             assert reg.measure(rnd) == oreg.measure(rnd)

@@ -474,13 +474,13 @@ void sync_all(tmfqs_hip_state *h) {
 }
 
 // Default most elementwise steps per flush: complex128 replays each step's
-// rounding (bit-identical to the dense backend), so a sweep carries 256 steps
-// (two FP64 adds per amplitude and step: past ~100 steps the sweep is bound by
-// FP64 issue, not HBM, and longer sweeps only save passes); complex64
-// composes a whole run into one affine map.
+// rounding (bit-identical to the dense backend), so a sweep carries 1024
+// steps (one FP64 add per amplitude and step for real states, two otherwise:
+// past ~100 steps a sweep is bound by FP64 issue, not HBM, and longer sweeps
+// only save passes); complex64 composes a whole run into one affine map.
 unsigned ew_block(const tmfqs_hip_state *h) {
 	if(h->cfg.grover_block > 0) return (unsigned)h->cfg.grover_block;
-	return h->prec == 0 ? 256u : 65536u;
+	return h->prec == 0 ? 1024u : 65536u;
 }
 
 void exec_elementwise(tmfqs_hip_state *h, const std::vector<QOp> &ops, size_t lo, size_t hi) {
@@ -516,13 +516,19 @@ void exec_elementwise(tmfqs_hip_state *h, const std::vector<QOp> &ops, size_t lo
 			for(size_t i = s0; i < s0 + cnt; ++i) pure = pure && steps[i].kind == EW_DIFFUSE;
 			if(pure) {
 				std::vector<double> m2(2 * cnt);
+				// the chain applied to an imaginary part +0 and -0, valid when every Im(2m) is +-0
+				bool zero_ok = true;
+				double zp = 0.0, zm = -0.0;
 				for(size_t i = 0; i < cnt; ++i) {
 					m2[2 * i] = steps[s0 + i].ar;
 					m2[2 * i + 1] = steps[s0 + i].ai;
+					zero_ok = zero_ok && steps[s0 + i].ai == 0.0;
+					zp = steps[s0 + i].ai - zp;
+					zm = steps[s0 + i].ai - zm;
 				}
 				const double *d = (const double *)arena_upload(h, m2.data(), m2.size() * sizeof(double));
 				ProfScope ps(h, "k_diffuse_chain", pass_bytes(h));
-				HIPC(launch_diffuse_chain(h->L(), h->amp, h->n, d, (unsigned)cnt));
+				HIPC(launch_diffuse_chain(h->L(), h->amp, h->n, d, (unsigned)cnt, zp, zm, zero_ok ? 1 : 0));
 			} else {
 				const EwStep *d = (const EwStep *)arena_upload(h, steps.data() + s0, cnt * sizeof(EwStep));
 				ProfScope ps(h, "k_ew_chain", pass_bytes(h));

@@ -130,8 +130,11 @@ unsigned qft_block_coalesce_bits(int prec, int requested);
 // bit reversal, in place, for 12 <= n <= 40; dcq holds the 63 constants
 // e^{i pi v / 2^t}, t = 0..5, v < 2^t (at 2^t - 1 + v).
 hipError_t launch_qft_tail_reverse(const Launch &L, void *a, unsigned n, const double *dcq);
-// Chain of diffusions only: dm2 holds nsteps pairs (2 Re m, 2 Im m).
-hipError_t launch_diffuse_chain(const Launch &L, void *a, unsigned n, const double *dm2, unsigned nsteps);
+// Chain of diffusions only: dm2 holds nsteps pairs (2 Re m, 2 Im m).  When
+// zero_ok (every 2 Im m is +-0), zp and zm are the chain applied to an
+// imaginary part +0 and -0; wavefronts of real amplitudes use them.
+hipError_t launch_diffuse_chain(const Launch &L, void *a, unsigned n, const double *dm2, unsigned nsteps, double zp, double zm,
+                                int zero_ok);
 hipError_t launch_copy(const Launch &L, const void *src, void *dst, unsigned n);
 
 struct CompareBlock {

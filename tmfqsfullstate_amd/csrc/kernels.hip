@@ -300,9 +300,17 @@ hipError_t launch_copy(const Launch &L, const void *src, void *dst, unsigned n) 
 // lane) and applies every step to all four in FP64 registers, so one sweep of
 // HBM carries the whole chain.  a = 2m - a rounds once per step exactly as the
 // reference's inversionAboutMean(mean) does.
+//
+// Real states (every imaginary part +-0, as in Grover from a basis state):
+// when every step's Im(2m) is +-0 too (zero_ok), the imaginary chain only
+// ever subtracts zeros, and IEEE subtraction makes its result depend on
+// nothing but the sign of the starting zero.  The host runs the chain on +0
+// and on -0 (zp, zm); a wavefront whose imaginary parts are all zero takes
+// those and computes only the real chain (half the FP64 adds), with the same
+// bits as the full chain.
 template <typename R>
 __global__ void __launch_bounds__(256) k_diffuse_chain(typename V2T<R>::t *__restrict__ a, u64 N, const double2 *__restrict__ m2,
-                                                       unsigned nsteps) {
+                                                       unsigned nsteps, double zp, double zm, int zero_ok) {
 	using V = typename V2T<R>::t;
 	constexpr int U = 4;
 	const u64 stride = (u64)gridDim.x * blockDim.x * U;
@@ -315,12 +323,26 @@ __global__ void __launch_bounds__(256) k_diffuse_chain(typename V2T<R>::t *__res
 			re[u] = v.x;
 			im[u] = v.y;
 		}
-		for(unsigned s = 0; s < nsteps; ++s) {
-			const double2 m = m2[s];
+		bool real = zero_ok != 0;
 #pragma unroll
-			for(int u = 0; u < U; ++u) {
-				re[u] = m.x - re[u];
-				im[u] = m.y - im[u];
+		for(int u = 0; u < U; ++u) real = real && im[u] == 0.0;
+		if(__all(real)) {
+#pragma unroll 8
+			for(unsigned s = 0; s < nsteps; ++s) {
+				const double mx = m2[s].x;
+#pragma unroll
+				for(int u = 0; u < U; ++u) re[u] = mx - re[u];
+			}
+#pragma unroll
+			for(int u = 0; u < U; ++u) im[u] = signbit(im[u]) ? zm : zp;
+		} else {
+			for(unsigned s = 0; s < nsteps; ++s) {
+				const double2 m = m2[s];
+#pragma unroll
+				for(int u = 0; u < U; ++u) {
+					re[u] = m.x - re[u];
+					im[u] = m.y - im[u];
+				}
 			}
 		}
 #pragma unroll
@@ -331,14 +353,17 @@ __global__ void __launch_bounds__(256) k_diffuse_chain(typename V2T<R>::t *__res
 	}
 }
 
-hipError_t launch_diffuse_chain(const Launch &L, void *a, unsigned n, const double *dm2, unsigned nsteps) {
+hipError_t launch_diffuse_chain(const Launch &L, void *a, unsigned n, const double *dm2, unsigned nsteps, double zp, double zm,
+                                int zero_ok) {
 	const u64 N = 1ull << n;
 	u64 blocks = (N + 1023) / 1024;
 	if(blocks > 262144) blocks = 262144;
 	if(L.prec == 0)
-		hipLaunchKernelGGL(k_diffuse_chain<double>, dim3((unsigned)blocks), dim3(256), 0, L.stream, (double2 *)a, N, (const double2 *)dm2, nsteps);
+		hipLaunchKernelGGL(k_diffuse_chain<double>, dim3((unsigned)blocks), dim3(256), 0, L.stream, (double2 *)a, N, (const double2 *)dm2, nsteps,
+		                   zp, zm, zero_ok);
 	else
-		hipLaunchKernelGGL(k_diffuse_chain<float>, dim3((unsigned)blocks), dim3(256), 0, L.stream, (float2 *)a, N, (const double2 *)dm2, nsteps);
+		hipLaunchKernelGGL(k_diffuse_chain<float>, dim3((unsigned)blocks), dim3(256), 0, L.stream, (float2 *)a, N, (const double2 *)dm2, nsteps,
+		                   zp, zm, zero_ok);
 	return hipGetLastError();
 }
 
