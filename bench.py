@@ -404,6 +404,19 @@ def main():
     copy_ms = hs.copy_probe(3)
     copy_gbps = 2.0 * 16 * N / (copy_ms * 1e-3) / 1e9
     tp = reg.total_probability()
+    # host hand-over of the whole state through the C-ABI (pinned, double-
+    # buffered staging over PCIe): reported beside, never part of `value`
+    t0 = time.perf_counter()
+    host = hs.export()
+    export_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hs.load(n, host)
+    hs.synchronize()
+    load_s = time.perf_counter() - t0
+    del host
+    state_io = {"bytes": 16.0 * N, "export_s": export_s, "export_GBps": 16.0 * N / export_s / 1e9,
+                "load_s": load_s, "load_GBps": 16.0 * N / load_s / 1e9}
+    log(f"state I/O: export {state_io['export_GBps']:.1f} GB/s, load {state_io['load_GBps']:.1f} GB/s (host buffers)")
     step_bytes = sum(v["bytes"] for v in prof.values()) / args.steps
     launches = {k: v["launches"] / args.steps for k, v in prof.items()}
     dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
@@ -423,6 +436,7 @@ def main():
         "gate_equivalent_GBps_not_roofline": (n + n * (n - 1) // 2 + 3 * (n // 2)) * 2.0 * 16 * N
         / (ms_per_step * 1e-3) / 1e9,
         "total_probability_after": tp,
+        "state_io_host_buffers": state_io,
         "kernels": prof,
     }
     roofline = {

@@ -1732,14 +1732,27 @@ ChainResult chain_walk(tmfqs_hip_state *h, int sel, bool find, double rnd, const
 }
 
 // ------------------------------------------------------------ bulk copies
+// The two staging slots' completion events, released on every exit path.
+struct SlotEvents {
+	hipEvent_t ev[2] = {nullptr, nullptr};
+	void create() {
+		HIPC(hipEventCreate(&ev[0]));
+		HIPC(hipEventCreate(&ev[1]));
+	}
+	~SlotEvents() {
+		for(hipEvent_t e : ev)
+			if(e) hipEventDestroy(e);
+	}
+};
+
 void load_host(tmfqs_hip_state *h, const double *src) {
 	const size_t stage = 64u << 20;
 	ensure_stage(h, 2 * stage);
 	const u64 total_states = h->N;
 	const u64 per = stage / h->esz;
-	hipEvent_t ev[2];
-	HIPC(hipEventCreate(&ev[0]));
-	HIPC(hipEventCreate(&ev[1]));
+	SlotEvents slots;
+	slots.create();
+	hipEvent_t *ev = slots.ev;
 	bool used[2] = {false, false};
 	u64 s0 = 0;
 	int slot = 0;
@@ -1760,28 +1773,40 @@ void load_host(tmfqs_hip_state *h, const double *src) {
 		slot ^= 1;
 	}
 	HIPC(hipStreamSynchronize(h->stream));
-	hipEventDestroy(ev[0]);
-	hipEventDestroy(ev[1]);
 }
 
+// Device -> host through two pinned slots: the copy of chunk i+1 is in
+// flight while chunk i is copied (or widened, complex64) into `out`.
 void export_host(tmfqs_hip_state *h, u64 first, u64 count, double *out) {
-	sync_all(h);
+	sync_all(h);  // exported values include every queued exact scatter
 	const size_t stage = 64u << 20;
-	ensure_stage(h, stage);
+	ensure_stage(h, 2 * stage);
 	const u64 per = stage / h->esz;
+	SlotEvents slots;
+	slots.create();
+	auto enqueue = [&](u64 s0, int slot) {
+		const u64 cnt = std::min<u64>(per, count - s0);
+		HIPC(hipMemcpyAsync(h->h_stage + slot * stage, (char *)h->amp + (first + s0) * h->esz, cnt * h->esz, hipMemcpyDeviceToHost,
+		                    h->stream));
+		HIPC(hipEventRecord(slots.ev[slot], h->stream));
+	};
+	if(count) enqueue(0, 0);
 	u64 s0 = 0;
+	int slot = 0;
 	while(s0 < count) {
 		const u64 cnt = std::min<u64>(per, count - s0);
-		HIPC(hipMemcpy(h->h_stage, (char *)h->amp + (first + s0) * h->esz, cnt * h->esz, hipMemcpyDeviceToHost));
+		if(s0 + cnt < count) enqueue(s0 + cnt, slot ^ 1);  // that slot's previous chunk is already copied out
+		HIPC(hipEventSynchronize(slots.ev[slot]));
+		const char *src = h->h_stage + slot * stage;
 		if(h->prec == 0) {
-			std::memcpy(out + 2 * s0, h->h_stage, cnt * 16);
+			std::memcpy(out + 2 * s0, src, cnt * 16);
 		} else {
-			const float *f = (const float *)h->h_stage;
+			const float *f = (const float *)src;
 			for(u64 i = 0; i < 2 * cnt; ++i) out[2 * s0 + i] = f[i];
 		}
 		s0 += cnt;
+		slot ^= 1;
 	}
-	// exported values reflect queued exact scatters already (sync_all flushed)
 }
 
 void device_zero(tmfqs_hip_state *h) { HIPC(hipMemsetAsync(h->amp, 0, h->amp_bytes, h->stream)); }
