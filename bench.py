@@ -505,7 +505,7 @@ def main():
             # Grover's state stays real (every imaginary part +-0), so each
             # diffusion step is one FP64 add per amplitude (the real part; the
             # imaginary part is the chain applied to +-0, computed once on the
-            # host): with 256 steps per sweep the sweeps sit between this rate
+            # host): with 1024 steps per sweep the sweeps are bound by this rate
             # and HBM bandwidth
             "diffusion_fp64_adds_per_s": (iters * 1.0 * N / (gprof["k_diffuse_chain"]["ms"] * 1e-3)
                                           if "k_diffuse_chain" in gprof else None),
