@@ -505,8 +505,8 @@ def main():
             # Grover's state stays real (every imaginary part +-0), so each
             # diffusion step is one FP64 add per amplitude (the real part; the
             # imaginary part is the chain applied to +-0, computed once on the
-            # host): with 1024 steps per sweep the sweeps are bound by this rate
-            # and HBM bandwidth
+            # host): with 1024 steps per sweep the sweeps are bound by this
+            # rate, not by HBM bandwidth
             "diffusion_fp64_adds_per_s": (iters * 1.0 * N / (gprof["k_diffuse_chain"]["ms"] * 1e-3)
                                           if "k_diffuse_chain" in gprof else None),
             "diffusion_hbm_GBps": (gprof["k_diffuse_chain"]["bytes"] / (gprof["k_diffuse_chain"]["ms"] * 1e-3) / 1e9
