@@ -1138,9 +1138,10 @@ template <typename R> struct QftTailWaves {
 // Workgroups of 256 lanes in the layout of TailLayout: every load instruction
 // reads whole 128 B lines.  Both tiles of a pair are read before
 // either is written; each is transformed in registers and staged through one
-// 64 x 64 LDS tile for its transposed store.  The loads of the workgroup's
-// next pair are issued as soon as the registers they replace are staged, so
-// they are in flight while the current pair is stored.
+// 64 x 64 LDS tile for its transposed store.  When a workgroup has several
+// pairs (complex64), the loads of its next pair are issued as soon as the
+// registers they replace are staged, so they are in flight while the current
+// pair is stored.
 template <typename R>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QftTailWaves<R>::value)))
 k_qft_tail_reverse(typename V2T<R>::t *__restrict__ a, QftTailParams P, const double2 *__restrict__ cq) {
@@ -1206,7 +1207,10 @@ static hipError_t launch_qft_tail_t(const Launch &L, void *a, unsigned n, const 
 	P.K = 1u << P.h;
 	P.npairs = (u64)((P.m & 1u) ? 2 : 1) * ((u64)P.K * (P.K + 1) / 2);
 	const size_t shmem = 4096 * sizeof(V);
-	const u64 cap = 8192;  // 16 (complex128) to 32 pairs per workgroup at n = 30 (measured best)
+	// Measured at n = 30: complex128 runs fastest with one pair per workgroup
+	// (6.50 ms against 6.80 ms at 16 pairs each), complex64 with about three
+	// (3.34 ms at a grid of 49152 against 3.44 ms at one pair each).
+	const u64 cap = sizeof(R) == 8 ? P.npairs : 49152;
 	const unsigned grid = (unsigned)(P.npairs < cap ? P.npairs : cap);
 	static bool attr = false;
 	if(!attr) {
