@@ -1294,7 +1294,7 @@ void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_
 		}
 		if(dual && !L.gidx.empty()) {
 			const uint32_t *dg = (const uint32_t *)arena_upload(h, L.gidx.data(), L.gidx.size() * sizeof(uint32_t));
-			HIPC(launch_grad_finish(h->L(), tgt->gpart, tile_grid(L.tp, true), (unsigned)L.gidx.size(), dg, tgt->grad_scale,
+			HIPC(launch_grad_finish(h->L(), tgt->gpart, tile_grid(L.tp, true, h->prec), (unsigned)L.gidx.size(), dg, tgt->grad_scale,
 			                        tgt->dgrads));
 		}
 	}

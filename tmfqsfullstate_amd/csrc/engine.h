@@ -146,7 +146,7 @@ hipError_t launch_compare(const Launch &L, int prec_ref, const void *ref, int pr
 // Register-stage width for a tile of T bits.
 unsigned tile_stage_width(int prec, unsigned T, bool dual);
 // Workgroups of a tiled pass over P; a dual pass's gpart holds nslots * grid doubles.
-unsigned tile_grid(const TilePass &P, bool dual);
+unsigned tile_grid(const TilePass &P, bool dual, int prec);
 // Tiled pass over one state (b == nullptr) or two states stepped together
 // (b != nullptr: TOP_GRAD partials into gpart); ldsops: some stage runs on
 // the LDS tile (a separate kernel build, so the common case keeps its

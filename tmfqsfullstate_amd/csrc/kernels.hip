@@ -2153,8 +2153,11 @@ unsigned tile_stage_width(int prec, unsigned T, bool dual) {
 	return T < w ? T : w;
 }
 
-unsigned tile_grid(const TilePass &P, bool dual) {
-	const u64 cap = dual ? 4096 : 65536;
+unsigned tile_grid(const TilePass &P, bool dual, int prec) {
+	// dual: at most 4096 gradient partials per slot.  Single-state passes
+	// over 2^30 amplitudes (Hadamard layer, measured): complex128 fastest at
+	// 65536 workgroups, complex64 at 32768 (6% faster than at 65536).
+	const u64 cap = dual ? 4096 : (prec == 0 ? 65536 : 32768);
 	return (unsigned)(P.ntiles < cap ? P.ntiles : cap);
 }
 
@@ -2172,7 +2175,7 @@ static hipError_t launch_staged_t(const Launch &L, void *a, void *b, const TileP
 		                          160 * 1024);
 		attr = true;
 	}
-	TMFQS_LAUNCH((k_tile_staged<R, W, DUAL, LDSOPS>), tile_grid(P, DUAL), 256, shmem, L.stream, (V *)a, (V *)b, P, dops, dstages, nstages,
+	TMFQS_LAUNCH((k_tile_staged<R, W, DUAL, LDSOPS>), tile_grid(P, DUAL, L.prec), 256, shmem, L.stream, (V *)a, (V *)b, P, dops, dstages, nstages,
 	             dmats, dterms, (const double2 *)dphase, gpart, nslots);
 	return hipGetLastError();
 }
