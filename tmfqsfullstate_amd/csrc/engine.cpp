@@ -2605,12 +2605,16 @@ void batch_free_plan(tmfqs_hip_batch *b) {
 	b->rounds.clear();
 }
 
+// Workgroups of a batch round's work-list kernel (the seed-42 workload:
+// 12.8 ms at 8192, 13.4 ms at 2048).
+constexpr unsigned kBatchGrid = 8192;
+
 void batch_enqueue(tmfqs_hip_batch *b) {
 	HIPC(launch_batch_init(b->L(), b->d_init, b->ninit));
 	const unsigned W = tile_stage_width(b->prec, b->T, false);
 	for(const auto &r : b->rounds)
 		HIPC(launch_batch_tiles(b->L(), W, b->d_passes + r.pass0, b->d_items + r.item0, (unsigned)r.nitems, b->d_ops, b->d_stages,
-		                        nullptr, 2048));
+		                        nullptr, kBatchGrid));
 	HIPC(launch_batch_permute(b->L(), b->d_perm, b->nperm));
 }
 
@@ -3078,7 +3082,7 @@ int tmfqs_hip_batch_adjoint(tmfqs_hip_batch *b, const tmfqs_hip_diag_term *terms
 		}
 	for(const DualRound &rd : rounds) {
 		HIPC(launch_batch_dual(b->L(), Wd, d_passes + rd.pass0, d_items + rd.item0, (unsigned)rd.nitems, d_ops, d_st, d_gpart,
-		                       2048));
+		                       kBatchGrid));
 		HIPC(launch_batch_grad_finish(b->L(), d_gents + rd.grad0, (unsigned)rd.ngrads, d_gpart, -2.0, d_out));
 	}
 	if(total) HIPC(hipMemcpyAsync(grads, d_out, total * sizeof(double), hipMemcpyDeviceToHost, b->stream));

@@ -897,8 +897,9 @@ static hipError_t launch_qft_block_t(const Launch &L, void *a, unsigned n, unsig
 	P.ntiles = 1ull << (n - 8 - C);
 	// complex128: a grid of 1024 workgroups that loop over their tiles, so each
 	// next tile's loads overlap stage B (3% faster at n = 30 than one tile per
-	// workgroup); complex64: one tile per workgroup, overlapping across them.
-	const u64 cap = QftBlockWaves<R, C>::prefetch ? 1024 : (1u << 20);
+	// workgroup); complex64 (no prefetch): at most 65536 workgroups, two tiles
+	// each at n = 30 (3% faster than one tile per workgroup).
+	const u64 cap = QftBlockWaves<R, C>::prefetch ? 1024 : 65536;
 	const unsigned grid = (unsigned)(P.ntiles < cap ? P.ntiles : cap);
 	const size_t shmem = (size_t)((256u << C) + 16u) * sizeof(V);
 	static bool attr = false;
