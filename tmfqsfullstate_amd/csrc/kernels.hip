@@ -286,8 +286,10 @@ __global__ void __launch_bounds__(256) k_copy(const typename V2T<R>::t *__restri
 	for(u64 i = gtid(); i < N; i += gstride()) nt_store(nt_load(&src[i]), &dst[i]);
 }
 
-// Streaming passes run on up to 2^18 workgroups: a 2^30-amplitude copy
-// measured 5.69 TB/s at that grid against 5.10 TB/s at 2^14.
+// Streaming passes (copies, elementwise chains, per-operation sweeps) run on
+// up to 2^18 workgroups: a 2^30-amplitude copy measured 5.69 TB/s at that
+// grid against 5.10 TB/s at 2^14; per-operation Hadamard and X sweeps were
+// 6-12% faster than at 2^13.
 constexpr u64 kStreamGrid = 262144;
 
 hipError_t launch_copy(const Launch &L, const void *src, void *dst, unsigned n) {
@@ -1970,18 +1972,18 @@ hipError_t launch_hadamard(const Launch &L, void *a, unsigned n, unsigned bit) {
 	const u64 np = (1ull << n) >> 1;
 	const double c = 1.0 / sqrt(2.0);
 	if(L.prec == 0)
-		TMFQS_LAUNCH(k_hadamard<double>, grid_for(np), 256, 0, L.stream, (double2 *)a, np, bit, c);
+		TMFQS_LAUNCH(k_hadamard<double>, grid_for(np, 256, kStreamGrid), 256, 0, L.stream, (double2 *)a, np, bit, c);
 	else
-		TMFQS_LAUNCH(k_hadamard<float>, grid_for(np), 256, 0, L.stream, (float2 *)a, np, bit, (float)c);
+		TMFQS_LAUNCH(k_hadamard<float>, grid_for(np, 256, kStreamGrid), 256, 0, L.stream, (float2 *)a, np, bit, (float)c);
 	return hipGetLastError();
 }
 
 hipError_t launch_pauli_x(const Launch &L, void *a, unsigned n, unsigned bit, u64 cmask) {
 	const u64 np = (1ull << n) >> 1;
 	if(L.prec == 0)
-		TMFQS_LAUNCH(k_pauli_x<double>, grid_for(np), 256, 0, L.stream, (double2 *)a, np, bit, cmask);
+		TMFQS_LAUNCH(k_pauli_x<double>, grid_for(np, 256, kStreamGrid), 256, 0, L.stream, (double2 *)a, np, bit, cmask);
 	else
-		TMFQS_LAUNCH(k_pauli_x<float>, grid_for(np), 256, 0, L.stream, (float2 *)a, np, bit, cmask);
+		TMFQS_LAUNCH(k_pauli_x<float>, grid_for(np, 256, kStreamGrid), 256, 0, L.stream, (float2 *)a, np, bit, cmask);
 	return hipGetLastError();
 }
 
@@ -1992,19 +1994,19 @@ hipError_t launch_cphase(const Launch &L, void *a, unsigned n, unsigned tbit, u6
 	const u64 nq = (1ull << n) >> 2;
 	const u64 setm = (1ull << tbit) | cmask;
 	if(L.prec == 0)
-		TMFQS_LAUNCH(k_cphase<double>, grid_for(nq), 256, 0, L.stream, (double2 *)a, nq, lo, hi, setm, cr, ci);
+		TMFQS_LAUNCH(k_cphase<double>, grid_for(nq, 256, kStreamGrid), 256, 0, L.stream, (double2 *)a, nq, lo, hi, setm, cr, ci);
 	else
-		TMFQS_LAUNCH(k_cphase<float>, grid_for(nq), 256, 0, L.stream, (float2 *)a, nq, lo, hi, setm, (float)cr, (float)ci);
+		TMFQS_LAUNCH(k_cphase<float>, grid_for(nq, 256, kStreamGrid), 256, 0, L.stream, (float2 *)a, nq, lo, hi, setm, (float)cr, (float)ci);
 	return hipGetLastError();
 }
 
 hipError_t launch_u1(const Launch &L, void *a, unsigned n, unsigned bit, const double *m) {
 	const u64 np = (1ull << n) >> 1;
 	if(L.prec == 0)
-		TMFQS_LAUNCH(k_u1<double>, grid_for(np), 256, 0, L.stream, (double2 *)a, np, bit, m[0], m[1], m[2], m[3], m[4], m[5],
+		TMFQS_LAUNCH(k_u1<double>, grid_for(np, 256, kStreamGrid), 256, 0, L.stream, (double2 *)a, np, bit, m[0], m[1], m[2], m[3], m[4], m[5],
 		             m[6], m[7]);
 	else
-		TMFQS_LAUNCH(k_u1<float>, grid_for(np), 256, 0, L.stream, (float2 *)a, np, bit, (float)m[0], (float)m[1],
+		TMFQS_LAUNCH(k_u1<float>, grid_for(np, 256, kStreamGrid), 256, 0, L.stream, (float2 *)a, np, bit, (float)m[0], (float)m[1],
 		             (float)m[2], (float)m[3], (float)m[4], (float)m[5], (float)m[6], (float)m[7]);
 	return hipGetLastError();
 }
@@ -2039,9 +2041,9 @@ hipError_t launch_bitswap(const Launch &L, void *a, unsigned n, unsigned p, unsi
 	}
 	const u64 nq = (1ull << n) >> 2;
 	if(L.prec == 0)
-		TMFQS_LAUNCH(k_bitswap<double>, grid_for(nq), 256, 0, L.stream, (double2 *)a, nq, p, q);
+		TMFQS_LAUNCH(k_bitswap<double>, grid_for(nq, 256, kStreamGrid), 256, 0, L.stream, (double2 *)a, nq, p, q);
 	else
-		TMFQS_LAUNCH(k_bitswap<float>, grid_for(nq), 256, 0, L.stream, (float2 *)a, nq, p, q);
+		TMFQS_LAUNCH(k_bitswap<float>, grid_for(nq, 256, kStreamGrid), 256, 0, L.stream, (float2 *)a, nq, p, q);
 	return hipGetLastError();
 }
 
