@@ -434,3 +434,23 @@ def test_qft_largest_sizes_basis_closed_form(precision, n):
             want = np.exp(2j * np.pi * (ph.astype(np.float64) / N)) / np.sqrt(N)
             err = np.abs(got - want).max() * np.sqrt(N)  # relative to the amplitude size
             assert err <= (1e-10 if precision == T.C128 else 1e-5), (first, err)
+
+
+@pytest.mark.parametrize("precision", [T.C128, T.C64], ids=["c128", "c64"])
+def test_load_export_round_trip_across_staging_chunks(precision):
+    # n = 24: 256 MiB complex128 / 128 MiB complex64, several 64 MiB staging
+    # chunks each way; partial exports start and end inside chunks.
+    n = 24
+    rng = np.random.default_rng(3)
+    inter = rng.uniform(-1, 1, 2 << n)
+    if precision == T.C64:
+        inter = inter.astype(np.float32).astype(np.float64)  # exactly representable
+    s = st(precision=precision).load(n, inter)
+    assert np.array_equal(s.export(), inter)
+    per = (64 << 20) // (16 if precision == T.C128 else 8)  # states per staging chunk
+    for first, count in ((per - 5, 11), (per // 2, per + 7), ((1 << n) - 3, 3), (0, 1)):
+        got = s.export(first, count)
+        assert np.array_equal(got, inter[2 * first:2 * (first + count)]), (first, count)
+    with pytest.raises(T.OutOfRange):  # a range past the last state
+        s.export(per // 2, 1 << n)
+    s.close()
