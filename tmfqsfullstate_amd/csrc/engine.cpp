@@ -1536,7 +1536,7 @@ void adjoint_enqueue(tmfqs_hip_state *h, std::vector<QOp> &qops, const std::vect
 	}
 	void *lam = h->adj_bufs[S];
 	const size_t P = params.size();
-	if(!h->d_gpart) HIPC(hipMalloc((void **)&h->d_gpart, (size_t)kMaxGradSlots * 4096 * sizeof(double)));
+	if(!h->d_gpart) HIPC(hipMalloc((void **)&h->d_gpart, (size_t)kMaxGradSlots * kMaxDualGrid * sizeof(double)));
 	// forward sweep with a checkpoint at every segment end
 	for(size_t s = 0; s < S; ++s) {
 		exec_gates(h, qops, bounds[s], bounds[s + 1]);

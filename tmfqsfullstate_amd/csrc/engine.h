@@ -32,6 +32,9 @@ enum TileOpKind : uint32_t {
 
 // Gradient slots one dual-pass launch can accumulate.
 constexpr unsigned kMaxGradSlots = 64;
+// Most workgroups of a dual (adjoint) tiled pass: its gradient partials
+// buffer holds kMaxGradSlots * kMaxDualGrid doubles.
+constexpr unsigned kMaxDualGrid = 8192;
 
 struct TwiddleTerm {
 	u64 cond;   // global bit that enables the term

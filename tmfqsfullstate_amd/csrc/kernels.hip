@@ -2161,10 +2161,10 @@ unsigned tile_stage_width(int prec, unsigned T, bool dual) {
 }
 
 unsigned tile_grid(const TilePass &P, bool dual, int prec) {
-	// dual: at most 4096 gradient partials per slot.  Single-state passes
+	// dual: at most kMaxDualGrid gradient partials per slot.  Single-state passes
 	// over 2^30 amplitudes (Hadamard layer, measured): complex128 fastest at
 	// 65536 workgroups, complex64 at 32768 (6% faster than at 65536).
-	const u64 cap = dual ? 4096 : (prec == 0 ? 65536 : 32768);
+	const u64 cap = dual ? kMaxDualGrid : (prec == 0 ? 65536 : 32768);
 	return (unsigned)(P.ntiles < cap ? P.ntiles : cap);
 }
 
