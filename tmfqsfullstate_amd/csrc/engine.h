@@ -1,4 +1,4 @@
-// Internal interfaces shared by the gfx950 kernels (kernels.hip) and the host
+// Internal interfaces shared by the gfx950 kernels (k_*.hip) and the host
 // engine (engine.cpp).  Not part of the public ABI.
 #pragma once
 
@@ -119,7 +119,7 @@ hipError_t launch_bitswap(const Launch &L, void *a, unsigned n, unsigned p, unsi
 hipError_t launch_involution(const Launch &L, void *a, unsigned n, const unsigned *perm, unsigned C);
 hipError_t launch_ew_chain(const Launch &L, void *a, unsigned n, const EwStep *dsteps, unsigned nsteps);
 hipError_t launch_scatter(const Launch &L, void *a, const Scatter *dsc, unsigned count);
-// QFT block pass over the block bits pbot+7 .. pbot (see kernels.hip): the
+// QFT block pass over the block bits pbot+7 .. pbot (see k_qft.hip): the
 // top K of them are targets (K = 8: a full block; fewer: a partial leading
 // pass), each with its whole ladder.  dcin holds the constants
 // e^{i pi v / 2^j} at 2^j - 1 + v (FP64, then the same in FP32).

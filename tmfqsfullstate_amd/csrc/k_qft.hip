@@ -1,0 +1,514 @@
+// gfx950 QFT kernels: eight-target block passes and the six-target tail fused
+// with the bit reversal.
+#include "device_common.h"
+
+namespace tmfqs_hip {
+
+// ------------------------------------------------------------------------
+// QFT block pass.  Targets are the 8 bit positions pbot+7 .. pbot (MSB first,
+// as makeQft orders its target qubits); for each target t it applies H on t
+// and then every controlled phase pi/2^(t-c) from the lower bits c < t, i.e.
+// the factor e^{i pi (x mod 2^t) / 2^t} on states with bit t set.  Splitting
+// x mod 2^t into the part inside the block and the part r below it, the inner
+// factors are constants c[j][v] = e^{i pi v / 2^j} and the outer ones commute
+// with the remaining H gates of the block, so they are applied last as one
+// twiddle per state: block bit j set contributes e^{i pi r / 2^(pb+j)}, so a
+// state with block bits y takes w^(r rev8(y)), w = e^{2 pi i / 2^(pb+8)}.
+//
+// A workgroup owns 2^(8+C) states: the 8 block bits times the C contiguous
+// low bits (coalesced 2^C-state runs).  Each thread holds 16 states in
+// registers: first the 16 values of block bits 7..4 (stage A), then, after an
+// LDS transpose, the 16 values of block bits 3..0 (stage B).  Butterflies are
+// unscaled; the eight Hadamard factors 1/sqrt(2) are one exact scale by 1/16,
+// folded into the twiddles.  The stage-A loads of the workgroup's next tile
+// are issued as soon as the current tile is in LDS, so they are in flight
+// during stage B and the stores.
+struct QftBlockParams {
+	u64 ntiles;  // 2^(n - 8 - C)
+	uint32_t n, pbot;
+	uint32_t K;    // targets: the top K block bits (8, or fewer in a partial pass)
+	uint32_t pad;
+	double scale;  // (1/sqrt(2))^K
+};
+
+
+// Waves per SIMD the QFT block kernel is compiled for, and whether a
+// workgroup prefetches its next tile during stage B.  complex128 (two waves
+// per SIMD) prefetches; complex64 keeps to 128 registers (four waves per
+// SIMD) by not prefetching, and other workgroups' loads overlap instead.
+template <typename R, int C> struct QftBlockWaves {
+	static constexpr int value = sizeof(R) == 8 ? 2 : 4;
+	static constexpr bool prefetch = sizeof(R) == 8;
+};
+
+// Stage-A loads of tile `tid`: register ra holds block bits 7..4 = ra.
+template <typename V, int C>
+__device__ __forceinline__ void qft_block_load(V *v, const V *__restrict__ a, u64 tid, unsigned pb, unsigned f4, unsigned lo) {
+	const u64 x = tid << C;
+	const V *src = a + ((((x >> pb) << (pb + 8)) | (x & ((1ull << pb) - 1ull))) + ((u64)f4 << pb) + lo);
+#pragma unroll
+	for(int ra = 0; ra < 16; ++ra) v[ra] = nt_load(src + ((u64)ra << (pb + 4)));
+}
+
+// PART: a partial pass whose targets are only the top K < 8 block bits (the
+// leading (n - 6) mod 8 targets of a QFT); the other block bits are ladder
+// controls only.
+template <typename R, int C, bool PART>
+__global__ void __launch_bounds__(16 << C) __attribute__((amdgpu_waves_per_eu(QftBlockWaves<R, C>::value)))
+k_qft_block(typename V2T<R>::t *__restrict__ a, QftBlockParams P, const double2 *__restrict__ ptab,
+            const double2 *__restrict__ cin, const typename LanePhase<typename V2T<R>::t>::t *__restrict__ cinp) {
+	using V = typename V2T<R>::t;
+	extern __shared__ char smem[];
+	V *tile = (V *)smem;
+	const unsigned t = threadIdx.x;
+	const unsigned lo = t & ((1u << C) - 1u);
+	const unsigned f4 = t >> C;  // stage A: block bits 3..0; stage B: block bits 7..4
+	const unsigned pb = P.pbot;
+	const u64 lowmask = (1ull << pb) - 1ull;
+	// Per-thread constants: the stage-A ladder factors of target 4 + b on
+	// register bits low below it, c[b][low] e^{i pi f4 / 2^(4+b)} with the
+	// in-block constants c[b][v] = cin[2^b - 1 + v] = e^{i pi v / 2^b}, and the
+	// lane's part w^lo of the twiddle base.  Stage B's in-block constants are
+	// read where they are used (uniform addresses: scalar loads), from cinp:
+	// the same table in the amplitudes' precision.
+	typename LanePhase<V>::t pa[15];
+#pragma unroll
+	for(int b = 0; b < 4; ++b) {
+		const double2 tp = ptab[(f4 << (3 - b)) & 255u];  // e^{i pi f4 / 2^(4+b)}
+#pragma unroll
+		for(int low = 0; low < (1 << b); ++low) pa[(1 << b) - 1 + low] = LanePhase<V>::of(cmul(cin[(1 << b) - 1 + low], tp));
+	}
+	const double2 zlo = phase_of_range((u64)lo << (56 - pb), ptab, 0, 4);
+	u64 tid = blockIdx.x;
+	if(tid >= P.ntiles) return;  // uniform
+	V v[16];
+	qft_block_load<V, C>(v, a, tid, pb, f4, lo);
+	for(;;) {
+		// ---- stage A: registers index block bits 7..4
+#pragma unroll
+		for(int j = 7; j >= 4; --j) {
+			const int b = j - 4;
+			if constexpr(PART)
+				if(j < 8 - (int)P.K) continue;  // uniform
+#pragma unroll
+			for(int ra = 0; ra < 16; ++ra)
+				if(!(ra & (1 << b))) butterfly(v[ra], v[ra | (1 << b)]);
+#pragma unroll
+			for(int ra = 0; ra < 16; ++ra) {
+				if(!(ra & (1 << b))) continue;
+				v[ra] = rotp(v[ra], pa[(1 << b) - 1 + (ra & ((1 << b) - 1))]);
+			}
+		}
+		__syncthreads();  // previous tile's stage-B reads are complete
+#pragma unroll
+		for(int ra = 0; ra < 16; ++ra) {
+			const unsigned l = ((((unsigned)ra << 4) | f4) << C) | lo;
+			tile[l + (l >> (4 + C))] = v[ra];
+		}
+		__syncthreads();
+		const u64 nxt = tid + gridDim.x;
+		const bool more = nxt < P.ntiles;  // uniform
+		if constexpr(QftBlockWaves<R, C>::prefetch)
+			if(more) qft_block_load<V, C>(v, a, nxt, pb, f4, lo);
+		// ---- stage B: registers index block bits 3..0
+		V w[16];
+#pragma unroll
+		for(int rb = 0; rb < 16; ++rb) {
+			const unsigned l = (((f4 << 4) | (unsigned)rb) << C) | lo;
+			w[rb] = tile[l + (l >> (4 + C))];
+		}
+#pragma unroll
+		for(int j = 3; j >= 0; --j) {
+			if constexpr(PART)
+				if(j < 8 - (int)P.K) continue;  // uniform
+#pragma unroll
+			for(int rb = 0; rb < 16; ++rb)
+				if(!(rb & (1 << j))) butterfly(w[rb], w[rb | (1 << j)]);
+			if(j == 0) continue;  // c[0][0] = 1
+#pragma unroll
+			for(int rb = 0; rb < 16; ++rb) {
+				if(!(rb & (1 << j))) continue;
+				const int low = rb & ((1 << j) - 1);
+				if(low == 0) continue;
+				w[rb] = rotp(w[rb], cinp[(1 << j) - 1 + low]);
+			}
+		}
+		// ---- twiddle w^(r rev8(y)) / 16 for y = (f4, rb), r = rhi + lo:
+		// rev8(y) = 16 rev4(rb) + rev4(f4), so register rb takes
+		// base u^rev4(rb) with base = w^(r rev4(f4)) / 16 and u = w^(16 r).
+		const u64 x = tid << C;
+		const u64 rhi = x & lowmask;
+		double2 z[4];
+		z[0] = cmul(phase_of_range(rhi << (56 - pb), ptab, 0, 4), zlo);  // w^r
+#pragma unroll
+		for(int k = 1; k < 4; ++k) z[k] = csqr(z[k - 1]);
+		const double2 u = csqr(z[3]);
+		// A partial pass keeps the target bits of rev8(y) only (its low K bits):
+		// f4 bit b counts when 3 - b < K, and register rb takes u^(rev4(rb) & m2).
+		double2 ph = make_double2(PART ? P.scale : 0.0625, 0.0);
+#pragma unroll
+		for(int b = 0; b < 4; ++b)
+			if(((f4 >> b) & 1u) && (!PART || 3 - b < (int)P.K)) ph = cmul(ph, z[3 - b]);
+		const unsigned m2 = PART ? (P.K > 4 ? (1u << (P.K - 4)) - 1u : 0u) : 15u;
+		const double2 ph0 = ph;
+#pragma unroll
+		for(int k = 0; k < 16; ++k) {
+			const int rb = (int)(__brev((unsigned)k) >> 28);
+			if(PART && k && !(k & m2)) ph = ph0;
+			w[rb] = rot<V>(w[rb], ph);
+			if(k < 15) ph = cmul(ph, u);
+		}
+		const u64 base = ((x >> pb) << (pb + 8)) | rhi;
+		V *dst = a + (base + ((u64)f4 << (pb + 4)) + lo);
+#pragma unroll
+		for(int rb = 0; rb < 16; ++rb) nt_store(w[rb], dst + ((u64)rb << pb));
+		if(!more) break;
+		if constexpr(!QftBlockWaves<R, C>::prefetch) qft_block_load<V, C>(v, a, nxt, pb, f4, lo);
+		tid = nxt;
+	}
+}
+
+template <typename R, int C, bool PART>
+static hipError_t launch_qft_block_t(const Launch &L, void *a, unsigned n, unsigned pbot, unsigned K, const double *dphase,
+                                     const double *dcin) {
+	using V = typename V2T<R>::t;
+	QftBlockParams P;
+	P.n = n;
+	P.pbot = pbot;
+	P.K = K;
+	P.pad = 0;
+	P.scale = std::ldexp(K & 1u ? 1.0 / std::sqrt(2.0) : 1.0, -(int)(K / 2));  // (1/sqrt(2))^K, one rounding at most
+	P.ntiles = 1ull << (n - 8 - C);
+	// complex128: a grid of 1024 workgroups that loop over their tiles, so each
+	// next tile's loads overlap stage B (3% faster at n = 30 than one tile per
+	// workgroup); complex64 (no prefetch): at most 65536 workgroups, two tiles
+	// each at n = 30 (3% faster than one tile per workgroup).
+	const u64 cap = QftBlockWaves<R, C>::prefetch ? 1024 : 65536;
+	const unsigned grid = (unsigned)(P.ntiles < cap ? P.ntiles : cap);
+	const size_t shmem = (size_t)((256u << C) + 16u) * sizeof(V);
+	static bool attr = false;
+	if(!attr) {
+		(void)hipFuncSetAttribute((const void *)k_qft_block<R, C, PART>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+		attr = true;
+	}
+	using PT = typename LanePhase<V>::t;
+	const PT *cinp = sizeof(R) == 8 ? (const PT *)dcin : (const PT *)(dcin + 126);  // FP32 copy after the 63 FP64 constants
+	hipLaunchKernelGGL((k_qft_block<R, C, PART>), dim3(grid), dim3(16 << C), shmem, L.stream, (V *)a, P,
+	                   (const double2 *)dphase, (const double2 *)dcin, cinp);
+	return hipGetLastError();
+}
+
+template <typename R, int C>
+static hipError_t launch_qft_block_k(const Launch &L, void *a, unsigned n, unsigned pbot, unsigned K, const double *dphase,
+                                     const double *dcin) {
+	return K == 8 ? launch_qft_block_t<R, C, false>(L, a, n, pbot, 8, dphase, dcin)
+	              : launch_qft_block_t<R, C, true>(L, a, n, pbot, K, dphase, dcin);
+}
+
+hipError_t launch_qft_block(const Launch &L, void *a, unsigned n, unsigned pbot, const double *dphase, const double *dcin,
+                            unsigned C, unsigned K) {
+	if(pbot < C || pbot + 8 > n || K < 1 || K > 8) return hipErrorInvalidValue;
+	if(L.prec == 0) {
+		if(C == 4) return launch_qft_block_k<double, 4>(L, a, n, pbot, K, dphase, dcin);
+		if(C == 5) return launch_qft_block_k<double, 5>(L, a, n, pbot, K, dphase, dcin);
+	} else {
+		if(C == 5) return launch_qft_block_k<float, 5>(L, a, n, pbot, K, dphase, dcin);
+		if(C == 6) return launch_qft_block_k<float, 6>(L, a, n, pbot, K, dphase, dcin);
+	}
+	return hipErrorInvalidValue;
+}
+
+unsigned qft_block_coalesce_bits(int prec, int requested) {
+	if(prec == 0) return requested == 4 ? 4u : 5u;  // 5: 512 B runs per row (measured faster)
+	return requested == 6 ? 6u : 5u;
+}
+
+// ------------------------------------------------------------------------
+// QFT tail fused with the final bit reversal (makeQft's swap network).
+// The last six targets 5..0 only mix the low six bits, so the tail is a
+// 64-point transform of every contiguous run of 64 amplitudes; the reversal
+// then sends index (H, M, Lo) -> (rev6(Lo), rev(M), rev6(H)) for the top six
+// bits H, middle bits M and low six bits Lo.  Tile M = {*, M, *} (64 rows of
+// 64 amplitudes) goes, transposed, into the place of tile rev(M) and tile
+// rev(M) into the place of tile M: the work is a list of tile pairs
+// {M, rev(M)} exchanged in place.
+//
+// Pairs are enumerated directly, with no index skipped.  Split the m = n - 12
+// middle bits into a top half x (h = m / 2 bits), a middle bit c when m is
+// odd, and a bottom half y.  With u = x and w = rev_h(y), M = (u, c, rev_h(w))
+// and rev(M) = (w, c, rev_h(u)): the reversal swaps u and w, so the pairs are
+// the entries u <= w of a K x K triangle (K = 2^h) per value of c, u == w
+// being the palindromes.  Rows r and K-1-r of the triangle hold K+1 entries
+// together, which folds it into a (K/2) x (K+1) rectangle.
+struct QftTailParams {
+	u64 npairs;
+	uint32_t n, m, h, K;
+};
+
+__device__ __forceinline__ void tail_pair(const QftTailParams &P, u64 pair, u64 &M, u64 &Mr) {
+	unsigned p = (unsigned)pair, c = 0;  // pair < 2^29 for n <= 40
+	if(P.m & 1u) {
+		c = p & 1u;
+		p >>= 1;
+	}
+	unsigned u = 0, w = 0;
+	if(P.K > 1) {
+		const unsigned r = p / (P.K + 1), col = p % (P.K + 1);
+		if(col < P.K - r) {
+			u = r;
+			w = r + col;
+		} else {
+			u = P.K - 1 - r;
+			w = u + (col - (P.K - r));
+		}
+	}
+	const unsigned top = P.m - P.h;
+	const u64 revu = P.h ? (__brev(u) >> (32 - P.h)) : 0u, revw = P.h ? (__brev(w) >> (32 - P.h)) : 0u;
+	M = ((u64)u << top) | ((u64)c << P.h) | revw;
+	Mr = ((u64)w << top) | ((u64)c << P.h) | revu;
+}
+
+__device__ __forceinline__ unsigned rev6(unsigned x) { return __brev(x) >> 26; }
+// Lane layout of a 64 x 64 tile: LB column bits in lanes (a row is one
+// 128 B line per load instruction: 8 lanes of complex128, 16 of complex64),
+// the other 6 - LB column bits in registers, and each lane holds 16 / 2^(6-LB)
+// rows spaced 256 / 2^LB apart, 16 amplitudes in all.
+template <typename V> struct TailLayout {
+	static constexpr unsigned LB = sizeof(V) == 16 ? 3u : 4u;  // lane (column) bits
+	static constexpr unsigned RB = 6u - LB;                    // register (column) bits
+	static constexpr unsigned LPR = 1u << LB;                  // lanes per row
+	static constexpr unsigned RPL = 16u >> RB;                 // rows per lane
+	static constexpr unsigned RSTEP = 256u >> LB;              // row spacing
+};
+
+// LDS slot of (row, col) in a 64 x 64 tile: the column's lane bits are XORed
+// with row bits 5..6-LB, which spreads the transposed reads (rows rev6(c) for
+// 2^LB consecutive c: one 128 B bank sweep) over the banks; the row-wise
+// writes (2^LB lanes per row) are spread by the column itself.  The
+// register bits of the column are left alone, so a lane's slots in a row are
+// constant offsets from one address.
+template <typename V> __device__ __forceinline__ unsigned tail_slot(unsigned row, unsigned col) {
+	constexpr unsigned LB = TailLayout<V>::LB;
+	return row * 64u + (col ^ ((row >> (6u - LB)) & ((1u << LB) - 1u)));
+}
+
+// Value of lane (lane ^ MASK) within each row of 16 lanes, by DPP moves:
+// quad_perm [1,0,3,2] and [2,3,0,1] for MASK 1 and 2; for MASK 4,
+// row_half_mirror (lane ^ 7) followed by quad_perm [3,2,1,0] (lane ^ 3); for
+// MASK 8, row_ror:8.
+template <unsigned MASK> __device__ __forceinline__ int dpp_xor(int x) {
+	if constexpr(MASK == 1) return __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);
+	else if constexpr(MASK == 2) return __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);
+	else if constexpr(MASK == 8) return __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);  // row_ror:8
+	else {
+		static_assert(MASK == 4, "lane masks 1, 2, 4, 8");
+		return __builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(x, 0x141, 0xF, 0xF, false), 0x1B, 0xF, 0xF, false);
+	}
+}
+template <unsigned MASK> __device__ __forceinline__ double dpp_xor(double x) {
+	const long long b = __double_as_longlong(x);
+	const int lo = dpp_xor<MASK>((int)(unsigned)b), hi = dpp_xor<MASK>((int)(unsigned)(b >> 32));
+	return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+template <unsigned MASK> __device__ __forceinline__ float dpp_xor(float x) {
+	return __int_as_float(dpp_xor<MASK>(__float_as_int(x)));
+}
+
+// Unscaled butterfly over the target bit MASK of the lane index: lanes with
+// the bit clear keep x + y, their partners y - x, computed as y + (+-x), which
+// rounds identically (the sign flip is exact).
+__device__ __forceinline__ double flip_if(double x, unsigned f) {
+	return __longlong_as_double(__double_as_longlong(x) ^ ((long long)f << 63));
+}
+__device__ __forceinline__ float flip_if(float x, unsigned f) { return __int_as_float(__float_as_int(x) ^ (int)(f << 31)); }
+
+template <unsigned MASK, typename V> __device__ __forceinline__ V lane_bfly(V x, unsigned lane) {
+	const unsigned f = (lane & MASK) ? 1u : 0u;
+	return mk<V>(dpp_xor<MASK>(x.x) + flip_if(x.x, f), dpp_xor<MASK>(x.y) + flip_if(x.y, f));
+}
+
+// The 64-point tail transform of one row held by 2^LB lanes: registers are
+// column bits 5..LB, the lane's position q in its row is column bits LB-1..0.
+// The ladder factor e^{i pi low / 2^t} of a register target t splits into a
+// constant of the register index (uniform: scalar loads) and the lane's
+// qf[t - LB] = e^{i pi q / 2^t}; the lane targets LB-1..0 are DPP
+// butterflies, and target t > 0 then rotates its partners by
+// e^{i pi (q mod 2^t) / 2^t} (qf[RB + LB - 1 - t]; 1 on the other lanes, so
+// every lane rotates and nothing is selected).  The six Hadamard factors
+// 1/sqrt(2) are one exact scale by 1/8.
+template <typename V, unsigned MASK> __device__ __forceinline__ V tail_lane_step(V x, unsigned lane, const double2 *qf) {
+	constexpr unsigned LB = TailLayout<V>::LB, RB = TailLayout<V>::RB;
+	x = lane_bfly<MASK>(x, lane);
+	if constexpr(MASK > 1) x = rot<V>(x, qf[RB + LB - 1 - (31 - __builtin_clz(MASK))]);
+	return x;
+}
+
+template <typename V>
+__device__ __forceinline__ void tail_transform(V *v, unsigned lane, const double2 *qf, const double2 *__restrict__ cq) {
+	constexpr int LB = (int)TailLayout<V>::LB, RB = (int)TailLayout<V>::RB, NR = 1 << RB;
+#pragma unroll
+	for(int tt = 5; tt >= LB; --tt) {
+		const int b = tt - LB;
+#pragma unroll
+		for(int ra = 0; ra < NR; ++ra)
+			if(!(ra & (1 << b))) butterfly(v[ra], v[ra | (1 << b)]);
+#pragma unroll
+		for(int ra = 0; ra < NR; ++ra) {
+			if(!(ra & (1 << b))) continue;
+			const int hi = ra & ((1 << b) - 1);  // register bits below the target
+			v[ra] = rot<V>(v[ra], hi ? cmul(cq[(1 << tt) - 1 + (hi << LB)], qf[b]) : qf[b]);
+		}
+	}
+#pragma unroll
+	for(int ra = 0; ra < NR; ++ra) {
+		V x = v[ra];
+		if constexpr(LB == 4) x = tail_lane_step<V, 8>(x, lane, qf);
+		x = tail_lane_step<V, 4>(x, lane, qf);
+		x = tail_lane_step<V, 2>(x, lane, qf);
+		x = tail_lane_step<V, 1>(x, lane, qf);
+		v[ra] = mk<V>(x.x * 0.125f, x.y * 0.125f);
+	}
+}
+
+// Transposed, bit-reversed store of the tile staged in LDS into tile `dst`
+// (workgroups of 256 lanes, 16 amplitudes per lane).  Lane t writes column
+// c = t mod 64 of output rows 4j + t/64; the output row r comes from LDS
+// column rev6(r) of row rev6(c).
+template <typename V>
+__device__ __forceinline__ void tail_store(const V *tile, V *__restrict__ a, unsigned sh, u64 dst) {
+	constexpr unsigned LB = TailLayout<V>::LB;
+	const unsigned w = threadIdx.x >> 6, c = threadIdx.x & 63u, rowp = rev6(c);
+	const V *src = tile + rowp * 64u + rev6(w);  // + (rev6(4j) ^ g): LDS column rev6(4j + w) ^ g
+	const unsigned g = (rowp >> (6u - LB)) & ((1u << LB) - 1u);
+	V *out = a + (((u64)w << sh) + (dst << 6) + c);
+	const u64 step = (u64)4 << sh;
+#pragma unroll
+	for(unsigned j = 0; j < 16; ++j) {
+		nt_store(src[(__brev(4u * j) >> 26) ^ g], out + j * step);
+		if((j & 3u) == 3u) __builtin_amdgcn_sched_barrier(0);  // at most four reads in flight: registers
+	}
+}
+
+// Row-wise writes of a lane's rows rp + RSTEP s into the LDS tile: register
+// ra of row s is column (ra << LB) | q.
+template <typename V>
+__device__ __forceinline__ void tail_stage(V *tile, const V *v, unsigned rp, unsigned q) {
+	using Lay = TailLayout<V>;
+#pragma unroll
+	for(unsigned s = 0; s < Lay::RPL; ++s) {
+		V *dst = tile + tail_slot<V>(rp + Lay::RSTEP * s, q);
+#pragma unroll
+		for(unsigned ra = 0; ra < (1u << Lay::RB); ++ra) dst[ra * Lay::LPR] = v[(s << Lay::RB) + ra];
+	}
+}
+
+template <typename V>
+__device__ __forceinline__ void tail_load(V *v, const V *__restrict__ a, unsigned rp, unsigned q, unsigned sh, u64 mm) {
+	using Lay = TailLayout<V>;
+#pragma unroll
+	for(unsigned s = 0; s < Lay::RPL; ++s) {
+		const V *src = a + ((((u64)(rp + Lay::RSTEP * s)) << sh) + (mm << 6) + q);
+#pragma unroll
+		for(unsigned ra = 0; ra < (1u << Lay::RB); ++ra) v[(s << Lay::RB) + ra] = nt_load(src + ra * Lay::LPR);
+	}
+}
+
+// Waves per SIMD the tail kernel is compiled for (one wave of each workgroup
+// per SIMD): its LDS tile (64 KiB complex128, 32 KiB complex64) allows two
+// workgroups per CU, or more for complex64, where registers allow three.
+template <typename R> struct QftTailWaves {
+	static constexpr int value = sizeof(R) == 8 ? 2 : 3;
+};
+
+// Workgroups of 256 lanes in the layout of TailLayout: every load instruction
+// reads whole 128 B lines.  Both tiles of a pair are read before
+// either is written; each is transformed in registers and staged through one
+// 64 x 64 LDS tile for its transposed store.  When a workgroup has several
+// pairs (complex64), the loads of its next pair are issued as soon as the
+// registers they replace are staged, so they are in flight while the current
+// pair is stored.
+template <typename R>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(QftTailWaves<R>::value)))
+k_qft_tail_reverse(typename V2T<R>::t *__restrict__ a, QftTailParams P, const double2 *__restrict__ cq) {
+	using V = typename V2T<R>::t;
+	extern __shared__ char smem[];
+	V *tile = (V *)smem;  // 64 x 64
+	const unsigned t = threadIdx.x;
+	using Lay = TailLayout<V>;
+	constexpr unsigned LB = Lay::LB, RB = Lay::RB;
+	const unsigned lane = t & 63u, q = t & (Lay::LPR - 1u), rp = t >> LB;
+	const unsigned sh = P.n - 6;
+	// qf[b] = e^{i pi q / 2^(LB + b)} for the register targets; then, for lane
+	// targets t = LB-1 .. 1, e^{i pi (q mod 2^t) / 2^t} on lanes with bit t set.
+	double2 qf[5];
+#pragma unroll
+	for(unsigned b = 0; b < RB; ++b) qf[b] = cq[(1u << (LB + b)) - 1u + q];
+#pragma unroll
+	for(unsigned tt = LB - 1; tt >= 1; --tt) {
+		const unsigned low = q & ((1u << tt) - 1u);
+		qf[RB + LB - 1 - tt] = ((q >> tt) & 1u) ? cq[(1u << tt) - 1u + low] : make_double2(1.0, 0.0);
+	}
+	u64 p = blockIdx.x;
+	if(p >= P.npairs) return;  // uniform
+	u64 M, Mr;
+	tail_pair(P, p, M, Mr);
+	V va[16], vb[16];
+	tail_load(va, a, rp, q, sh, M);
+	tail_load(vb, a, rp, q, sh, Mr);
+	for(;;) {
+		const u64 pn = p + gridDim.x;
+		const bool more = pn < P.npairs;
+		u64 Mn = 0, Mnr = 0;
+		if(more) tail_pair(P, pn, Mn, Mnr);
+#pragma unroll
+		for(unsigned s = 0; s < Lay::RPL; ++s) {
+			tail_transform(va + (s << RB), lane, qf, cq);
+			tail_transform(vb + (s << RB), lane, qf, cq);
+		}
+		__syncthreads();  // the previous pair's transposed reads are done
+		tail_stage(tile, va, rp, q);
+		__syncthreads();
+		if(more) tail_load(va, a, rp, q, sh, Mn);
+		tail_store(tile, a, sh, Mr);  // tile M into the place of tile rev(M)
+		__syncthreads();
+		tail_stage(tile, vb, rp, q);
+		__syncthreads();
+		if(more) tail_load(vb, a, rp, q, sh, Mnr);
+		tail_store(tile, a, sh, M);  // palindromes (M == rev(M)) write the same values twice
+		if(!more) break;
+		p = pn;
+		M = Mn;
+		Mr = Mnr;
+	}
+}
+
+template <typename R>
+static hipError_t launch_qft_tail_t(const Launch &L, void *a, unsigned n, const double *dcq) {
+	using V = typename V2T<R>::t;
+	QftTailParams P;
+	P.n = n;
+	P.m = n - 12;
+	P.h = P.m / 2;
+	P.K = 1u << P.h;
+	P.npairs = (u64)((P.m & 1u) ? 2 : 1) * ((u64)P.K * (P.K + 1) / 2);
+	const size_t shmem = 4096 * sizeof(V);
+	// Measured at n = 30: complex128 runs fastest with one pair per workgroup
+	// (6.50 ms against 6.80 ms at 16 pairs each), complex64 with about three
+	// (3.34 ms at a grid of 49152 against 3.44 ms at one pair each).
+	const u64 cap = sizeof(R) == 8 ? P.npairs : 49152;
+	const unsigned grid = (unsigned)(P.npairs < cap ? P.npairs : cap);
+	static bool attr = false;
+	if(!attr) {
+		(void)hipFuncSetAttribute((const void *)k_qft_tail_reverse<R>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+		attr = true;
+	}
+	hipLaunchKernelGGL(k_qft_tail_reverse<R>, dim3(grid), dim3(256), shmem, L.stream, (V *)a, P, (const double2 *)dcq);
+	return hipGetLastError();
+}
+
+hipError_t launch_qft_tail_reverse(const Launch &L, void *a, unsigned n, const double *dcq) {
+	if(n < 12 || n > 40) return hipErrorInvalidValue;
+	return L.prec == 0 ? launch_qft_tail_t<double>(L, a, n, dcq) : launch_qft_tail_t<float>(L, a, n, dcq);
+}
+
+}  // namespace tmfqs_hip

@@ -1,0 +1,10 @@
+// Tiled-pass kernels: complex128.
+#include "tile_device.h"
+
+namespace tmfqs_hip {
+
+TMFQS_TILE_ENTRY(launch_tile_c128) {
+	return launch_staged_w<double, false, false>(L, W, a, b, P, dops, dstages, nstages, dmats, dterms, dphase, gpart, nslots);
+}
+
+}  // namespace tmfqs_hip

@@ -1,0 +1,10 @@
+// Tiled-pass kernels: complex128, dual (adjoint) form.
+#include "tile_device.h"
+
+namespace tmfqs_hip {
+
+TMFQS_TILE_ENTRY(launch_tile_c128_dual) {
+	return launch_staged_w<double, true, false>(L, W, a, b, P, dops, dstages, nstages, dmats, dterms, dphase, gpart, nslots);
+}
+
+}  // namespace tmfqs_hip

@@ -1,0 +1,10 @@
+// Tiled-pass kernels: complex128, dual (adjoint) form, with k-qubit matrix stages on the LDS tile.
+#include "tile_device.h"
+
+namespace tmfqs_hip {
+
+TMFQS_TILE_ENTRY(launch_tile_c128_dual_lds) {
+	return launch_staged_w<double, true, true>(L, W, a, b, P, dops, dstages, nstages, dmats, dterms, dphase, gpart, nslots);
+}
+
+}  // namespace tmfqs_hip
