@@ -92,11 +92,18 @@ void orc_random_phase(unsigned n, uint32_t seed, double *out) {
 /* backend_validation.h:37-39: qubit q -> mask 1 << (n-q-1). */
 static uint64_t orc_mask(unsigned q, unsigned n) { return (uint64_t)1 << (n - q - 1u); }
 
-/* Pair traversal, pair_kernel_executor.h:42-50: for each base step 2m, each
- * offset < m, pair (base+offset, base+offset+m). */
-#define ORC_PAIR_LOOP(N, M, S0, S1)                               \
-	for(uint64_t base_ = 0; base_ < (N); base_ += 2 * (M))        \
-		for(uint64_t S0 = base_, S1 = base_ + (M); S0 < base_ + (M); ++S0, ++S1)
+/* Pair traversal, pair_kernel_executor.h:42-50: the pairs (s, s + m) with
+ * bit m of s clear.  Pairs are disjoint and each is computed on its own, so
+ * visiting them in any order (here: split over OpenMP threads for states of
+ * 2^16 amplitudes and more) gives the same bits as the reference's
+ * base-then-offset order.  The one-trip inner loop lets a body `continue`. */
+#define ORC_PAR_MIN ((uint64_t)1 << 16)
+#define ORC_PAIR_LOOP(N, M, S0, S1)                                                                      \
+	_Pragma("omp parallel for schedule(static) if((N) >= ORC_PAR_MIN)")                                  \
+	for(int64_t t_ = 0; t_ < (int64_t)((N) >> 1); ++t_)                                                  \
+		for(uint64_t S0 = (((uint64_t)t_ & ~((M) - 1u)) << 1) | ((uint64_t)t_ & ((M) - 1u)), S1 = S0 + (M), \
+		             once_ = 1;                                                                          \
+		    once_; once_ = 0)
 
 /* dense_state_backend.cpp:255-273 (zero pairs skipped). */
 int orc_hadamard(double *a, unsigned n, unsigned qubit) {
@@ -222,19 +229,25 @@ int orc_apply_gate(double *a, unsigned n, const double *mat, unsigned k, const u
 	for(unsigned bit = 0; bit < n; ++bit)
 		if(!isTarget[bit]) nonTarget[nNon++] = bit;
 	uint64_t *offsets = (uint64_t *)malloc(sizeof(uint64_t) * dim);
-	double *loc = (double *)malloc(sizeof(double) * 2 * dim);
-	uint64_t *idx = (uint64_t *)malloc(sizeof(uint64_t) * dim);
-	if(!offsets || !loc || !idx) {
-		free(offsets); free(loc); free(idx);
-		return ORC_RUNTIME_ERROR;
-	}
+	if(!offsets) return ORC_RUNTIME_ERROR;
 	for(uint64_t r = 0; r < dim; ++r) {
 		uint64_t off = 0;
 		for(unsigned bit = 0; bit < k; ++bit)
 			if((r >> (k - bit - 1u)) & 1u) off |= tmask[bit];
 		offsets[r] = off;
 	}
-	for(uint64_t block = 0; block < numBlocks; ++block) {
+	int failed = 0;
+	/* Blocks are disjoint and computed on their own: any visiting order (here
+	 * split over threads) gives the reference's bits. */
+#pragma omp parallel if((uint64_t)1 << n >= ORC_PAR_MIN) reduction(| : failed)
+	{
+	double *loc = (double *)malloc(sizeof(double) * 2 * dim);
+	uint64_t *idx = (uint64_t *)malloc(sizeof(uint64_t) * dim);
+	if(!loc || !idx) failed = 1;
+#pragma omp for schedule(static)
+	for(int64_t block_ = 0; block_ < (int64_t)numBlocks; ++block_) {
+		const uint64_t block = (uint64_t)block_;
+		if(!loc || !idx) continue;
 		uint64_t baseState = 0;
 		for(unsigned j = 0; j < nNon; ++j) baseState |= ((block >> j) & 1u) << nonTarget[j];
 		if(k == 1) {
@@ -273,10 +286,11 @@ int orc_apply_gate(double *a, unsigned n, const double *mat, unsigned k, const u
 			a[2 * idx[r] + 1] = ni;
 		}
 	}
-	free(offsets);
 	free(loc);
 	free(idx);
-	return ORC_OK;
+	}
+	free(offsets);
+	return failed ? ORC_RUNTIME_ERROR : ORC_OK;
 }
 
 /* dense_state_backend.cpp:205-210. */
@@ -290,6 +304,7 @@ int orc_phase_flip(double *a, unsigned n, uint64_t state) {
 /* dense_state_backend.cpp:230-240: a = 2*mean - a. */
 int orc_inversion_about_mean_value(double *a, unsigned n, double mean_re, double mean_im) {
 	const uint64_t N = (uint64_t)1 << n;
+#pragma omp parallel for schedule(static) if(N >= ORC_PAR_MIN)
 	for(uint64_t s = 0; s < N; ++s) {
 		const double re = a[2 * s], im = a[2 * s + 1];
 		a[2 * s] = 2.0 * mean_re - re;
@@ -312,6 +327,7 @@ int orc_inversion_about_mean(double *a, unsigned n) {
 /* dense_state_backend.cpp:242-252: a = scale*a + bias. */
 int orc_affine(double *a, unsigned n, double s_re, double s_im, double b_re, double b_im) {
 	const uint64_t N = (uint64_t)1 << n;
+#pragma omp parallel for schedule(static) if(N >= ORC_PAR_MIN)
 	for(uint64_t s = 0; s < N; ++s) {
 		const double re = a[2 * s], im = a[2 * s + 1];
 		a[2 * s] = s_re * re - s_im * im + b_re;

@@ -294,6 +294,37 @@ def test_sample_boundaries_exact():
             assert s.sample(float(r)) == O.sample(init, n, float(r))
 
 
+@pytest.mark.parametrize("n,basis", [(13, 5000), (14, 12288), (5, 31)])
+def test_sample_zero_draw_before_leading_zero_chunks(n, basis):
+    # rnd == 0 is a legal draw: the sequential chain stops at state 0, whose
+    # cumulative probability 0 already satisfies rnd <= cum, even when whole
+    # leading chunks of the state hold zero probability
+    s = st().init_basis(n, basis)
+    amps = s.export()
+    want = O.sample(amps, n, 0.0)
+    assert want == 0
+    assert s.sample(0.0) == want
+    oreg = O.Register(n, amps)
+    oreg.inversion(False, complex(0.0, 0.0))  # overlay scale -1, bias 0
+    assert s.sample_affine(0.0, complex(-1, 0), complex(0, 0)) == oreg.measure(0.0) == 0
+    # a later draw still lands on the basis state
+    assert s.sample(0.5) == O.sample(amps, n, 0.5) == basis
+
+
+@pytest.mark.parametrize("fusion", [0, 1])
+def test_complex64_amplitude_reads_match_storage(fusion):
+    n = 6
+    s = st(T.C64, fusion=fusion).init_basis(n, 0)
+    s.set_amplitude(5, 0.1 - 0.3j)
+    want = complex(np.float32(0.1), np.float32(-0.3))
+    assert s.amplitude(5) == want
+    assert s.probability(5) == want.real * want.real + want.imag * want.imag
+    s.phase_flip(5)
+    assert s.amplitude(5) == -want
+    got = s.export()
+    assert complex(got[10], got[11]) == s.amplitude(5)
+
+
 def test_affine_overlay_reads():
     n = 8
     init = rand_state(n, 4)

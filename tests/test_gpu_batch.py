@@ -1,20 +1,17 @@
 """Batched circuits (tmfqs_hip_batch_*) against the oracle and against the
-engine's own one-circuit-at-a-time path."""
+engine's own one-circuit-at-a-time path.  The seed-42 workload itself is
+checked against the oracle in tests/test_gpu_configs.py."""
 
 import ctypes
-import os
 
 import numpy as np
 import pytest
 
 import oracle as O
 import tmfqsfullstate_amd as T
-from tmfqsfullstate_amd.batch import Batch, load_circuit_list
+from tmfqsfullstate_amd.batch import Batch
 
 pytestmark = pytest.mark.gpu
-
-PATH = os.path.join(os.path.dirname(__file__), "golden", "batch_circuits_seed42.txt")
-
 
 def oracle_ops(ops, cnt):
     arr = (O.Op * max(1, cnt))()
@@ -84,18 +81,6 @@ def test_graph_replays_are_stable():
             b.run()
             for i in range(len(circuits)):
                 assert np.array_equal(b.export(i), first[i])
-
-
-def test_seed42_workload_vs_sequential_runs():
-    circuits = load_circuit_list(PATH)
-    print("seed-42 workload: 256 circuits", flush=True)
-    with Batch(circuits) as b:
-        secs = b.run()
-        info = b.info()
-        assert info["circuits"] == 256 and info["graph"] == 1 and secs > 0
-        for i, (n, ops, cnt) in enumerate(circuits):
-            if n <= 16 or i % 16 == 0:
-                assert np.array_equal(b.export(i), sequential(n, ops, cnt)), f"circuit {i} (n={n})"
 
 
 def test_tuned_batch_stays_bitexact():

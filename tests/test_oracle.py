@@ -287,3 +287,35 @@ def test_oracle_bitexact_vs_reference_build():
         init = O.random_phase(n, trial)
         got = O.run_circuit(n, seq, init=init).amplitudes()
         assert np.array_equal(got, O.ref_run(n, seq, init=init)[0])
+
+
+@needs_ref
+def test_threaded_oracle_loops_bitexact_vs_reference_build():
+    # states of 2^16 amplitudes and more run the oracle's pair, block and
+    # elementwise loops split over threads: the bits must not change
+    n = 17
+    rng = np.random.default_rng(1717)
+    mats, quals, seq = [], [], []
+    for trial in range(60):
+        kind = int(rng.integers(0, 7))
+        q0, q1, q2 = (int(x) for x in rng.choice(n, 3, replace=False))
+        if kind == 5:
+            seq.append((O.OP_DIFFUSE_MAT,))
+        elif kind == 6:
+            k = int(rng.integers(1, 4))
+            qs = [int(x) for x in rng.choice(n, k, replace=False)]
+            d = 1 << k
+            m = rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d))
+            u, _ = np.linalg.qr(m)
+            seq.append((O.OP_MATRIX, len(quals), 0, 0.0, sum(x.size for x in mats), k))
+            mats.append(O.to_interleaved(u.reshape(-1)))
+            quals.extend(qs)
+        else:
+            seq.append((kind, q0, q1, float(rng.uniform(0, 2 * np.pi))))
+    pool = np.ascontiguousarray(np.concatenate(mats), dtype=np.float64)
+    qpool = np.ascontiguousarray(quals, dtype=np.uint32)
+    init = O.random_phase(n, 17)
+    got = O.run_circuit(n, seq, init=init, mat_pool=pool, mat_qubits=qpool).amplitudes()
+    want = O.ref_run(n, seq, init=init, mat_pool=pool, mat_qubits=qpool)[0]
+    assert sum(t[0] == O.OP_MATRIX for t in seq) >= 5 and sum(t[0] == O.OP_DIFFUSE_MAT for t in seq) >= 5
+    assert np.array_equal(got, want)

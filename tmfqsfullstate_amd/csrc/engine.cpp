@@ -587,6 +587,18 @@ Cplx read_device(tmfqs_hip_state *h, u64 s) {
 	return v;
 }
 
+// A value as the state's storage holds it: complex64 keeps FP32 parts.
+Cplx storage_value(const tmfqs_hip_state *h, Cplx v) {
+	if(h->prec == 1) {
+		v.re = (double)(float)v.re;
+		v.im = (double)(float)v.im;
+	}
+	return v;
+}
+
+// Tracked values replay pending elementwise steps in FP64 (complex64 rounds
+// them when a flush stores them); queries report them through storage_value,
+// as the storage holds them once the steps are flushed.
 Cplx current_value(tmfqs_hip_state *h, u64 s) {
 	auto it = h->tracked.find(s);
 	if(it != h->tracked.end()) return it->second;
@@ -1701,7 +1713,17 @@ ChainResult chain_walk(tmfqs_hip_state *h, int sel, bool find, double rnd, const
 	double S = 0.0;
 	for(u64 c = 0; c < nch; ++c) {
 		const ChunkTicks &t = tk[c];
-		if(!t.nonzero) continue;  // every value is +-0: S unchanged
+		if(!t.nonzero) {  // every value is +-0: S unchanged
+			// the chunk's first state already satisfies rnd <= S (rnd == 0 on
+			// a leading run of zero probabilities): the reference stops there
+			if(find && rnd <= S) {
+				r.found = true;
+				r.index = c * chunk;
+				r.total = S;
+				return r;
+			}
+			continue;
+		}
 		bool fast = false;
 		if(ulog[c] != INT32_MIN && !t.tie && std::fabs(S) > 0x1p-1000 && std::ilogb(S) - 52 == ulog[c]) {
 			const long long K = (long long)std::ldexp(S, -ulog[c]);  // exact integer, |K| < 2^53
@@ -2089,7 +2111,7 @@ int tmfqs_hip_amplitude(tmfqs_hip_state *h, uint64_t state, double *re, double *
 	require_init(h, "amplitude query");
 	validate_state(h, state, "HipStateBackend::amplitude");
 	HIPC(hipSetDevice(h->cfg.device));
-	const Cplx v = current_value(h, state);
+	const Cplx v = storage_value(h, current_value(h, state));
 	*re = v.re;
 	*im = v.im;
 	ABI_END
@@ -2101,13 +2123,14 @@ int tmfqs_hip_set_amplitude(tmfqs_hip_state *h, uint64_t state, double re, doubl
 	require_init(h, "state update");
 	validate_state(h, state, "HipStateBackend::setAmplitude");
 	HIPC(hipSetDevice(h->cfg.device));
+	const Cplx v = storage_value(h, Cplx{re, im});
 	if(!h->cfg.fusion) {
 		sync_all(h);
-		device_set(h, state, re, im);
-		h->tracked[state] = Cplx{re, im};
+		device_set(h, state, v.re, v.im);
+		h->tracked[state] = v;
 		return TMFQS_HIP_OK;
 	}
-	h->tracked[state] = Cplx{re, im};
+	h->tracked[state] = v;
 	mark_dirty(h, state);
 	QOp op;
 	op.kind = Q_SET;
@@ -2122,7 +2145,7 @@ int tmfqs_hip_probability(tmfqs_hip_state *h, uint64_t state, double *out) {
 	require_init(h, "amplitude query");
 	validate_state(h, state, "HipStateBackend::amplitude");
 	HIPC(hipSetDevice(h->cfg.device));
-	const Cplx v = current_value(h, state);
+	const Cplx v = storage_value(h, current_value(h, state));
 	*out = v.re * v.re + v.im * v.im;
 	ABI_END
 }
