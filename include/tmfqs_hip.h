@@ -244,6 +244,19 @@ typedef struct tmfqs_hip_diag_term {
 /* <psi|O|psi> of the current state (FP64 accumulation in both precisions). */
 int tmfqs_hip_expectation(tmfqs_hip_state *h, const tmfqs_hip_diag_term *terms, size_t nterms, double *out);
 
+/* No counterpart in the reference.  For callers that cannot allocate (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc):
+ * tmfqs_hip_expectation over a state in the caller's device memory (2^n
+ * interleaved complex128 or complex64 amplitudes at amps), enqueued on the
+ * caller's stream (a hipStream_t; null = the default stream) of device
+ * `device`; *out (device memory, one double) receives <psi|O|psi>, the same
+ * bits as tmfqs_hip_expectation.  Returns without waiting; allocates
+ * nothing: its workgroup partials live in a fixed 8 KiB device array per GPU
+ * (any n <= 63 fits).  Calls that overlap on one GPU share that array and
+ * race; order them on one stream. */
+int tmfqs_hip_expectation_device(int device, void *stream, int precision, const void *amps, unsigned n,
+                                 const tmfqs_hip_diag_term *terms, size_t nterms, double *out);
+
 /* Adjoint gradient: runs the gate list (H, X, CX, CPHASE, PHASE, SWAP,
  * MATRIX) forward from the handle's current state, leaving the final state in
  * the handle, and returns E = <psi|O|psi> and dE/dtheta for every CPHASE and

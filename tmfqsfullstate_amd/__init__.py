@@ -170,6 +170,8 @@ def lib():
         L.tmfqs_hip_qubit_count.argtypes = [vp, ctypes.POINTER(ctypes.c_uint)]
         L.tmfqs_hip_run.argtypes = [vp, ctypes.POINTER(Op), ctypes.c_size_t, _dp, ctypes.POINTER(ctypes.c_uint)]
         L.tmfqs_hip_expectation.argtypes = [vp, ctypes.POINTER(DiagTerm), ctypes.c_size_t, _dp]
+        L.tmfqs_hip_expectation_device.argtypes = [ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_uint,
+                                                   ctypes.POINTER(DiagTerm), ctypes.c_size_t, vp]
         L.tmfqs_hip_adjoint.argtypes = [vp, ctypes.POINTER(Op), ctypes.c_size_t, _dp, ctypes.POINTER(ctypes.c_uint),
                                         ctypes.POINTER(DiagTerm), ctypes.c_size_t, ctypes.c_uint, _dp, _dp,
                                         ctypes.c_size_t]
@@ -195,6 +197,15 @@ def default_config(**kw):
     for k, v in kw.items():
         setattr(cfg, k, int(v))
     return cfg
+
+
+def expectation_device(amps_ptr, n, terms, out_ptr, precision=C128, device=0, stream=None):
+    """tmfqs_hip_expectation_device: <psi|O|psi> of the 2^n amplitudes at the
+    device address amps_ptr into the device double at out_ptr, enqueued on
+    `stream` (a hipStream_t as an integer; None = the default stream) without
+    allocating or waiting."""
+    arr, cnt = make_terms(terms)
+    check(lib().tmfqs_hip_expectation_device(device, stream, precision, amps_ptr, n, arr, cnt, out_ptr))
 
 
 class HipState:

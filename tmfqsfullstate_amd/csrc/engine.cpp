@@ -1910,6 +1910,15 @@ void destroy_handle(tmfqs_hip_state *h) {
 
 }  // namespace
 
+// Helpers other translation units of the library share (engine.h).
+namespace tmfqs_hip {
+ObsParams observable_params(unsigned n, const tmfqs_hip_diag_term *terms, size_t nterms) {
+	return make_observable(n, terms, nterms);
+}
+void check_device(hipError_t e, const char *what) { hip_check(e, what); }
+int status_of_current_exception() { return map_exception(); }
+}  // namespace tmfqs_hip
+
 // =========================================================================
 // extern "C" ABI
 static void throw_status(int rc) {

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "tmfqs_hip.h"
+
 #include <cstddef>
 #include <cstdint>
 
@@ -174,6 +176,15 @@ struct ObsParams {
 // dpart holds kReduceBlocks doubles.
 hipError_t launch_observable(const Launch &L, const void *a, void *lam, unsigned n, const ObsParams &O, double *dpart,
                              double *dout);
+// The observable of a C-ABI term list on n qubits (throws std::invalid_argument
+// / std::out_of_range as tmfqs_hip_expectation reports them).
+ObsParams observable_params(unsigned n, const tmfqs_hip_diag_term *terms, size_t nterms);
+// Throws the library's exception for a failed HIP call (std::bad_alloc when
+// out of memory).
+void check_device(hipError_t e, const char *what);
+// Inside a catch block: records the message for tmfqs_hip_last_error and
+// returns the matching TMFQS_HIP_ERR_* status.
+int status_of_current_exception();
 // Reductions write `blocks` partials into dpart, then reduce into dout
 // (sum: 2 doubles; norm: 1 double).
 constexpr unsigned kReduceBlocks = 1024;
