@@ -65,11 +65,30 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
+def kernel_sources_sha():
+    """SHA-256 (first 16 hex digits) of the HIP kernel sources."""
+    import hashlib
+
+    csrc = os.path.join(ROOT, "tmfqsfullstate_amd", "csrc")
+    h = hashlib.sha256()
+    for f in sorted(os.listdir(csrc)):
+        if f.endswith((".hip", ".h")):
+            h.update(f.encode())
+            h.update(open(os.path.join(csrc, f), "rb").read())
+    return h.hexdigest()[:16]
+
+
 def read_traffic(kernel, workload):
-    """HBM bytes per launch from the committed PMC summary, if present."""
+    """HBM bytes per launch from the committed PMC summary (two rocprofv3
+    --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel's
+    readings), or None when it is absent or was measured on other kernel
+    sources than these."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         d = json.load(open(path))
+        if d.get("kernel_sources_sha") != kernel_sources_sha():
+            log("profiles/traffic.json was measured on other kernel sources: traffic reported as null")
+            return None
         return d[workload][kernel]["bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         return None
@@ -157,6 +176,13 @@ def time_adjoint(T, S, precision, n=24, layers=8, seed=3, steps=3):
         "max_abs_diff_vs_central_fd": float(np.max(np.abs(g - fd))),
         "kernels": prof,
     })
+    if precision != T.C128:
+        # central differences at h = 1e-6 on FP32 amplitudes measure the FP32
+        # rounding of two energies (~1e-7 / 2e-6), not the gradient: the
+        # complex64 gradients are compared with the complex128 adjoint instead,
+        # and the 1e-9 agreement target applies to complex128
+        out["fd_on_fp32_state_not_an_accuracy_figure"] = out.pop("max_abs_diff_vs_central_fd")
+        out["param_shift_on_fp32_state"] = out.pop("max_abs_diff_vs_param_shift")
     return out, g
 
 
@@ -320,54 +346,81 @@ def time_batch(T, precision, steps=3):
     }
 
 
-def cpu_baseline(n):
-    """The reference's dense backend on this host: time each QFT-30 operation
-    type on a 2^n state and extrapolate to the full op mix."""
+def cpu_baseline(n, gpu_seconds):
+    """The reference's dense backend on this host, one thread (it has no
+    threads), against the GPU's times of the same workloads:
+
+    * QFT-n and Grover-n (minutes and hours on a CPU) from a bounded sample:
+      per-operation times of makeQft(n)'s operation kinds and of 17 Grover
+      iterations on one dense 2^n-state register, extrapolated by operation
+      count (every Grover iteration after the first is the same sweep; the
+      first also computes the cached amplitude sum);
+    * Grover-20 and QFT-26 run in full through sim::Simulator (its timed
+      region, as on the GPU).
+    """
     import oracle as O
 
-    kind = "reference" if O.ref_available() else "port"
-    timer = O.ref_time_ops if kind == "reference" else None
-    if timer is None:
-        O.lib().orc_time_ops.argtypes = [ctypes.c_uint, ctypes.POINTER(O.Op), ctypes.c_size_t,
-                                         ctypes.POINTER(ctypes.c_double)]
-
-        def timer(nq, seq):
-            ops, c = O.make_ops(seq)
-            s = ctypes.c_double()
-            rc = O.lib().orc_time_ops(nq, ops, c, ctypes.byref(s))
-            if rc:
-                raise RuntimeError(f"orc_time_ops failed {rc}")
-            return s.value
-    # about ten seconds of single-thread work at n = 30: operations of
-    # makeQft(n) on qubits spread over the register
+    reference = O.ref_available()
+    kind = "reference" if reference else "port"
+    marked = (1 << (n - 1)) - 1
     cp_pairs = [(1, 0), (5, 2), (9, 4), (13, 6), (17, 8), (21, 10), (25, 12), (29, 14), (29, 28), (20, 19), (27, 1),
                 (15, 3)]
-    samples = {
-        "h": [(O.OP_H, q % n) for q in (0, 10, 20, 29)],
-        "cphase": [(O.OP_CPHASE, c % n, t % n, math.pi / 2 ** ((c % n) - (t % n))) for c, t in cp_pairs
-                   if c % n > t % n],
-        "swap": [(O.OP_SWAP, a % n, b % n) for a, b in ((0, 29), (5, 24), (14, 15)) if a % n != b % n],
-    }
-    per = {}
-    total_s = 0.0
-    for name, seq in samples.items():
-        t = timer(n, seq)
-        total_s += t
-        per[name] = t / len(seq)
+    seq = [(O.OP_H, q % n) for q in (0, 10, 20, 29)]
+    seq += [(O.OP_CPHASE, c % n, t % n, math.pi / 2 ** ((c % n) - (t % n))) for c, t in cp_pairs if c % n > t % n]
+    seq += [(O.OP_SWAP, a % n, b % n) for a, b in ((0, 29), (5, 24), (14, 15)) if a % n != b % n]
+    nq = len(seq)
+    grover_iters_sampled = 17
+    seq += [(O.OP_PHASE_FLIP, 0, 0, 0.0, marked), (O.OP_DIFFUSE_MAT,)] * grover_iters_sampled
+    t0 = time.perf_counter()
+    total_s, per = O.time_ops_each(n, seq, reference)
+    kinds = [t[0] for t in seq[:nq]]
+    per_op = {name: float(np.mean([p for p, k in zip(per[:nq], kinds) if k == code]))
+              for name, code in (("h", O.OP_H), ("cphase", O.OP_CPHASE), ("swap", O.OP_SWAP))}
+    it = per[nq::2] + per[nq + 1::2]
+    first_iter, steady_iter = float(it[0]), float(np.mean(it[1:]))
     nh, ncp, nsw = n, n * (n - 1) // 2, n // 2
-    projected = nh * per["h"] + ncp * per["cphase"] + nsw * per["swap"]
+    qft_s = nh * per_op["h"] + ncp * per_op["cphase"] + nsw * per_op["swap"]
+    iters = O.grover_iterations(n, 1)
+    # the H layer of Grover starts from a basis state, where the reference
+    # skips all-zero pairs: the dense-state H time bounds it from above
+    grover_s = nh * per_op["h"] + first_iter + (iters - 1) * steady_iter
+    sample_s = time.perf_counter() - t0
+    workloads = {
+        f"qft{n}_c128": {"cpu_s": qft_s, "cpu_measured": False, "gpu_s": gpu_seconds.get("qft")},
+        f"grover{n}_c128": {"cpu_s": grover_s, "cpu_measured": False, "gpu_s": gpu_seconds.get("grover"),
+                            "iterations": iters, "cpu_sampled_iterations": grover_iters_sampled,
+                            "cpu_first_iteration_s": first_iter, "cpu_iteration_s": steady_iter},
+    }
+    full = {}
+    if reference:
+        from tmfqsfullstate_amd import sim as S
+        g20 = O.grover_ops(20, [524287], True)
+        full["grover20_c128"] = (O.ref_run(20, g20)[3], gpu_seconds.get("grover20"))
+        init26 = S.random_phase(26, 7)
+        full["qft26_c128"] = (O.ref_run(26, O.qft_ops(26), init=init26)[3], gpu_seconds.get("qft26"))
+        del init26
+    for name, (cs, gs) in full.items():
+        workloads[name] = {"cpu_s": cs, "cpu_measured": True, "gpu_s": gs}
+    for w in workloads.values():
+        w["speedup"] = w["cpu_s"] / w["gpu_s"] if w.get("gpu_s") else None
     return {
-        "value": 1.0 / projected,
+        "value": 1.0 / qft_s,
         "unit": f"QFT-{n} circuits/s",
         "cores": 1,
         "kind": kind,
-        "projected_qft_seconds": projected,
-        "per_op_seconds": per,
-        "sample": (f"{len(samples['h'])} H, {len(samples['cphase'])} controlled-phase and {len(samples['swap'])} swap "
-                   f"(3 CX each) operations of makeQft({n}) timed on a 2^{n}-state "
-                   f"complex128 register with uniform amplitudes ({total_s:.1f} s of CPU work), extrapolated "
-                   f"to the circuit's {nh} H + {ncp} controlled-phase + {nsw} swap operations; single thread, "
-                   f"{os.cpu_count()} host CPUs, {cpu_model()}"),
+        "projected_qft_seconds": qft_s,
+        "projected_grover_seconds": grover_s,
+        "per_op_seconds": per_op,
+        "workloads": workloads,
+        "sample": (f"one dense 2^{n}-state complex128 register (distinct real amplitudes, so no zero skips): "
+                   f"{len([k for k in kinds if k == O.OP_H])} H, {len([k for k in kinds if k == O.OP_CPHASE])} "
+                   f"controlled-phase and {len([k for k in kinds if k == O.OP_SWAP])} swap (3 CX each) operations "
+                   f"of makeQft({n}) and {grover_iters_sampled} Grover iterations (phase flip + materialized "
+                   f"diffusion) timed per operation ({total_s:.1f} s of CPU work), extrapolated to makeQft({n})'s "
+                   f"{nh} H + {ncp} controlled-phase + {nsw} swap operations and makeGrover({n})'s {nh} H + "
+                   f"{iters} iterations; Grover-20 and QFT-26 (randomPhase(26, 7)) run in full through "
+                   f"sim::Simulator" + ("" if reference else " (skipped: no reference build)") +
+                   f"; {sample_s:.0f} s in all; single thread, {os.cpu_count()} host CPUs, {cpu_model()}"),
     }
 
 
@@ -583,11 +636,24 @@ def main():
             bt["complex64"] = time_batch(T, T.C64)
         result["batch"] = bt
 
+    # ---- QFT-26 complex128 (the reference's largest QFT campaign size)
+    if not args.skip_cpu:
+        log("QFT-26 complex128")
+        r26, wall26, _, _, _ = time_qft(S, 26, T.C128, args.steps, args.warmup)
+        result["qft26"] = {"qubits": 26, "wall_s": wall26 / args.steps}
+        del r26
+
     cpu = None
     if not args.skip_cpu:
         log("CPU baseline (reference dense backend, one thread)")
-        cpu = cpu_baseline(n)
-        log(f"CPU: projected QFT-{n} {cpu['projected_qft_seconds']:.1f} s")
+        gpu_s = {"qft": ms_per_step / 1000.0,
+                 "grover": result.get("grover", {}).get("wall_s"),
+                 "grover20": result.get("grover20", {}).get("wall_s"),
+                 "qft26": result.get("qft26", {}).get("wall_s")}
+        cpu = cpu_baseline(n, gpu_s)
+        log(f"CPU: projected QFT-{n} {cpu['projected_qft_seconds']:.1f} s, Grover-{n} "
+            f"{cpu['projected_grover_seconds']:.0f} s; " +
+            ", ".join(f"{k} {v['cpu_s']:.2f} s ({(v['speedup'] or 0):.0f}x)" for k, v in cpu["workloads"].items()))
 
     value = 1000.0 / ms_per_step
     line = {
@@ -612,6 +678,9 @@ def main():
     line.update(result)
     if cpu is not None:
         line["speedup_vs_cpu_reference"] = value / cpu["value"]
+        g = cpu["workloads"].get(f"grover{n}_c128", {})
+        if g.get("speedup"):
+            line["grover_speedup_vs_cpu_reference"] = g["speedup"]
     print(json.dumps(line), flush=True)
 
 

@@ -139,6 +139,7 @@ def ref():
         R.ref_register_inversion.argtypes = [ctypes.c_uint, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_int, ctypes.c_double, _dp, _u64p, _dp]
         R.ref_time_ops.argtypes = [ctypes.c_uint, ctypes.POINTER(Op), ctypes.c_size_t, _dp]
+        R.ref_time_ops_each.argtypes = [ctypes.c_uint, ctypes.POINTER(Op), ctypes.c_size_t, _dp, _dp]
         R.ref_unit_draws.argtypes = [ctypes.c_uint32, ctypes.c_size_t, _dp]
         R.ref_random_phase.argtypes = [ctypes.c_uint, ctypes.c_uint, _dp]
         _ref = R
@@ -401,10 +402,23 @@ def ref_random_phase(n, seed):
 
 
 def ref_time_ops(n, seq_or_ops):
+    return time_ops_each(n, seq_or_ops, reference=True)[0]
+
+
+def time_ops_each(n, seq_or_ops, reference):
+    """Single-thread CPU time of ops on a uniform 2^n-state register: the
+    reference build (reference=True) or the restatement.  Returns (total
+    seconds, per-operation seconds)."""
     if isinstance(seq_or_ops, tuple):
         ops, cnt = seq_or_ops
     else:
         ops, cnt = make_ops(seq_or_ops)
     secs = ctypes.c_double()
-    _check(ref().ref_time_ops(ctypes.c_uint(n), ops, ctypes.c_size_t(cnt), ctypes.byref(secs)), "ref_time_ops")
-    return secs.value
+    per = np.zeros(max(1, cnt), dtype=np.float64)
+    if reference:
+        fn = ref().ref_time_ops_each
+    else:
+        fn = lib().orc_time_ops_each
+        fn.argtypes = [ctypes.c_uint, ctypes.POINTER(Op), ctypes.c_size_t, _dp, _dp]
+    _check(fn(ctypes.c_uint(n), ops, ctypes.c_size_t(cnt), ctypes.byref(secs), _dptr(per)), "time_ops")
+    return secs.value, per[:cnt]

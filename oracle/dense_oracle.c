@@ -783,25 +783,54 @@ long long orc_grover_ops(unsigned n, const uint64_t *marked, size_t m, int mater
 }
 
 /* ------------------------------------------------------------------------- */
-/* CPU-baseline timing of the restatement: ops applied to an n-qubit register
- * whose amplitudes all equal 2^(-n/2) (so no zero-skip fires); wall seconds of
- * the op loop only.  Used by the benchmark when the reference build is absent. */
+/* CPU-baseline timing of the restatement: ops applied to a dense n-qubit
+ * register, real amplitudes 2^(-n/2) (1 + d_i) with distinct small d_i (so no
+ * zero-skip or equal-pair skip fires, as on a random-phase input); wall
+ * seconds of the op loop only, and each operation's own seconds when per_op
+ * is not null.
+ * Runs on one thread, as the reference's dense backend does.  Used by the
+ * benchmark when the reference build is absent. */
+#include <omp.h>
 #include <time.h>
-int orc_time_ops(unsigned n, const orc_op *ops, size_t count, double *seconds) {
+static double orc_now(void) {
+	struct timespec t;
+	clock_gettime(CLOCK_MONOTONIC, &t);
+	return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+
+int orc_time_ops_each(unsigned n, const orc_op *ops, size_t count, double *seconds, double *per_op) {
 	orc_register *r = orc_reg_create(n);
 	if(!r) return ORC_RUNTIME_ERROR;
 	const double a = 1.0 / sqrt((double)r->N);
 	for(uint64_t s = 0; s < r->N; ++s) {
-		r->amp[2 * s] = a;
+		uint64_t x = s * 0x9E3779B97F4A7C15ull;
+		x ^= x >> 29;
+		x *= 0xBF58476D1CE4E5B9ull;
+		x ^= x >> 32;
+		r->amp[2 * s] = a * (1.0 + 0x1p-24 * (double)(x & 0xFFFFFull));
 		r->amp[2 * s + 1] = 0.0;
 	}
-	struct timespec t0, t1;
-	clock_gettime(CLOCK_MONOTONIC, &t0);
-	const int rc = orc_reg_run(r, ops, count, NULL, NULL);
-	clock_gettime(CLOCK_MONOTONIC, &t1);
-	*seconds = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+	const int threads = omp_get_max_threads();
+	omp_set_num_threads(1);
+	int rc = ORC_OK;
+	const double t0 = orc_now();
+	double prev = t0;
+	for(size_t i = 0; i < count && rc == ORC_OK; ++i) {
+		rc = orc_reg_run(r, ops + i, 1, NULL, NULL);
+		if(per_op) {
+			const double t = orc_now();
+			per_op[i] = t - prev;
+			prev = t;
+		}
+	}
+	*seconds = orc_now() - t0;
+	omp_set_num_threads(threads);
 	orc_reg_free(r);
 	return rc;
+}
+
+int orc_time_ops(unsigned n, const orc_op *ops, size_t count, double *seconds) {
+	return orc_time_ops_each(n, ops, count, seconds, NULL);
 }
 
 /* ------------------------------------------------------------------------- */

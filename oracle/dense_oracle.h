@@ -139,6 +139,7 @@ unsigned orc_grover_iterations(uint64_t state_count, size_t marked_count);
 long long orc_grover_ops(unsigned n, const uint64_t *marked, size_t m, int materialized, orc_op *out, size_t cap);
 /* Wall seconds of ops on a uniform-amplitude n-qubit register (CPU-baseline timing). */
 int orc_time_ops(unsigned n, const orc_op *ops, size_t count, double *seconds);
+int orc_time_ops_each(unsigned n, const orc_op *ops, size_t count, double *seconds, double *per_op);
 
 /* ---- diagonal observables and the adjoint benchmark circuit ---- */
 /* O = sum of Z-strings coef * prod_{q in bits} Z_q (Z_q = +1 when qubit q is 0)

@@ -8,6 +8,7 @@
 #include "tmfqsfs.h"
 
 #include <chrono>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
@@ -63,6 +64,15 @@ tmfqs::circuit::Circuit buildCircuit(unsigned n, const orc_op *ops, size_t count
 		}
 	}
 	return c;
+}
+
+// 20 pseudo-random bits of i (a 64-bit mix); only the timing states use it.
+inline uint64_t timing_jitter(uint64_t i) {
+	uint64_t x = i * 0x9E3779B97F4A7C15ull;
+	x ^= x >> 29;
+	x *= 0xBF58476D1CE4E5B9ull;
+	x ^= x >> 32;
+	return x & 0xFFFFFull;
 }
 
 } // namespace
@@ -158,10 +168,18 @@ int ref_register_inversion(unsigned n, const double *amps, int materialized, dou
 	}
 }
 
-// Times the reference dense backend on ops[0..count) over an n-qubit state
-// whose amplitudes are all 2^(-n/2) (no zero-skips fire).  Returns the
-// wall seconds of the op loop only.
+// Times the reference dense backend on ops[0..count) over a dense n-qubit
+// state: real amplitudes 2^(-n/2) (1 + d_i) with distinct small d_i, so that
+// no zero-skip or equal-pair skip fires, as on a random-phase input.
+// Returns the wall seconds of the op loop only, and, when perOp is not null,
+// each operation's own seconds.
+int ref_time_ops_each(unsigned n, const orc_op *ops, size_t count, double *seconds, double *perOp);
+
 int ref_time_ops(unsigned n, const orc_op *ops, size_t count, double *seconds) {
+	return ref_time_ops_each(n, ops, count, seconds, nullptr);
+}
+
+int ref_time_ops_each(unsigned n, const orc_op *ops, size_t count, double *seconds, double *perOp) {
 	try {
 		using namespace tmfqs;
 		RegisterConfig cfg;
@@ -169,9 +187,10 @@ int ref_time_ops(unsigned n, const orc_op *ops, size_t count, double *seconds) {
 		const size_t states = static_cast<size_t>(1) << n;
 		AmplitudesVector v(2 * states, 0.0);
 		const double a = 1.0 / std::sqrt(static_cast<double>(states));
-		for(size_t i = 0; i < states; ++i) v[2 * i] = a;
+		for(size_t i = 0; i < states; ++i) v[2 * i] = a * (1.0 + 0x1p-24 * static_cast<double>(timing_jitter(i)));
 		QuantumRegister reg(n, std::move(v), cfg);
 		const auto t0 = std::chrono::steady_clock::now();
+		auto tprev = t0;
 		for(size_t i = 0; i < count; ++i) {
 			const orc_op &op = ops[i];
 			switch(op.kind) {
@@ -184,6 +203,11 @@ int ref_time_ops(unsigned n, const orc_op *ops, size_t count, double *seconds) {
 				case ORC_OP_DIFFUSE_LAZY: reg.applyInversionAboutMean(); break;
 				case ORC_OP_DIFFUSE_MATERIALIZED: reg.applyInversionAboutMeanMaterialized(); break;
 				default: throw std::invalid_argument("op kind not timed");
+			}
+			if(perOp) {
+				const auto t = std::chrono::steady_clock::now();
+				perOp[i] = std::chrono::duration<double>(t - tprev).count();
+				tprev = t;
 			}
 		}
 		*seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
