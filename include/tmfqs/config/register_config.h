@@ -13,7 +13,8 @@ enum class StorageStrategyKind {
 	Blosc,
 	Zfp,
 	Auto,
-	/** MI355X (gfx950) device state vector; selected only explicitly. */
+	/** MI355X (gfx950) device state vector: the default strategy of this
+	 *  library and what Auto resolves to (see RegisterConfig). */
 	Hip
 };
 
@@ -89,8 +90,13 @@ struct HipTuningOverrides {
 };
 
 struct RegisterConfig {
-	/** Hip by default: this library ships only the device backend, so the
-	 *  reference's default-constructed registers and simulators run on the GPU. */
+	/** Differs from the reference, whose default is Dense.  This library ships
+	 *  only the device backend, so registers and simulators built without a
+	 *  strategy run on the GPU, Auto resolves to Hip whatever the state size,
+	 *  and Dense, Blosc and Zfp report unavailable (constructing them throws
+	 *  std::runtime_error, as an unavailable codec backend does in the
+	 *  reference).  Inside the reference itself, where Hip is added to its
+	 *  own registry, the default stays Dense and Hip is opt-in. */
 	StorageStrategyKind strategy = StorageStrategyKind::Hip;
 	size_t autoThresholdBytes = 8u * 1024u * 1024u;
 	StorageWorkloadHint workloadHint = StorageWorkloadHint::Generic;

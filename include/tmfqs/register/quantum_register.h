@@ -66,32 +66,45 @@ class QuantumRegister {
 		/** The storage backend (device handles are reachable through hipBackendHandle). */
 		IStateBackend *backend() const noexcept { return backend_.get(); }
 		/** True when no lazy affine overlay is pending (stored == logical amplitudes). */
-		bool overlayIsIdentity() const noexcept { return !affineOverlayActive_; }
+		bool overlayIsIdentity() const noexcept { return !overlay_.has_value(); }
 		/** Hands the backend to an operation outside the register's gate set (the
 		 *  adjoint gradient): flushes a pending overlay and drops the cached
 		 *  amplitude sum, since the caller is about to change the state. */
 		IStateBackend *beginExternalUpdate();
 
 	private:
+		// What the backend stores and what the register reports can differ by
+		// an affine map, logical = scale * stored + bias: the lazy diffusion
+		// keeps reflections about the mean in this map instead of sweeping
+		// the state.  No map means stored == logical.
+		struct AffineMap {
+			Amplitude scale{1.0, 0.0};
+			Amplitude bias{0.0, 0.0};
+			Amplitude toLogical(Amplitude stored) const;
+			Amplitude toStored(Amplitude logical) const;
+			AffineMap reflectedAbout(Amplitude mean) const;
+		};
+
 		unsigned int numQubits_ = 0;
 		StateIndex numStates_ = 0;
 		RegisterConfig config_{};
 		StorageStrategyKind resolvedStrategy_ = StorageStrategyKind::Dense;
 		std::unique_ptr<IStateBackend> backend_;
-		mutable std::optional<Amplitude> cachedLogicalAmplitudeSum_;
-		Amplitude affineScale_{1.0, 0.0};
-		Amplitude affineBias_{0.0, 0.0};
-		bool affineOverlayActive_ = false;
+		std::optional<AffineMap> overlay_;
+		// The exact sequential sum of the logical amplitudes when it is known
+		// (Grover's mean); any gate forgets it.
+		mutable std::optional<Amplitude> logicalSum_;
 
-		void initializeBackend(unsigned int qubits);
-		void requireInitialized(const char *operation) const;
-		void resetAffineOverlay();
-		void flushAffineOverlay();
-		Amplitude applyAffineOverlay(Amplitude backendAmplitude) const;
-		Amplitude removeAffineOverlay(Amplitude logicalAmplitude) const;
-		Amplitude logicalAmplitudeSum() const;
-		void visitLogicalAmplitudeChunks(const std::function<void(StateIndex, const double *, size_t)> &visitor) const;
-		AmplitudesVector snapshotLogicalAmplitudes() const;
+		QuantumRegister(unsigned int numQubits, const RegisterConfig &cfg,
+		                const std::function<void(IStateBackend &, unsigned int)> &init);
+		IStateBackend &backendFor(const char *operation) const;
+		void collapseOverlay();
+		void gateUpdate(const char *operation, const std::function<void()> &validate,
+		                const std::function<void(IStateBackend &)> &apply);
+		Amplitude logicalSum() const;
+		void reflectSum(Amplitude mean) const;
+		bool scanLogical(const std::function<bool(StateIndex, Amplitude)> &visit) const;
+		void printLogical(std::ostream &os, double epsilon) const;
 };
 
 }  // namespace tmfqs

@@ -19,6 +19,8 @@ bool isHipStateBackendAvailable();
 std::unique_ptr<IStateBackend> createHipStateBackend(const RegisterConfig &cfg);
 /** The C-ABI handle behind a backend created by createHipStateBackend (nullptr otherwise). */
 tmfqs_hip_state *hipBackendHandle(IStateBackend *backend);
+/** Throws the exception a tmfqs_hip_* status stands for (nothing for TMFQS_HIP_OK). */
+void raiseHipStatus(int status);
 
 }  // namespace tmfqs
 

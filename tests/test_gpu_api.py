@@ -31,7 +31,7 @@ def cplx(x):
 def test_cpp_drivers(exe):
     p = subprocess.run([os.path.join(BIN, exe)], capture_output=True, text=True, timeout=300)
     assert p.returncode == 0, p.stdout + p.stderr
-    assert "passed" in p.stdout
+    assert " 0 failures" in p.stdout and "FAIL" not in p.stdout
 
 
 def test_circuit_builders_match_oracle():

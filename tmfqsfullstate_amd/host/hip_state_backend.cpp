@@ -16,9 +16,8 @@
 #include "tmfqs_hip.h"
 
 namespace tmfqs {
-namespace {
 
-void raise(int rc) {
+void raiseHipStatus(int rc) {
 	if(rc == TMFQS_HIP_OK) return;
 	const std::string msg = tmfqs_hip_last_error();
 	switch(rc) {
@@ -29,6 +28,10 @@ void raise(int rc) {
 		default: throw std::runtime_error(msg);
 	}
 }
+
+namespace {
+
+void raise(int rc) { raiseHipStatus(rc); }
 
 tmfqs_hip_config toHipConfig(const HipConfig &c) {
 	tmfqs_hip_config h;

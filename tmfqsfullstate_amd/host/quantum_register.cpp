@@ -1,10 +1,19 @@
-// Register facade over a storage backend, with the reference's semantics
-// (src/tmfqs/register/quantum_register.cpp): validation order, swap as three
-// controlled-NOTs, the cached logical amplitude sum with its algebraic
-// recurrence (:176-189, :405-420, :433-467) and the lazy affine overlay
-// (:152-231).  Where the backend offers the optional accelerated reads
-// (amplitudeSum, totalProbabilityAffine, sampleMeasurementAffine, applySwap)
-// they replace the host loops that would stream the state to the host.
+// QuantumRegister on top of an IStateBackend.
+//
+// The register adds three things to the backend's storage, each with the
+// reference's semantics (src/tmfqs/register/quantum_register.cpp:61-539):
+//   * argument validation, with its error classes and scopes;
+//   * an optional affine map between stored and logical amplitudes (the lazy
+//     diffusion, :152-231, :423-444), collapsed into the storage before any
+//     gate;
+//   * the exact sequential sum of the logical amplitudes once it has been
+//     computed, kept up to date through phase flips and reflections about
+//     the mean (:176-189, :405-420, :447-467), so that Grover's mean costs
+//     no sweep after the first.
+// Every arithmetic step on amplitudes and sums is written in the reference's
+// operation order, which keeps results bit-identical.  Where the backend
+// offers accelerated reads (amplitudeSum, totalProbabilityAffine,
+// sampleMeasurementAffine) or a one-pass swap, they replace host scans.
 #include "tmfqs/register/quantum_register.h"
 
 #include <cmath>
@@ -12,7 +21,6 @@
 #include <stdexcept>
 #include <string>
 #include <utility>
-#include <vector>
 
 #include "tmfqs/core/random.h"
 #include "tmfqs/core/state_space.h"
@@ -22,404 +30,361 @@
 namespace tmfqs {
 namespace {
 
-constexpr double kDefaultPrintEpsilon = 1e-12;
+// Cutoff of operator<< (the reference prints amplitudes above 1e-12).
+constexpr double kStreamPrintCutoff = 1e-12;
 
-Amplitude addA(Amplitude a, Amplitude b) { return {a.real + b.real, a.imag + b.imag}; }
-Amplitude subA(Amplitude a, Amplitude b) { return {a.real - b.real, a.imag - b.imag}; }
-Amplitude negA(Amplitude a) { return {-a.real, -a.imag}; }
-Amplitude scaleA(Amplitude a, double s) { return {a.real * s, a.imag * s}; }
-Amplitude mulA(Amplitude a, Amplitude b) { return {a.real * b.real - a.imag * b.imag, a.real * b.imag + a.imag * b.real}; }
-Amplitude divA(Amplitude n, Amplitude d) {
-	const double dd = d.real * d.real + d.imag * d.imag;
-	if(dd == 0.0) throw std::logic_error("QuantumRegister affine overlay scale is zero");
-	return {(n.real * d.real + n.imag * d.imag) / dd, (n.imag * d.real - n.real * d.imag) / dd};
+void requireDistinctPair(const char *scope, QubitIndex a, QubitIndex b, unsigned int qubits) {
+	validateQubitIndex(scope, a, qubits);
+	validateQubitIndex(scope, b, qubits);
+	validateDistinctQubits(scope, a, b);
 }
-double probOf(Amplitude a) { return a.real * a.real + a.imag * a.imag; }
+
+void printAmplitude(std::ostream &os, StateIndex s, Amplitude a) { os << s << ": " << a.real << " + " << a.imag << "i\n"; }
 
 }  // namespace
 
-QuantumRegister::QuantumRegister() {
-	initializeBackend(0);
-	backend_->initBasis(numQubits_, 0, {1.0, 0.0});
+// ------------------------------------------------------------ affine map
+Amplitude QuantumRegister::AffineMap::toLogical(Amplitude x) const {
+	const double re = scale.real * x.real - scale.imag * x.imag;
+	const double im = scale.real * x.imag + scale.imag * x.real;
+	return {re + bias.real, im + bias.imag};
 }
 
-void QuantumRegister::initializeBackend(unsigned int qubits) {
-	numQubits_ = qubits;
-	numStates_ = checkedStateCount(numQubits_);
+Amplitude QuantumRegister::AffineMap::toStored(Amplitude l) const {
+	const double norm = scale.real * scale.real + scale.imag * scale.imag;
+	if(norm == 0.0) throw std::logic_error("QuantumRegister affine overlay scale is zero");
+	const double dr = l.real - bias.real, di = l.imag - bias.imag;
+	return {(dr * scale.real + di * scale.imag) / norm, (di * scale.real - dr * scale.imag) / norm};
+}
+
+// Reflection about the mean composed after this map: 2m - (s x + b) = (-s) x + (2m - b).
+QuantumRegister::AffineMap QuantumRegister::AffineMap::reflectedAbout(Amplitude mean) const {
+	AffineMap r;
+	r.scale = {-scale.real, -scale.imag};
+	r.bias = {mean.real * 2.0 - bias.real, mean.imag * 2.0 - bias.imag};
+	return r;
+}
+
+// ------------------------------------------------------------ construction
+QuantumRegister::QuantumRegister(unsigned int qubits, const RegisterConfig &cfg,
+                                 const std::function<void(IStateBackend &, unsigned int)> &init)
+    : numQubits_(qubits), numStates_(checkedStateCount(qubits)), config_(cfg) {
 	BackendSelection sel = StateBackendFactory::createSelection(numQubits_, config_);
 	config_ = sel.config;
 	resolvedStrategy_ = sel.strategy;
 	backend_ = std::move(sel.backend);
+	init(*backend_, numQubits_);
 }
 
-QuantumRegister::QuantumRegister(unsigned int qubits, const RegisterConfig &cfg) {
-	config_ = cfg;
-	initializeBackend(qubits);
-	backend_->initBasis(numQubits_, 0, {1.0, 0.0});
-}
+QuantumRegister::QuantumRegister() : QuantumRegister(0u, RegisterConfig{}) {}
 
-QuantumRegister::QuantumRegister(unsigned int qubits, StateIndex initState, const RegisterConfig &cfg) {
-	config_ = cfg;
-	initializeBackend(qubits);
-	backend_->initBasis(numQubits_, initState, {1.0, 0.0});
-}
+QuantumRegister::QuantumRegister(unsigned int qubits, const RegisterConfig &cfg)
+    : QuantumRegister(qubits, StateIndex{0}, Amplitude{1.0, 0.0}, cfg) {}
 
-QuantumRegister::QuantumRegister(unsigned int qubits, StateIndex initState, Amplitude amp, const RegisterConfig &cfg) {
-	config_ = cfg;
-	initializeBackend(qubits);
-	backend_->initBasis(numQubits_, initState, amp);
-}
+QuantumRegister::QuantumRegister(unsigned int qubits, StateIndex initState, const RegisterConfig &cfg)
+    : QuantumRegister(qubits, initState, Amplitude{1.0, 0.0}, cfg) {}
 
-QuantumRegister::QuantumRegister(unsigned int qubits, AmplitudesVector amplitudes, const RegisterConfig &cfg) {
-	config_ = cfg;
-	initializeBackend(qubits);
-	backend_->loadAmplitudes(numQubits_, std::move(amplitudes));
-}
+QuantumRegister::QuantumRegister(unsigned int qubits, StateIndex initState, Amplitude amp, const RegisterConfig &cfg)
+    : QuantumRegister(qubits, cfg, [&](IStateBackend &b, unsigned int n) { b.initBasis(n, initState, amp); }) {}
+
+QuantumRegister::QuantumRegister(unsigned int qubits, AmplitudesVector amplitudes, const RegisterConfig &cfg)
+    : QuantumRegister(qubits, cfg, [&](IStateBackend &b, unsigned int n) { b.loadAmplitudes(n, std::move(amplitudes)); }) {}
 
 QuantumRegister::QuantumRegister(const QuantumRegister &o)
     : numQubits_(o.numQubits_),
       numStates_(o.numStates_),
       config_(o.config_),
       resolvedStrategy_(o.resolvedStrategy_),
-      cachedLogicalAmplitudeSum_(o.cachedLogicalAmplitudeSum_),
-      affineScale_(o.affineScale_),
-      affineBias_(o.affineBias_),
-      affineOverlayActive_(o.affineOverlayActive_) {
-	if(o.backend_) backend_ = o.backend_->clone();
-}
+      backend_(o.backend_ ? o.backend_->clone() : nullptr),
+      overlay_(o.backend_ ? o.overlay_ : std::nullopt),
+      logicalSum_(o.backend_ ? o.logicalSum_ : std::nullopt) {}
 
 QuantumRegister &QuantumRegister::operator=(const QuantumRegister &o) {
-	if(this == &o) return *this;
-	numQubits_ = o.numQubits_;
-	numStates_ = o.numStates_;
-	config_ = o.config_;
-	resolvedStrategy_ = o.resolvedStrategy_;
-	if(!o.backend_) {
-		backend_.reset();
-		resetAffineOverlay();
-		return *this;
+	if(this != &o) {
+		QuantumRegister copy(o);
+		*this = std::move(copy);
 	}
-	backend_ = o.backend_->clone();
-	affineScale_ = o.affineScale_;
-	affineBias_ = o.affineBias_;
-	affineOverlayActive_ = o.affineOverlayActive_;
-	cachedLogicalAmplitudeSum_ = o.cachedLogicalAmplitudeSum_;
 	return *this;
 }
 
 QuantumRegister::~QuantumRegister() = default;
 
-void QuantumRegister::requireInitialized(const char *operation) const {
+// ------------------------------------------------------------ internals
+IStateBackend &QuantumRegister::backendFor(const char *operation) const {
 	if(!backend_) throw std::logic_error(std::string("QuantumRegister is not initialized for ") + operation);
+	return *backend_;
 }
 
-void QuantumRegister::resetAffineOverlay() {
-	affineScale_ = {1.0, 0.0};
-	affineBias_ = {0.0, 0.0};
-	affineOverlayActive_ = false;
-	cachedLogicalAmplitudeSum_ = std::nullopt;
+// Writes the pending affine map into the storage; the logical state, and so
+// its known sum, do not change.
+void QuantumRegister::collapseOverlay() {
+	if(!overlay_) return;
+	backend_->applyAffineTransform(overlay_->scale, overlay_->bias);
+	overlay_.reset();
 }
 
-Amplitude QuantumRegister::applyAffineOverlay(Amplitude b) const {
-	if(!affineOverlayActive_) return b;
-	return addA(mulA(affineScale_, b), affineBias_);
+void QuantumRegister::gateUpdate(const char *operation, const std::function<void()> &validate,
+                                 const std::function<void(IStateBackend &)> &apply) {
+	IStateBackend &b = backendFor(operation);
+	validate();
+	collapseOverlay();
+	apply(b);
+	logicalSum_.reset();
 }
 
-Amplitude QuantumRegister::removeAffineOverlay(Amplitude l) const {
-	if(!affineOverlayActive_) return l;
-	return divA(subA(l, affineBias_), affineScale_);
+// Streams the logical amplitudes in index order until visit returns false;
+// returns whether it stopped early.
+bool QuantumRegister::scanLogical(const std::function<bool(StateIndex, Amplitude)> &visit) const {
+	IStateBackend &b = backendFor("amplitude chunk iteration");
+	bool stopped = false;
+	b.forEachAmplitudeChunk([&](StateIndex base, const double *d, size_t elems) {
+		for(size_t e = 0; !stopped && e + 1u < elems; e += 2u) {
+			Amplitude a{d[e], d[e + 1u]};
+			if(overlay_) a = overlay_->toLogical(a);
+			stopped = !visit(base + static_cast<StateIndex>(e / 2u), a);
+		}
+	});
+	return stopped;
 }
 
-Amplitude QuantumRegister::logicalAmplitudeSum() const {
-	if(cachedLogicalAmplitudeSum_.has_value()) return *cachedLogicalAmplitudeSum_;
+Amplitude QuantumRegister::logicalSum() const {
+	if(logicalSum_) return *logicalSum_;
 	Amplitude sum{0.0, 0.0};
-	if(affineOverlayActive_ || !backend_->amplitudeSum(sum)) {
+	if(overlay_ || !backendFor("amplitude chunk iteration").amplitudeSum(sum)) {
 		sum = {0.0, 0.0};
-		visitLogicalAmplitudeChunks([&](StateIndex, const double *d, size_t elems) {
-			for(size_t e = 0; e + 1u < elems; e += 2u) {
-				sum.real += d[e];
-				sum.imag += d[e + 1u];
-			}
+		scanLogical([&](StateIndex, Amplitude a) {
+			sum.real += a.real;
+			sum.imag += a.imag;
+			return true;
 		});
 	}
-	cachedLogicalAmplitudeSum_ = sum;
+	logicalSum_ = sum;
 	return sum;
 }
 
-void QuantumRegister::visitLogicalAmplitudeChunks(const std::function<void(StateIndex, const double *, size_t)> &visitor) const {
-	requireInitialized("amplitude chunk iteration");
-	if(!affineOverlayActive_) {
-		backend_->forEachAmplitudeChunk(visitor);
+// The known sum after a reflection about `mean`: sum' = 2 m N - sum.
+void QuantumRegister::reflectSum(Amplitude mean) const {
+	if(!logicalSum_) return;
+	const double N = static_cast<double>(numStates_);
+	logicalSum_->real = 2.0 * mean.real * N - logicalSum_->real;
+	logicalSum_->imag = 2.0 * mean.imag * N - logicalSum_->imag;
+}
+
+void QuantumRegister::printLogical(std::ostream &os, double epsilon) const {
+	if(!overlay_) {
+		backend_->printNonZeroStates(os, epsilon);
 		return;
 	}
-	std::vector<double> logical;
-	backend_->forEachAmplitudeChunk([&](StateIndex base, const double *d, size_t elems) {
-		logical.resize(elems);
-		for(size_t e = 0; e + 1u < elems; e += 2u) {
-			const Amplitude l = applyAffineOverlay({d[e], d[e + 1u]});
-			logical[e] = l.real;
-			logical[e + 1u] = l.imag;
-		}
-		visitor(base, logical.data(), elems);
+	scanLogical([&](StateIndex s, Amplitude a) {
+		if(std::abs(a.real) > epsilon || std::abs(a.imag) > epsilon) printAmplitude(os, s, a);
+		return true;
 	});
 }
 
-AmplitudesVector QuantumRegister::snapshotLogicalAmplitudes() const {
-	AmplitudesVector out;
-	backend_->exportAmplitudes(out);
-	if(affineOverlayActive_) {
-		for(size_t e = 0; e + 1u < out.size(); e += 2u) {
-			const Amplitude l = applyAffineOverlay({out[e], out[e + 1u]});
-			out[e] = l.real;
-			out[e + 1u] = l.imag;
-		}
-	}
-	return out;
-}
-
-void QuantumRegister::flushAffineOverlay() {
-	if(!affineOverlayActive_) return;
-	const std::optional<Amplitude> saved = cachedLogicalAmplitudeSum_;
-	backend_->applyAffineTransform(affineScale_, affineBias_);
-	resetAffineOverlay();
-	cachedLogicalAmplitudeSum_ = saved;
-}
-
 IStateBackend *QuantumRegister::beginExternalUpdate() {
-	requireInitialized("external update");
-	flushAffineOverlay();
-	cachedLogicalAmplitudeSum_.reset();
-	return backend_.get();
+	IStateBackend &b = backendFor("external update");
+	collapseOverlay();
+	logicalSum_.reset();
+	return &b;
 }
 
+// ------------------------------------------------------------ queries
 unsigned int QuantumRegister::qubitCount() const { return numQubits_; }
 StateIndex QuantumRegister::stateCount() const { return numStates_; }
 size_t QuantumRegister::amplitudeElementCount() const { return static_cast<size_t>(numStates_) * 2u; }
+StorageStrategyKind QuantumRegister::storageStrategy() const { return resolvedStrategy_; }
 
-Amplitude QuantumRegister::amplitude(StateIndex s) const {
-	requireInitialized("amplitude query");
-	validateStateIndex("QuantumRegister::amplitude", s, numStates_);
-	return applyAffineOverlay(backend_->amplitude(s));
+Amplitude QuantumRegister::amplitude(StateIndex state) const {
+	IStateBackend &b = backendFor("amplitude query");
+	validateStateIndex("QuantumRegister::amplitude", state, numStates_);
+	const Amplitude stored = b.amplitude(state);
+	return overlay_ ? overlay_->toLogical(stored) : stored;
+}
+
+double QuantumRegister::probability(StateIndex state) const {
+	backendFor("probability query");
+	validateStateIndex("QuantumRegister::probability", state, numStates_);
+	const Amplitude a = amplitude(state);
+	return a.real * a.real + a.imag * a.imag;
 }
 
 AmplitudesVector QuantumRegister::amplitudes() const {
-	requireInitialized("amplitude export");
-	return snapshotLogicalAmplitudes();
-}
-
-double QuantumRegister::probability(StateIndex s) const {
-	requireInitialized("probability query");
-	validateStateIndex("QuantumRegister::probability", s, numStates_);
-	return probOf(amplitude(s));
+	IStateBackend &b = backendFor("amplitude export");
+	AmplitudesVector out;
+	b.exportAmplitudes(out);
+	if(overlay_)
+		for(size_t e = 0; e + 1u < out.size(); e += 2u) {
+			const Amplitude l = overlay_->toLogical({out[e], out[e + 1u]});
+			out[e] = l.real;
+			out[e + 1u] = l.imag;
+		}
+	return out;
 }
 
 double QuantumRegister::totalProbability() const {
-	requireInitialized("probability summation");
-	if(!affineOverlayActive_) return backend_->totalProbability();
+	IStateBackend &b = backendFor("probability summation");
+	if(!overlay_) return b.totalProbability();
 	double total = 0.0;
-	if(backend_->totalProbabilityAffine(affineScale_, affineBias_, total)) return total;
+	if(b.totalProbabilityAffine(overlay_->scale, overlay_->bias, total)) return total;
 	total = 0.0;
-	visitLogicalAmplitudeChunks([&](StateIndex, const double *d, size_t elems) {
-		for(size_t e = 0; e + 1u < elems; e += 2u) total += d[e] * d[e] + d[e + 1u] * d[e + 1u];
+	scanLogical([&](StateIndex, Amplitude a) {
+		total += a.real * a.real + a.imag * a.imag;
+		return true;
 	});
 	return total;
 }
 
-StateIndex QuantumRegister::measure(IRandomSource &src) const {
-	requireInitialized("measurement");
-	if(!affineOverlayActive_) return backend_->sampleMeasurement(src.nextUnitDouble());
-	const double rnd = src.nextUnitDouble();
-	StateIndex out = 0;
-	if(backend_->sampleMeasurementAffine(rnd, affineScale_, affineBias_, out)) return out;
-	double cum = 0.0;
-	StateIndex found = numStates_;
-	visitLogicalAmplitudeChunks([&](StateIndex base, const double *d, size_t elems) {
-		if(found != numStates_) return;
-		for(size_t i = 0; i < elems / 2u; ++i) {
-			cum += d[2 * i] * d[2 * i] + d[2 * i + 1] * d[2 * i + 1];
-			if(rnd <= cum) {
-				found = base + i;
-				return;
-			}
+StateIndex QuantumRegister::measure(IRandomSource &randomSource) const {
+	IStateBackend &b = backendFor("measurement");
+	if(!overlay_) return b.sampleMeasurement(randomSource.nextUnitDouble());
+	const double rnd = randomSource.nextUnitDouble();
+	StateIndex hit = 0;
+	if(b.sampleMeasurementAffine(rnd, overlay_->scale, overlay_->bias, hit)) return hit;
+	double cumulative = 0.0;
+	const bool found = scanLogical([&](StateIndex s, Amplitude a) {
+		cumulative += a.real * a.real + a.imag * a.imag;
+		if(rnd <= cumulative) {
+			hit = s;
+			return false;
 		}
+		return true;
 	});
-	if(found != numStates_) return found;
-	throw std::runtime_error("QuantumRegister::measure cumulative probability did not reach sample");
-}
-
-void QuantumRegister::setAmplitude(StateIndex s, Amplitude a) {
-	requireInitialized("state update");
-	validateStateIndex("QuantumRegister::setAmplitude", s, numStates_);
-	backend_->setAmplitude(s, removeAffineOverlay(a));
-	cachedLogicalAmplitudeSum_ = std::nullopt;
-}
-
-void QuantumRegister::loadAmplitudes(AmplitudesVector a) {
-	requireInitialized("amplitude loading");
-	if(a.size() != amplitudeElementCount()) throw std::invalid_argument("QuantumRegister::loadAmplitudes amplitudes size mismatch");
-	resetAffineOverlay();
-	backend_->loadAmplitudes(numQubits_, std::move(a));
-}
-
-void QuantumRegister::initUniformSuperposition(const BasisStateList &states) {
-	requireInitialized("uniform superposition initialization");
-	if(states.empty()) throw std::invalid_argument("QuantumRegister::initUniformSuperposition requires at least one basis state");
-	for(StateIndex s : states) validateStateIndex("QuantumRegister::initUniformSuperposition", s, numStates_);
-	resetAffineOverlay();
-	backend_->initUniformSuperposition(numQubits_, states);
-}
-
-StorageStrategyKind QuantumRegister::storageStrategy() const { return resolvedStrategy_; }
-
-std::ostream &operator<<(std::ostream &os, const QuantumRegister &reg) {
-	reg.requireInitialized("state printing");
-	if(!reg.affineOverlayActive_) {
-		reg.backend_->printNonZeroStates(os, kDefaultPrintEpsilon);
-		return os;
-	}
-	reg.visitLogicalAmplitudeChunks([&](StateIndex base, const double *d, size_t elems) {
-		for(size_t i = 0; i < elems / 2u; ++i)
-			if(std::abs(d[2 * i]) > kDefaultPrintEpsilon || std::abs(d[2 * i + 1]) > kDefaultPrintEpsilon)
-				os << (base + i) << ": " << d[2 * i] << " + " << d[2 * i + 1] << "i\n";
-	});
-	return os;
+	if(!found) throw std::runtime_error("QuantumRegister::measure cumulative probability did not reach sample");
+	return hit;
 }
 
 void QuantumRegister::printStatesVector(double epsilon) const {
-	requireInitialized("state printing");
+	backendFor("state printing");
 	if(epsilon < 0.0) throw std::invalid_argument("QuantumRegister::printStatesVector epsilon must be non-negative");
-	if(!affineOverlayActive_) {
-		backend_->printNonZeroStates(std::cout, epsilon);
+	printLogical(std::cout, epsilon);
+}
+
+std::ostream &operator<<(std::ostream &os, const QuantumRegister &reg) {
+	reg.backendFor("state printing");
+	reg.printLogical(os, kStreamPrintCutoff);
+	return os;
+}
+
+// ------------------------------------------------------------ state updates
+void QuantumRegister::setAmplitude(StateIndex state, Amplitude amp) {
+	IStateBackend &b = backendFor("state update");
+	validateStateIndex("QuantumRegister::setAmplitude", state, numStates_);
+	b.setAmplitude(state, overlay_ ? overlay_->toStored(amp) : amp);
+	logicalSum_.reset();
+}
+
+void QuantumRegister::loadAmplitudes(AmplitudesVector amplitudes) {
+	IStateBackend &b = backendFor("amplitude loading");
+	if(amplitudes.size() != amplitudeElementCount())
+		throw std::invalid_argument("QuantumRegister::loadAmplitudes amplitudes size mismatch");
+	overlay_.reset();
+	logicalSum_.reset();
+	b.loadAmplitudes(numQubits_, std::move(amplitudes));
+}
+
+void QuantumRegister::initUniformSuperposition(const BasisStateList &basisStates) {
+	IStateBackend &b = backendFor("uniform superposition initialization");
+	if(basisStates.empty())
+		throw std::invalid_argument("QuantumRegister::initUniformSuperposition requires at least one basis state");
+	for(StateIndex s : basisStates) validateStateIndex("QuantumRegister::initUniformSuperposition", s, numStates_);
+	overlay_.reset();
+	logicalSum_.reset();
+	b.initUniformSuperposition(numQubits_, basisStates);
+}
+
+// ------------------------------------------------------------ Grover steps
+void QuantumRegister::applyPhaseFlipBasisState(StateIndex state) {
+	IStateBackend &b = backendFor("basis-state phase flip");
+	validateStateIndex("QuantumRegister::applyPhaseFlipBasisState", state, numStates_);
+	if(!logicalSum_ && !overlay_) {
+		b.phaseFlipBasisState(state);
 		return;
 	}
-	visitLogicalAmplitudeChunks([&](StateIndex base, const double *d, size_t elems) {
-		for(size_t i = 0; i < elems / 2u; ++i)
-			if(std::abs(d[2 * i]) > epsilon || std::abs(d[2 * i + 1]) > epsilon)
-				std::cout << (base + i) << ": " << d[2 * i] << " + " << d[2 * i + 1] << "i\n";
-	});
-}
-
-void QuantumRegister::applyGate(const QuantumGate &gate, const QubitList &qubits) {
-	requireInitialized("gate application");
-	validateGateTargets("QuantumRegister::applyGate", qubits, numQubits_, gate.dimension());
-	if(affineOverlayActive_) flushAffineOverlay();
-	backend_->applyGate(gate, qubits);
-	cachedLogicalAmplitudeSum_ = std::nullopt;
-}
-
-void QuantumRegister::applyPhaseFlipBasisState(StateIndex s) {
-	requireInitialized("basis-state phase flip");
-	validateStateIndex("QuantumRegister::applyPhaseFlipBasisState", s, numStates_);
-	if(cachedLogicalAmplitudeSum_.has_value() || affineOverlayActive_) {
-		const Amplitude l = amplitude(s);
-		if(cachedLogicalAmplitudeSum_.has_value()) {
-			cachedLogicalAmplitudeSum_->real -= 2.0 * l.real;
-			cachedLogicalAmplitudeSum_->imag -= 2.0 * l.imag;
-		}
-		if(affineOverlayActive_) {
-			backend_->setAmplitude(s, removeAffineOverlay(negA(l)));
-			return;
-		}
+	const Amplitude before = amplitude(state);
+	if(logicalSum_) {
+		logicalSum_->real -= 2.0 * before.real;
+		logicalSum_->imag -= 2.0 * before.imag;
 	}
-	backend_->phaseFlipBasisState(s);
+	if(overlay_)
+		b.setAmplitude(state, overlay_->toStored({-before.real, -before.imag}));
+	else
+		b.phaseFlipBasisState(state);
 }
 
 void QuantumRegister::applyInversionAboutMean() {
-	requireInitialized("inversion about mean");
-	const Amplitude sum = logicalAmplitudeSum();
-	applyInversionAboutMean({sum.real / static_cast<double>(numStates_), sum.imag / static_cast<double>(numStates_)});
+	backendFor("inversion about mean");
+	const Amplitude s = logicalSum();
+	const double N = static_cast<double>(numStates_);
+	applyInversionAboutMean({s.real / N, s.imag / N});
 }
 
 void QuantumRegister::applyInversionAboutMean(Amplitude mean) {
-	requireInitialized("inversion about mean");
-	if(cachedLogicalAmplitudeSum_.has_value()) {
-		cachedLogicalAmplitudeSum_->real = 2.0 * mean.real * static_cast<double>(numStates_) - cachedLogicalAmplitudeSum_->real;
-		cachedLogicalAmplitudeSum_->imag = 2.0 * mean.imag * static_cast<double>(numStates_) - cachedLogicalAmplitudeSum_->imag;
-	}
-	const Amplitude scale = affineOverlayActive_ ? affineScale_ : Amplitude{1.0, 0.0};
-	const Amplitude bias = affineOverlayActive_ ? affineBias_ : Amplitude{0.0, 0.0};
-	affineScale_ = negA(scale);
-	affineBias_ = subA(scaleA(mean, 2.0), bias);
-	affineOverlayActive_ = true;
+	backendFor("inversion about mean");
+	reflectSum(mean);
+	overlay_ = overlay_.value_or(AffineMap{}).reflectedAbout(mean);
 }
 
 void QuantumRegister::applyInversionAboutMeanMaterialized() {
-	requireInitialized("inversion about mean");
-	const Amplitude sum = logicalAmplitudeSum();
-	applyInversionAboutMeanMaterialized({sum.real / static_cast<double>(numStates_), sum.imag / static_cast<double>(numStates_)});
+	backendFor("inversion about mean");
+	const Amplitude s = logicalSum();
+	const double N = static_cast<double>(numStates_);
+	applyInversionAboutMeanMaterialized({s.real / N, s.imag / N});
 }
 
 void QuantumRegister::applyInversionAboutMeanMaterialized(Amplitude mean) {
-	requireInitialized("inversion about mean");
-	if(cachedLogicalAmplitudeSum_.has_value()) {
-		cachedLogicalAmplitudeSum_->real = 2.0 * mean.real * static_cast<double>(numStates_) - cachedLogicalAmplitudeSum_->real;
-		cachedLogicalAmplitudeSum_->imag = 2.0 * mean.imag * static_cast<double>(numStates_) - cachedLogicalAmplitudeSum_->imag;
-	}
-	if(affineOverlayActive_) flushAffineOverlay();
-	backend_->inversionAboutMean(mean);
+	IStateBackend &b = backendFor("inversion about mean");
+	reflectSum(mean);
+	collapseOverlay();
+	b.inversionAboutMean(mean);
 }
 
-void QuantumRegister::applyHadamard(QubitIndex q) {
-	requireInitialized("Hadamard");
-	validateQubitIndex("QuantumRegister::applyHadamard", q, numQubits_);
-	if(affineOverlayActive_) flushAffineOverlay();
-	backend_->applyHadamard(q);
-	cachedLogicalAmplitudeSum_ = std::nullopt;
+// ------------------------------------------------------------ gates
+void QuantumRegister::applyGate(const QuantumGate &gate, const QubitList &qubits) {
+	gateUpdate(
+	    "gate application", [&] { validateGateTargets("QuantumRegister::applyGate", qubits, numQubits_, gate.dimension()); },
+	    [&](IStateBackend &b) { b.applyGate(gate, qubits); });
 }
 
-void QuantumRegister::applyPauliX(QubitIndex q) {
-	requireInitialized("PauliX");
-	validateQubitIndex("QuantumRegister::applyPauliX", q, numQubits_);
-	if(affineOverlayActive_) flushAffineOverlay();
-	backend_->applyPauliX(q);
-	cachedLogicalAmplitudeSum_ = std::nullopt;
+void QuantumRegister::applyHadamard(QubitIndex qubit) {
+	gateUpdate(
+	    "Hadamard", [&] { validateQubitIndex("QuantumRegister::applyHadamard", qubit, numQubits_); },
+	    [&](IStateBackend &b) { b.applyHadamard(qubit); });
 }
 
-void QuantumRegister::applyControlledPhaseShift(QubitIndex c, QubitIndex t, double theta) {
-	requireInitialized("controlled phase shift");
-	validateQubitIndex("QuantumRegister::applyControlledPhaseShift", c, numQubits_);
-	validateQubitIndex("QuantumRegister::applyControlledPhaseShift", t, numQubits_);
-	validateDistinctQubits("QuantumRegister::applyControlledPhaseShift", c, t);
-	if(affineOverlayActive_) flushAffineOverlay();
-	backend_->applyControlledPhaseShift(c, t, theta);
-	cachedLogicalAmplitudeSum_ = std::nullopt;
+void QuantumRegister::applyPauliX(QubitIndex qubit) {
+	gateUpdate(
+	    "PauliX", [&] { validateQubitIndex("QuantumRegister::applyPauliX", qubit, numQubits_); },
+	    [&](IStateBackend &b) { b.applyPauliX(qubit); });
 }
 
-void QuantumRegister::applyControlledNot(QubitIndex c, QubitIndex t) {
-	requireInitialized("controlled not");
-	validateQubitIndex("QuantumRegister::applyControlledNot", c, numQubits_);
-	validateQubitIndex("QuantumRegister::applyControlledNot", t, numQubits_);
-	validateDistinctQubits("QuantumRegister::applyControlledNot", c, t);
-	if(affineOverlayActive_) flushAffineOverlay();
-	backend_->applyControlledNot(c, t);
-	cachedLogicalAmplitudeSum_ = std::nullopt;
+void QuantumRegister::applyControlledPhaseShift(QubitIndex controlQubit, QubitIndex targetQubit, double theta) {
+	gateUpdate(
+	    "controlled phase shift",
+	    [&] { requireDistinctPair("QuantumRegister::applyControlledPhaseShift", controlQubit, targetQubit, numQubits_); },
+	    [&](IStateBackend &b) { b.applyControlledPhaseShift(controlQubit, targetQubit, theta); });
 }
 
-// reference :518-527: three controlled-NOTs; a backend that supports the
-// equivalent single permutation runs that instead.
-void QuantumRegister::applySwap(QubitIndex a, QubitIndex b) {
-	requireInitialized("swap");
-	validateQubitIndex("QuantumRegister::applySwap", a, numQubits_);
-	validateQubitIndex("QuantumRegister::applySwap", b, numQubits_);
-	validateDistinctQubits("QuantumRegister::applySwap", a, b);
-	if(affineOverlayActive_) flushAffineOverlay();
-	if(backend_->applySwap(a, b)) {
-		cachedLogicalAmplitudeSum_ = std::nullopt;
-		return;
-	}
-	applyControlledNot(a, b);
-	applyControlledNot(b, a);
-	applyControlledNot(a, b);
+void QuantumRegister::applyControlledNot(QubitIndex controlQubit, QubitIndex targetQubit) {
+	gateUpdate(
+	    "controlled not", [&] { requireDistinctPair("QuantumRegister::applyControlledNot", controlQubit, targetQubit, numQubits_); },
+	    [&](IStateBackend &b) { b.applyControlledNot(controlQubit, targetQubit); });
 }
 
-void QuantumRegister::beginOperationBatch() {
-	requireInitialized("operation batch begin");
-	backend_->beginOperationBatch();
+// A swap is three controlled-NOTs (reference :518-527); a backend with a
+// one-pass swap (a bit permutation, bit-identical) runs that instead.
+void QuantumRegister::applySwap(QubitIndex qubit1, QubitIndex qubit2) {
+	gateUpdate(
+	    "swap", [&] { requireDistinctPair("QuantumRegister::applySwap", qubit1, qubit2, numQubits_); },
+	    [&](IStateBackend &b) {
+		    if(b.applySwap(qubit1, qubit2)) return;
+		    b.applyControlledNot(qubit1, qubit2);
+		    b.applyControlledNot(qubit2, qubit1);
+		    b.applyControlledNot(qubit1, qubit2);
+	    });
 }
 
-void QuantumRegister::endOperationBatch() {
-	requireInitialized("operation batch end");
-	backend_->endOperationBatch();
-}
+void QuantumRegister::beginOperationBatch() { backendFor("operation batch begin").beginOperationBatch(); }
+
+void QuantumRegister::endOperationBatch() { backendFor("operation batch end").endOperationBatch(); }
 
 }  // namespace tmfqs
