@@ -56,6 +56,40 @@ def test_mixed_circuit_vs_oracle():
     assert np.max(np.abs(g - ps)) <= 1e-10
 
 
+def test_five_qubit_blocks_in_the_backward_sweep():
+    # 5-qubit matrices between the ring layers at n = 12 (11-bit dual tiles:
+    # FP64 MFMA on both states of the backward sweep) against the oracle's
+    # parameter shift
+    from tests import numpy_ref as R
+    n = 12
+    rng = np.random.default_rng(55)
+    ring, rcnt = O.ring_ops(n, 3, 9)
+    seq = []
+    for i in range(rcnt):
+        o = ring[i]
+        seq.append((o.kind, o.q0, o.q1, o.theta, o.state, o.k))
+        if i % 20 == 19:
+            seq.append((O.OP_MATRIX, 0, 0, 0.0, 0, 5))
+    ops, cnt = O.make_ops(seq)
+    mats, qubits = [], []
+    for i in range(cnt):
+        if ops[i].kind == O.OP_MATRIX:
+            ops[i].q0 = len(qubits)
+            ops[i].state = len(mats)
+            qubits.extend(int(q) for q in rng.choice(n, 5, replace=False))
+            mats.extend(R.interleave(R.random_unitary(rng, 32).reshape(-1)))
+    mats = np.array(mats, dtype=np.float64)
+    qubits = np.array(qubits, dtype=np.uint32)
+    assert sum(1 for i in range(cnt) if ops[i].kind == O.OP_MATRIX) >= 2
+    print("ring n=12 with 5-qubit blocks", flush=True)
+    e, g, final = gpu_adjoint(n, ops, cnt, Z0_PLUS_P0, mats=mats, qubits=qubits)
+    reg = O.run_circuit(n, (ops, cnt), mat_pool=mats, mat_qubits=qubits)
+    assert np.max(np.abs(final - reg.amplitudes())) <= 1e-12
+    assert abs(e - O.expectation(reg.amplitudes(), n, Z0_PLUS_P0)) <= 1e-12
+    ps = O.param_shift_grad(n, ops, cnt, Z0_PLUS_P0, mat_pool=mats, mat_qubits=qubits)
+    assert np.max(np.abs(g - ps)) <= 1e-10
+
+
 def test_checkpoint_spacing_does_not_change_gradients():
     n = 9
     ops, cnt = O.ring_ops(n, 4, 7)

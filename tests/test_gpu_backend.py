@@ -425,6 +425,47 @@ def test_complex64_five_qubit_blocks(n):
     assert 1.0 - abs(np.vdot(w, got)) ** 2 / (np.vdot(w, w).real * np.vdot(got, got).real) <= 1e-6
 
 
+@pytest.mark.parametrize("n", [11, 12, 14, 17])
+def test_complex128_five_qubit_blocks_mfma(n):
+    # complex128 k = 5 matrices on tiles of >= 11 bits run on FP64 MFMA
+    # (FP64 accumulation in another order than the dense backend's): within
+    # 1e-12 of the oracle; the exact mode keeps the reference's VALU order and
+    # its bits.  Matrices mixed with other gates in the same tiled passes.
+    rng = np.random.default_rng(100 + n)
+    init = rand_state(n, n)
+    seq, mats, quals = [], [], []
+    for i in range(8):
+        qubits = [int(q) for q in rng.choice(n, size=5, replace=False)]
+        seq.append(("u", R.random_unitary(rng, 32), qubits))
+        a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+        seq.append(("h", a))
+        seq.append(("cp", a, b, float(rng.uniform(0, 6.3))))
+    want = init.copy()
+    for g in seq:
+        if g[0] == "u":
+            want = O.apply_gate(want, n, R.interleave(g[1]), g[2])
+        elif g[0] == "h":
+            want = O.run_circuit(n, [(O.OP_H, g[1])], init=want).amplitudes()
+        else:
+            want = O.run_circuit(n, [(O.OP_CPHASE, g[1], g[2], g[3])], init=want).amplitudes()
+    for exact in (0, 1):
+        s = st(exact_phases=exact).load(n, init)
+        s.begin_batch()
+        for g in seq:
+            if g[0] == "u":
+                s.gate(g[1], g[2])
+            elif g[0] == "h":
+                s.h(g[1])
+            else:
+                s.cphase(g[1], g[2], g[3])
+        s.end_batch()
+        got = s.export()
+        if exact:
+            assert np.array_equal(got, want)
+        else:
+            assert np.abs(cplx(got) - cplx(want)).max() <= 1e-12
+
+
 def test_qft_large_vs_fft():
     # n = 24: a size-independent property (the DFT) rather than the oracle.
     n = 24

@@ -919,7 +919,7 @@ struct PassLaunch {
 	std::vector<double> mats;
 	std::vector<TwiddleTerm> terms;
 	std::vector<uint32_t> gidx;  // dual: global gradient slot of each TOP_GRAD slot
-	bool ldsops = false;
+	int ldsops = 0;  // kernel build, see launch_tile_staged
 };
 
 struct GatePlan {
@@ -934,6 +934,7 @@ struct PlanCfg {
 	bool ladders = true;  // fuse QFT ladders (not bit-identical to one product per gate)
 	unsigned T = 12, C = 4, qft_c = 5;
 	unsigned W = 0;  // stage width of tile passes; 0 picks tile_stage_width(prec, T, dual)
+	bool mfma_blocks = true;  // complex128 five-qubit blocks on FP64 MFMA (not the reference's accumulation order)
 };
 
 GatePlan plan_gates(const PlanCfg &cfg, unsigned n, const std::vector<QOp> &in, size_t lo, size_t hi,
@@ -1084,6 +1085,27 @@ GatePlan plan_gates(const PlanCfg &cfg, unsigned n, const std::vector<QOp> &in, 
 						const size_t at = mats.size();
 						mats.resize(at + af.size() / 2);
 						std::memcpy(mats.data() + at, af.data(), af.size() * sizeof(float));
+					} else if(cfg.prec == 0 && op.k == 5) {
+						t.twid = kNoMfmaOperands;
+						if(cfg.mfma_blocks) {
+							// complex128 five-qubit blocks on FP64 MFMA: lane l of wavefront w
+							// takes A[s] = Mbig[16 w + l % 16][4 s + l / 16] (see
+							// tile_mfma5_f64), 16 B aligned after the matrix.
+							if(mats.size() & 1) mats.push_back(0.0);
+							t.twid = (unsigned)mats.size();
+							const double *g = qmats.data() + op.mat;
+							mats.resize(mats.size() + 4 * 64 * 16);
+							double *ad = mats.data() + t.twid;
+							for(unsigned w = 0; w < 4; ++w)
+								for(unsigned lane = 0; lane < 64; ++lane)
+									for(unsigned st = 0; st < 16; ++st) {
+										const unsigned R = 16 * w + (lane & 15u), K = 4 * st + (lane >> 4);
+										const unsigned r = R & 31u, c = K & 31u;
+										const double re = g[2 * (r * 32 + c)], im = g[2 * (r * 32 + c) + 1];
+										// [[Mr, -Mi], [Mi, Mr]]
+										ad[(w * 64 + lane) * 16 + st] = R < 32 ? (K < 32 ? re : -im) : (K < 32 ? im : re);
+									}
+						}
 					}
 					tops.push_back(t);
 					++k;
@@ -1195,7 +1217,18 @@ GatePlan plan_gates(const PlanCfg &cfg, unsigned n, const std::vector<QOp> &in, 
 			}
 			L.ops.assign(tops.begin() + s0, tops.begin() + e);
 			L.stages = plan_stages(L.ops.data(), L.ops.size(), Tp, W);
-			for(const TileStage &st : L.stages) L.ldsops = L.ldsops || st.lds;
+			for(const TileStage &st : L.stages)
+				if(st.lds) L.ldsops = 1;
+			if(L.ldsops && cfg.prec == 0 && Tp >= 11) {
+				// complex128 five-qubit blocks, all with MFMA operands: the MFMA build
+				bool k5 = false, all = true;
+				for(const TileOp &o : L.ops)
+					if(o.kind == TOP_UK && o.k == 5) {
+						k5 = true;
+						all = all && o.twid != kNoMfmaOperands;
+					}
+				if(k5 && all) L.ldsops = 2;
+			}
 			L.mats = mats;
 			L.terms = allterms;
 			L.tp = tp;
@@ -1268,6 +1301,7 @@ void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_
 	cfg.prec = h->prec;
 	cfg.dual = dual;
 	cfg.ladders = !h->cfg.exact_phases;
+	cfg.mfma_blocks = !h->cfg.exact_phases;
 	cfg.T = tile_bits(h);
 	if(tgt && tgt->tile_delta) cfg.T = (unsigned)std::max<int>(1, std::min<int>((int)n, (int)cfg.T + tgt->tile_delta));
 	cfg.C = coalesce_bits(h, cfg.T);

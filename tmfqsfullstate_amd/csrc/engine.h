@@ -63,6 +63,11 @@ struct TileOp {
 	double cr, ci;        // CP phase
 };
 
+// TOP_UK with k = 5: TileOp::twid is the offset (doubles) in the matrix pool
+// of the matrix's MFMA operands, laid out lane by lane, or this value when the
+// block runs on VALU in the reference's accumulation order.
+constexpr uint32_t kNoMfmaOperands = 0xffffffffu;
+
 // A run of tile ops applied between two LDS round trips.  In a register
 // stage every thread holds the 2^W amplitudes whose local indices differ in
 // the stage's W bit positions (ascending) and applies the stage's ops in
@@ -153,12 +158,14 @@ unsigned tile_stage_width(int prec, unsigned T, bool dual);
 // Workgroups of a tiled pass over P; a dual pass's gpart holds nslots * grid doubles.
 unsigned tile_grid(const TilePass &P, bool dual, int prec);
 // Tiled pass over one state (b == nullptr) or two states stepped together
-// (b != nullptr: TOP_GRAD partials into gpart); ldsops: some stage runs on
-// the LDS tile (a separate kernel build, so the common case keeps its
-// register budget).
+// (b != nullptr: TOP_GRAD partials into gpart); ldsops: 0 register stages
+// only, 1 some stage runs k-qubit matrices on the LDS tile (a separate kernel
+// build, so the common case keeps its register budget), 2 as 1 with every
+// five-qubit block on FP64 MFMA (complex128 tiles of >= 11 bits whose
+// blocks carry MFMA operands; no VALU five-qubit code, fewer registers).
 hipError_t launch_tile_staged(const Launch &L, void *a, void *b, const TilePass &P, unsigned W, const TileOp *dops,
                               const TileStage *dstages, unsigned nstages, const double *dmats, const TwiddleTerm *dterms,
-                              const double *dphase, double *gpart, unsigned nslots, bool ldsops);
+                              const double *dphase, double *gpart, unsigned nslots, int ldsops);
 // grads[gidx[s]] = scale * sum_i gpart[s * blocks + i], s < nslots.
 hipError_t launch_grad_finish(const Launch &L, const double *gpart, unsigned blocks, unsigned nslots, const uint32_t *gidx,
                               double scale, double *grads);

@@ -22,7 +22,7 @@ __global__ void __launch_bounds__(256) k_batch_tiles(const BatchPass *__restrict
 		__syncthreads();  // the previous item's tiles are done with hitab
 		tile_hitab<R>(hitab, bp.tp);
 		for(u64 tid = item.tile0; tid < item.tile0 + item.ntiles; ++tid)
-			staged_tile<R, W, false, false>((V *)bp.state, nullptr, bp.tp, tid, ops + bp.op0, stages + bp.st0, bp.nst, nullptr,
+			staged_tile<R, W, false, 0>((V *)bp.state, nullptr, bp.tp, tid, ops + bp.op0, stages + bp.st0, bp.nst, nullptr,
 			                                nullptr, ptab, ta, nullptr, hitab, nullptr);
 	}
 }
@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(256) k_batch_dual(const BatchPass *__restrict_
 		tile_hitab<R>(hitab, bp.tp);
 		for(unsigned i = threadIdx.x; i < 4 * kMaxGradSlots; i += blockDim.x) gacc[i] = 0.0;
 		for(u64 tid = item.tile0; tid < item.tile0 + item.ntiles; ++tid)
-			staged_tile<R, W, true, false>((V *)bp.state, (V *)bp.state2, bp.tp, tid, ops + bp.op0, stages + bp.st0, bp.nst, nullptr,
+			staged_tile<R, W, true, 0>((V *)bp.state, (V *)bp.state2, bp.tp, tid, ops + bp.op0, stages + bp.st0, bp.nst, nullptr,
 			                               nullptr, nullptr, ta, tb, hitab, gacc);
 		__syncthreads();
 		for(unsigned s = threadIdx.x; s < bp.nslots; s += blockDim.x)

@@ -22,13 +22,17 @@ unsigned tile_grid(const TilePass &P, bool dual, int prec) {
 
 hipError_t launch_tile_staged(const Launch &L, void *a, void *b, const TilePass &P, unsigned W, const TileOp *dops,
                               const TileStage *dstages, unsigned nstages, const double *dmats, const TwiddleTerm *dterms,
-                              const double *dphase, double *gpart, unsigned nslots, bool ldsops) {
+                              const double *dphase, double *gpart, unsigned nslots, int ldsops) {
 	if(W > P.T || nslots > kMaxGradSlots) return hipErrorInvalidValue;
-	static const decltype(&launch_tile_c128) entry[2][2][2] = {
-	    {{launch_tile_c128, launch_tile_c128_lds}, {launch_tile_c128_dual, launch_tile_c128_dual_lds}},
-	    {{launch_tile_c64, launch_tile_c64_lds}, {launch_tile_c64_dual, launch_tile_c64_dual_lds}}};
-	return entry[L.prec == 0 ? 0 : 1][b ? 1 : 0][ldsops ? 1 : 0](L, W, a, b, P, dops, dstages, nstages, dmats, dterms, dphase,
-	                                                               gpart, nslots);
+	if(ldsops < 0 || ldsops > 2) return hipErrorInvalidValue;
+	// ldsops: 0 register stages only, 1 with k-qubit matrix stages, 2 with
+	// five-qubit blocks on FP64 MFMA (complex128; complex64 blocks always use
+	// FP32 MFMA in build 1)
+	static const decltype(&launch_tile_c128) entry[2][2][3] = {
+	    {{launch_tile_c128, launch_tile_c128_lds, launch_tile_c128_mfma},
+	     {launch_tile_c128_dual, launch_tile_c128_dual_lds, launch_tile_c128_dual_mfma}},
+	    {{launch_tile_c64, launch_tile_c64_lds, launch_tile_c64_lds}, {launch_tile_c64_dual, launch_tile_c64_dual_lds, launch_tile_c64_dual_lds}}};
+	return entry[L.prec == 0 ? 0 : 1][b ? 1 : 0][ldsops](L, W, a, b, P, dops, dstages, nstages, dmats, dterms, dphase, gpart, nslots);
 }
 
 }  // namespace tmfqs_hip

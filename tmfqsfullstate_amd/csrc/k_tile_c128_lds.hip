@@ -4,7 +4,7 @@
 namespace tmfqs_hip {
 
 TMFQS_TILE_ENTRY(launch_tile_c128_lds) {
-	return launch_staged_w<double, false, true>(L, W, a, b, P, dops, dstages, nstages, dmats, dterms, dphase, gpart, nslots);
+	return launch_staged_w<double, false, 1>(L, W, a, b, P, dops, dstages, nstages, dmats, dterms, dphase, gpart, nslots);
 }
 
 }  // namespace tmfqs_hip

@@ -58,7 +58,7 @@ __device__ __forceinline__ void tile_uk(typename V2T<R>::t *tile, u64 nloc, cons
 // Applies one tile op to a tile of 2^T amplitudes held in LDS.  `base` is the
 // global index of the tile's element 0; local index l maps to global
 // base | (l & lowmask) | hitab[l >> C].
-template <typename R>
+template <typename R, int MAXK = 5>
 __device__ __forceinline__ void tile_apply(typename V2T<R>::t *tile, const TileOp &op, u64 nloc, u64 base, unsigned C,
                                            const u64 *hitab, const double *__restrict__ mats,
                                            const TwiddleTerm *__restrict__ terms, const double2 *__restrict__ ptab) {
@@ -118,7 +118,9 @@ __device__ __forceinline__ void tile_apply(typename V2T<R>::t *tile, const TileO
 				case 2: tile_uk<R, 2>(tile, nloc, op, mats); break;
 				case 3: tile_uk<R, 3>(tile, nloc, op, mats); break;
 				case 4: tile_uk<R, 4>(tile, nloc, op, mats); break;
-				case 5: tile_uk<R, 5>(tile, nloc, op, mats); break;
+				case 5:
+					if constexpr(MAXK >= 5) tile_uk<R, 5>(tile, nloc, op, mats);
+					break;
 				default: break;
 			}
 			break;
@@ -403,9 +405,84 @@ __device__ void tile_mfma5(float2 *tile, const TileOp &op, unsigned T, const dou
 	}
 }
 
+// complex128 five-qubit matrix on an LDS tile with FP64 MFMA (16 x 16 x 4,
+// FP64 accumulation).  As in tile_mfma5, Y = [[Mr, -Mi], [Mi, Mr]] [Xr; Xi]
+// is a real 64 x 64 by 64 x 2^(T-5) product; wavefront w owns output rows
+// 16 w .. 16 w + 15 (w = 0, 1: real parts of amplitude rows 0..15, 16..31;
+// w = 2, 3: imaginary parts) and keeps its 16 A operands (one per K step of
+// four rows) in registers.  Columns are blocks of 32 amplitudes taken sixteen
+// at a time, four such groups in flight per wavefront so consecutive MFMAs
+// do not wait on one accumulator.  K steps s and s + 8 take the real and
+// imaginary part of the same amplitude, so one LDS read serves both.
+// Operand layout (v_mfma_f64_16x16x4f64): lane l supplies A[l % 16][l / 16]
+// and B[l / 16][l % 16] and receives D[l / 16 + 4 v][l % 16], v = 0..3.
+// Needs T >= 11 (at least four groups of sixteen blocks) and four wavefronts.
+// Not the dense backend's accumulation order: results agree with it to
+// rounding (the exact mode keeps tile_uk instead).
+typedef double mfma_f64x4 __attribute__((ext_vector_type(4)));
+
+__device__ void tile_mfma5_f64(double2 *tile, const TileOp &op, unsigned T, const double *__restrict__ mats) {
+	const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+	unsigned jl[5], sorted[5];
+#pragma unroll
+	for(int i = 0; i < 5; ++i) jl[i] = sorted[i] = op.jl[i];
+	for(int i = 0; i < 5; ++i)
+		for(int j = i + 1; j < 5; ++j)
+			if(sorted[j] < sorted[i]) {
+				const unsigned t = sorted[i];
+				sorted[i] = sorted[j];
+				sorted[j] = t;
+			}
+	// A[s] = Mbig[16 wave + lane % 16][4 s + lane / 16], prepared by the planner
+	const double2 *a2 = (const double2 *)(mats + op.twid) + (wave * 64u + lane) * 8u;
+	double A[16];
+#pragma unroll
+	for(int i = 0; i < 8; ++i) {
+		const double2 v = a2[i];
+		A[2 * i] = v.x;
+		A[2 * i + 1] = v.y;
+	}
+	const unsigned col = lane & 15u, kq = lane >> 4;
+	const unsigned part = wave >> 1, rbase = 16u * (wave & 1u);
+	const unsigned ngroups = 1u << (T - 9);  // groups of 16 blocks of 32 amplitudes
+	unsigned roff[8], woff[4];
+#pragma unroll
+	for(int s2 = 0; s2 < 8; ++s2) roff[s2] = block_offset(4u * (unsigned)s2 + kq, jl);
+#pragma unroll
+	for(int v = 0; v < 4; ++v) woff[v] = block_offset(rbase + kq + 4u * (unsigned)v, jl);
+	for(unsigned g0 = 0; g0 < ngroups; g0 += 4) {
+		unsigned base[4];
+		mfma_f64x4 acc[4];
+#pragma unroll
+		for(int u = 0; u < 4; ++u) {
+			unsigned b = (g0 + (unsigned)u) * 16u + col;  // this lane's B column (block index)
+#pragma unroll
+			for(int i = 0; i < 5; ++i) b = ((b >> sorted[i]) << (sorted[i] + 1)) | (b & ((1u << sorted[i]) - 1u));
+			base[u] = b;
+			acc[u] = mfma_f64x4{0.0, 0.0, 0.0, 0.0};
+		}
+#pragma unroll
+		for(int s2 = 0; s2 < 8; ++s2) {
+			double2 x[4];
+#pragma unroll
+			for(int u = 0; u < 4; ++u) x[u] = tile[base[u] | roff[s2]];
+#pragma unroll
+			for(int u = 0; u < 4; ++u) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[s2], x[u].x, acc[u], 0, 0, 0);
+#pragma unroll
+			for(int u = 0; u < 4; ++u) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[s2 + 8], x[u].y, acc[u], 0, 0, 0);
+		}
+		__syncthreads();  // every wavefront has read these columns
+#pragma unroll
+		for(int u = 0; u < 4; ++u)
+#pragma unroll
+			for(int v = 0; v < 4; ++v) ((double *)&tile[base[u] | woff[v]])[part] = acc[u][v];
+		__syncthreads();
+	}
+}
+
 // One tile of a staged pass: loads it into LDS, runs every stage, stores it.
 // ta/tb/hitab/gacc are the workgroup's LDS buffers; hitab must hold P's table.
-template <typename R, int W, bool DUAL, bool LDSOPS>
+template <typename R, int W, bool DUAL, int LDS>
 __device__ __forceinline__ void staged_tile(typename V2T<R>::t *__restrict__ a, typename V2T<R>::t *__restrict__ b,
                                             const TilePass &P, u64 tid, const TileOp *__restrict__ ops,
                                             const TileStage *__restrict__ stages, unsigned nstages,
@@ -414,7 +491,7 @@ __device__ __forceinline__ void staged_tile(typename V2T<R>::t *__restrict__ a, 
                                             typename V2T<R>::t *tb, const u64 *hitab, double *gacc) {
 	using V = typename V2T<R>::t;
 	constexpr int NR = 1 << W;
-	constexpr bool PAD = !LDSOPS;
+	constexpr bool PAD = LDS == 0;
 	const unsigned nloc = 1u << P.T;
 	const unsigned C = P.C;
 	const u64 lowmask = (1ull << C) - 1ull;
@@ -452,7 +529,7 @@ __device__ __forceinline__ void staged_tile(typename V2T<R>::t *__restrict__ a, 
 	__syncthreads();
 	for(unsigned s = 0; s < nstages; ++s) {
 		const TileStage st = stages[s];
-		if(LDSOPS && st.lds) {  // k-qubit matrices: their registers would cap every other stage's occupancy
+		if(LDS && st.lds) {  // k-qubit matrices: their registers would cap every other stage's occupancy
 			for(unsigned o = 0; o < st.nops; ++o) {
 				const TileOp op = ops[st.op0 + o];
 				if((tbase & op.gcond) != op.gcond) continue;  // uniform across the block
@@ -466,9 +543,17 @@ __device__ __forceinline__ void staged_tile(typename V2T<R>::t *__restrict__ a, 
 						__syncthreads();
 						continue;
 					}
+				} else if constexpr(LDS == 2) {
+					// the planner picks this build only for tiles of >= 11 bits whose
+					// five-qubit blocks all carry MFMA operands
+					if(op.kind == TOP_UK && op.k == 5) {
+						tile_mfma5_f64((double2 *)ta, op, P.T, mats);
+						if(DUAL) tile_mfma5_f64((double2 *)tb, op, P.T, mats);
+						continue;
+					}
 				}
-				tile_apply<R>(ta, op, nloc, tbase, C, hitab, mats, terms, ptab);
-				if(DUAL) tile_apply<R>(tb, op, nloc, tbase, C, hitab, mats, terms, ptab);
+				tile_apply<R, LDS == 2 ? 4 : 5>(ta, op, nloc, tbase, C, hitab, mats, terms, ptab);
+				if(DUAL) tile_apply<R, LDS == 2 ? 4 : 5>(tb, op, nloc, tbase, C, hitab, mats, terms, ptab);
 				__syncthreads();
 			}
 			continue;
@@ -555,7 +640,7 @@ __host__ __device__ __forceinline__ unsigned staged_tile_elems(unsigned T, bool 
 	return ldsops ? (1u << T) : (1u << T) + ((1u << T) >> 4);
 }
 
-template <typename R, int W, bool DUAL, bool LDSOPS>
+template <typename R, int W, bool DUAL, int LDS>
 __global__ void __launch_bounds__(256) k_tile_staged(typename V2T<R>::t *__restrict__ a, typename V2T<R>::t *__restrict__ b,
                                                      TilePass P, const TileOp *__restrict__ ops,
                                                      const TileStage *__restrict__ stages, unsigned nstages,
@@ -564,7 +649,7 @@ __global__ void __launch_bounds__(256) k_tile_staged(typename V2T<R>::t *__restr
                                                      unsigned nslots) {
 	using V = typename V2T<R>::t;
 	extern __shared__ char smem[];
-	const unsigned nlocp = staged_tile_elems(P.T, LDSOPS);
+	const unsigned nlocp = staged_tile_elems(P.T, LDS != 0);
 	V *ta = (V *)smem;
 	V *tb = ta + nlocp;  // second tile (dual passes only)
 	u64 *hitab = (u64 *)(ta + (DUAL ? 2u : 1u) * nlocp);
@@ -573,7 +658,7 @@ __global__ void __launch_bounds__(256) k_tile_staged(typename V2T<R>::t *__restr
 	if(DUAL)
 		for(unsigned i = threadIdx.x; i < 4 * kMaxGradSlots; i += blockDim.x) gacc[i] = 0.0;
 	for(u64 tid = blockIdx.x; tid < P.ntiles; tid += gridDim.x)
-		staged_tile<R, W, DUAL, LDSOPS>(a, b, P, tid, ops, stages, nstages, mats, terms, ptab, ta, tb, hitab, gacc);
+		staged_tile<R, W, DUAL, LDS>(a, b, P, tid, ops, stages, nstages, mats, terms, ptab, ta, tb, hitab, gacc);
 	if(DUAL) {
 		__syncthreads();
 		for(unsigned s = threadIdx.x; s < nslots; s += blockDim.x)
@@ -582,31 +667,31 @@ __global__ void __launch_bounds__(256) k_tile_staged(typename V2T<R>::t *__restr
 	}
 }
 
-template <typename R, int W, bool DUAL, bool LDSOPS>
+template <typename R, int W, bool DUAL, int LDS>
 static hipError_t launch_staged_t(const Launch &L, void *a, void *b, const TilePass &P, const TileOp *dops, const TileStage *dstages,
                                   unsigned nstages, const double *dmats, const TwiddleTerm *dterms, const double *dphase,
                                   double *gpart, unsigned nslots) {
 	using V = typename V2T<R>::t;
-	const size_t nlocp = staged_tile_elems(P.T, LDSOPS);
+	const size_t nlocp = staged_tile_elems(P.T, LDS != 0);
 	const size_t shmem = (DUAL ? 2 : 1) * sizeof(V) * nlocp + (8ull << P.nh) + (DUAL ? 4 * kMaxGradSlots * sizeof(double) : 0);
 	if(shmem > 160 * 1024) return hipErrorInvalidValue;
 	static bool attr = false;
 	if(!attr) {
-		(void)hipFuncSetAttribute((const void *)k_tile_staged<R, W, DUAL, LDSOPS>, hipFuncAttributeMaxDynamicSharedMemorySize,
+		(void)hipFuncSetAttribute((const void *)k_tile_staged<R, W, DUAL, LDS>, hipFuncAttributeMaxDynamicSharedMemorySize,
 		                          160 * 1024);
 		attr = true;
 	}
-	TMFQS_LAUNCH((k_tile_staged<R, W, DUAL, LDSOPS>), tile_grid(P, DUAL, L.prec), 256, shmem, L.stream, (V *)a, (V *)b, P, dops, dstages, nstages,
+	TMFQS_LAUNCH((k_tile_staged<R, W, DUAL, LDS>), tile_grid(P, DUAL, L.prec), 256, shmem, L.stream, (V *)a, (V *)b, P, dops, dstages, nstages,
 	             dmats, dterms, (const double2 *)dphase, gpart, nslots);
 	return hipGetLastError();
 }
 
-template <typename R, bool DUAL, bool LDSOPS>
+template <typename R, bool DUAL, int LDS>
 static hipError_t launch_staged_w(const Launch &L, unsigned W, void *a, void *b, const TilePass &P, const TileOp *dops,
                                   const TileStage *dstages, unsigned nstages, const double *dmats, const TwiddleTerm *dterms,
                                   const double *dphase, double *gpart, unsigned nslots) {
 #define TMFQS_STAGED(WW) \
-	launch_staged_t<R, WW, DUAL, LDSOPS>(L, a, b, P, dops, dstages, nstages, dmats, dterms, dphase, gpart, nslots)
+	launch_staged_t<R, WW, DUAL, LDS>(L, a, b, P, dops, dstages, nstages, dmats, dterms, dphase, gpart, nslots)
 	switch(W) {
 		case 1: return TMFQS_STAGED(1);
 		case 2: return TMFQS_STAGED(2);
@@ -627,6 +712,8 @@ TMFQS_TILE_ENTRY(launch_tile_c128);
 TMFQS_TILE_ENTRY(launch_tile_c128_lds);
 TMFQS_TILE_ENTRY(launch_tile_c128_dual);
 TMFQS_TILE_ENTRY(launch_tile_c128_dual_lds);
+TMFQS_TILE_ENTRY(launch_tile_c128_mfma);
+TMFQS_TILE_ENTRY(launch_tile_c128_dual_mfma);
 TMFQS_TILE_ENTRY(launch_tile_c64);
 TMFQS_TILE_ENTRY(launch_tile_c64_lds);
 TMFQS_TILE_ENTRY(launch_tile_c64_dual);
