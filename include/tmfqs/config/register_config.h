@@ -78,7 +78,9 @@ struct HipConfig {
 	unsigned int coalesceQubits = 0;
 	/** Most Grover/affine steps folded into one sweep (0: automatic). */
 	unsigned int groverBlock = 0;
-	/** Apply every controlled phase as its own product (bit-identical to the dense backend). */
+	/** The dense backend's arithmetic bit for bit everywhere: every controlled
+	 *  phase as its own product and five-qubit blocks in its accumulation
+	 *  order on VALU (otherwise fused ladders and MFMA blocks, within 1e-12). */
 	bool exactPhases = false;
 };
 

@@ -63,8 +63,10 @@ typedef struct tmfqs_hip_config {
 	int coalesce_qubits; /* low qubits always inside a tile (0: automatic) */
 	int grover_block;    /* most elementwise steps folded into one sweep (0: 1024 for complex128, which replays
 	                        every step's rounding; complex64 composes up to 65536 steps into one affine map) */
-	int exact_phases;    /* 1: apply every controlled phase as its own complex product
-	                        (bit-identical to the reference); 0: fuse QFT phase ladders */
+	int exact_phases;    /* 1: the reference's arithmetic bit for bit everywhere: every controlled
+	                        phase as its own complex product, five-qubit blocks accumulated in
+	                        the dense backend's order on VALU; 0: fuse QFT phase ladders and run
+	                        five-qubit blocks on MFMA (both within 1e-12 of the reference) */
 } tmfqs_hip_config;
 
 /* One circuit operation (the reference's circuit::Operation variant,
