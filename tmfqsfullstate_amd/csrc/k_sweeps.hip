@@ -413,61 +413,90 @@ __global__ void __launch_bounds__(256) k_chunk_sum(const typename V2T<R>::t *__r
 // Per-chunk integer increments of the sequential chain S = fl(S + x_i) on the
 // grid u = 2^ulog[c]: while every partial sum stays inside one binade of that
 // ulp, each step adds rint(x_i / u) ulps exactly, unless x_i / u ends in
-// exactly one half (a tie, flagged).  Each thread walks a run of consecutive
-// states so that the chunk's prefix minimum and maximum (in ulps, relative to
-// the chunk start) come out in state order.
+// exactly one half (a tie, flagged).  The chunk is read with coalesced loads
+// (lane-consecutive states) into an LDS array of ticks; each thread then
+// walks a run of consecutive ticks, and a workgroup scan of the runs gives
+// the chunk's total and its prefix minimum and maximum (in ulps, relative to
+// the chunk start) in state order.
+constexpr unsigned kTickThreads = 256;
+__device__ __forceinline__ unsigned tick_slot(unsigned i) { return i + (i >> 4); }  // skew: 2-way LDS banks at most
+
+__device__ __forceinline__ long long shfl_up_ll(long long v, unsigned d) { return __shfl_up(v, d, 64); }
+
 template <typename R>
 __global__ void __launch_bounds__(256) k_chunk_ticks(const typename V2T<R>::t *__restrict__ a, u64 chunk, u64 nchunks,
                                                      const double *af, int sel, const int32_t *ulog, ChunkTicks *out) {
-	__shared__ long long sh[256];
-	__shared__ long long smin[256];
-	__shared__ long long smax[256];
-	__shared__ int sflag[256];
-	const unsigned T = blockDim.x;
-	const u64 per = chunk >= T ? chunk / T : 1;
-	const bool active = chunk >= T || threadIdx.x < chunk;
+	__shared__ long long tk[(1u << kSampleChunkLog) + ((1u << kSampleChunkLog) >> 4)];
+	__shared__ long long wsum[kTickThreads / 64], wmin[kTickThreads / 64], wmax[kTickThreads / 64];
+	__shared__ int wflag[kTickThreads / 64];
+	const unsigned t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+	const unsigned per = chunk >= kTickThreads ? (unsigned)(chunk / kTickThreads) : 1u;
+	const bool active = chunk >= kTickThreads || t < chunk;
 	for(u64 c = blockIdx.x; c < nchunks; c += gridDim.x) {
 		const int32_t ul = ulog[c];
-		long long cum = 0, lmin = LLONG_MAX, lmax = LLONG_MIN;
+		const bool grid = ul != INT32_MIN;
 		int tie = 0, nz = 0;
-		if(active) {
-			const u64 first = c * chunk + (u64)threadIdx.x * per;
-			for(u64 j = 0; j < per; ++j) {
-				const auto v = a[first + j];
+		// 1. coalesced: lane-consecutive states, their ticks into LDS
+		if(active)
+			for(unsigned j = 0; j < per; ++j) {
+				const unsigned i = j * kTickThreads + t;
+				const auto v = a[c * chunk + i];
 				const double x = chain_value(v.x, v.y, sel, af);
 				nz |= (x != 0.0);
-				if(ul != INT32_MIN) {
+				if(grid) {
 					const double q = ldexp(x, -ul);
 					if(q - floor(q) == 0.5) tie = 1;
-					cum += (long long)rint(q);
-					lmin = cum < lmin ? cum : lmin;
-					lmax = cum > lmax ? cum : lmax;
+					tk[tick_slot(i)] = (long long)rint(q);
 				}
 			}
-		}
-		sh[threadIdx.x] = cum;
-		smin[threadIdx.x] = lmin;
-		smax[threadIdx.x] = lmax;
-		sflag[threadIdx.x] = tie | (nz << 1);
 		__syncthreads();
-		if(threadIdx.x == 0) {
-			long long run = 0, gmin = LLONG_MAX, gmax = LLONG_MIN;
-			int anyTie = 0, anyNz = 0;
-			for(unsigned t = 0; t < T; ++t) {
-				if(smin[t] != LLONG_MAX) {
-					gmin = run + smin[t] < gmin ? run + smin[t] : gmin;
-					gmax = run + smax[t] > gmax ? run + smax[t] : gmax;
-				}
-				run += sh[t];
-				anyTie |= sflag[t] & 1;
-				anyNz |= sflag[t] >> 1;
+		// 2. each thread's run of consecutive states, in state order
+		long long cum = 0, lmin = LLONG_MAX, lmax = LLONG_MIN;
+		if(active && grid)
+			for(unsigned j = 0; j < per; ++j) {
+				cum += tk[tick_slot(t * per + j)];
+				lmin = cum < lmin ? cum : lmin;
+				lmax = cum > lmax ? cum : lmax;
+			}
+		// 3. exclusive prefix of the runs over the workgroup, then the extremes
+		long long inc = cum;
+		for(unsigned d = 1; d < 64; d <<= 1) {
+			const long long o = shfl_up_ll(inc, d);
+			if(lane >= d) inc += o;
+		}
+		if(lane == 63) wsum[wave] = inc;
+		__syncthreads();
+		long long before = inc - cum;
+		for(unsigned w = 0; w < wave; ++w) before += wsum[w];
+		long long mn = lmin == LLONG_MAX ? LLONG_MAX : before + lmin;
+		long long mx = lmax == LLONG_MIN ? LLONG_MIN : before + lmax;
+		for(unsigned d = 32; d > 0; d >>= 1) {
+			const long long omn = __shfl_down(mn, d, 64), omx = __shfl_down(mx, d, 64);
+			mn = omn < mn ? omn : mn;
+			mx = omx > mx ? omx : mx;
+		}
+		const int anyTie = __any(tie) ? 1 : 0, anyNz = __any(nz) ? 1 : 0;
+		if(lane == 0) {
+			wmin[wave] = mn;
+			wmax[wave] = mx;
+			wflag[wave] = anyTie | (anyNz << 1);
+		}
+		__syncthreads();
+		if(t == 0) {
+			long long total = 0, gmin = LLONG_MAX, gmax = LLONG_MIN;
+			int flags = 0;
+			for(unsigned w = 0; w < kTickThreads / 64; ++w) {
+				total += wsum[w];
+				gmin = wmin[w] < gmin ? wmin[w] : gmin;
+				gmax = wmax[w] > gmax ? wmax[w] : gmax;
+				flags |= wflag[w];
 			}
 			ChunkTicks o;
-			o.ticks = run;
+			o.ticks = total;
 			o.pmin = gmin == LLONG_MAX ? 0 : gmin;
 			o.pmax = gmax == LLONG_MIN ? 0 : gmax;
-			o.tie = anyTie;
-			o.nonzero = anyNz;
+			o.tie = flags & 1;
+			o.nonzero = flags >> 1;
 			out[c] = o;
 		}
 		__syncthreads();
