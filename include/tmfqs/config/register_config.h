@@ -82,6 +82,11 @@ struct HipConfig {
 	 *  phase as its own product and five-qubit blocks in its accumulation
 	 *  order on VALU (otherwise fused ladders and MFMA blocks, within 1e-12). */
 	bool exactPhases = false;
+	/** Gate-run composition width: runs of consecutive gates on at most this
+	 *  many qubits (2..5) become one dense block (0: off). */
+	unsigned int fuseQubits = 0;
+	/** Fewest gates a run needs to be composed (0: 24). */
+	unsigned int fuseMinGates = 0;
 };
 
 struct HipTuningOverrides {

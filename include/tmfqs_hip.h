@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define TMFQS_HIP_ABI_VERSION 1
+#define TMFQS_HIP_ABI_VERSION 2
 
 typedef struct tmfqs_hip_state tmfqs_hip_state;
 
@@ -67,6 +67,11 @@ typedef struct tmfqs_hip_config {
 	                        phase as its own complex product, five-qubit blocks accumulated in
 	                        the dense backend's order on VALU; 0: fuse QFT phase ladders and run
 	                        five-qubit blocks on MFMA (both within 1e-12 of the reference) */
+	int fuse_qubits;     /* gate-run composition: a run of consecutive gates acting on at most this many
+	                        qubits (2..5) is multiplied into one dense matrix on the host in FP64 and
+	                        applied as one k-qubit block (five-qubit blocks on MFMA); 0: off.  Not in
+	                        the exact mode.  tmfqs_hip_tune chooses it per circuit. */
+	int fuse_min_gates;  /* fewest gates a run needs to be composed (0: 24) */
 } tmfqs_hip_config;
 
 /* One circuit operation (the reference's circuit::Operation variant,
@@ -242,6 +247,21 @@ typedef struct tmfqs_hip_diag_term {
 	double coef;
 	uint64_t bits;      /* Z: qubit mask; PROJECTOR: basis state */
 } tmfqs_hip_diag_term;
+
+/* No counterpart in the reference (its tuning hook,
+ * StorageStrategyRegistry::tuneConfig in
+ * src/tmfqs/storage/factory/storage_strategy_registry.cpp:77-85, returns its
+ * input unchanged).  Autotuner: times the gate list (H, X, CX, CPHASE, PHASE,
+ * SWAP, MATRIX records) on a scratch copy of the handle's state for every
+ * candidate of the tile width (automatic, default - 1, default) and the
+ * gate-run composition width fuse_qubits (0, 3, 4, 5), two runs each, and
+ * keeps the fastest in the handle's configuration (fuse_qubits stays as it
+ * was in the exact mode).  The state is unchanged; one extra state buffer is
+ * allocated.  best_ms (may be null) receives the winner's device time. */
+int tmfqs_hip_tune(tmfqs_hip_state *h, const tmfqs_hip_op *ops, size_t count, const double *mat_pool, const unsigned *qubit_pool,
+                   int *tile_qubits, int *fuse_qubits, double *best_ms);
+/* Sets the two tuned parameters directly (a cached tuning result). */
+int tmfqs_hip_set_tuning(tmfqs_hip_state *h, int tile_qubits, int fuse_qubits);
 
 /* <psi|O|psi> of the current state (FP64 accumulation in both precisions). */
 int tmfqs_hip_expectation(tmfqs_hip_state *h, const tmfqs_hip_diag_term *terms, size_t nterms, double *out);

@@ -41,6 +41,8 @@ class Config(ctypes.Structure):
         ("coalesce_qubits", ctypes.c_int),
         ("grover_block", ctypes.c_int),
         ("exact_phases", ctypes.c_int),
+        ("fuse_qubits", ctypes.c_int),
+        ("fuse_min_gates", ctypes.c_int),
     ]
 
 
@@ -72,6 +74,19 @@ def make_terms(terms):
         arr[i].coef = coef
         arr[i].bits = bits
     return arr, len(terms)
+
+
+def circuit_tune_key(n, precision, ops, count):
+    """Key of a tuned single-circuit configuration: precision, qubit count and
+    the circuit's shape (each gate's kind and qubits; angles and matrix
+    entries do not change what is fastest)."""
+    import hashlib
+
+    h = hashlib.sha1(f"circuit;precision={int(precision)};n={int(n)}".encode())
+    for i in range(count):
+        o = ops[i]
+        h.update(f";{o.kind},{o.q0},{o.q1},{o.k}".encode())
+    return h.hexdigest()
 
 
 def _pools(mat_pool, qubit_pool):
@@ -170,6 +185,10 @@ def lib():
         L.tmfqs_hip_qubit_count.argtypes = [vp, ctypes.POINTER(ctypes.c_uint)]
         L.tmfqs_hip_run.argtypes = [vp, ctypes.POINTER(Op), ctypes.c_size_t, _dp, ctypes.POINTER(ctypes.c_uint)]
         L.tmfqs_hip_expectation.argtypes = [vp, ctypes.POINTER(DiagTerm), ctypes.c_size_t, _dp]
+        L.tmfqs_hip_tune.argtypes = [vp, ctypes.POINTER(Op), ctypes.c_size_t, _dp, ctypes.POINTER(ctypes.c_uint),
+                                     ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _dp]
+        L.tmfqs_hip_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.tmfqs_hip_get_config.argtypes = [vp, ctypes.POINTER(Config)]
         L.tmfqs_hip_expectation_device.argtypes = [ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_uint,
                                                    ctypes.POINTER(DiagTerm), ctypes.c_size_t, vp]
         L.tmfqs_hip_adjoint.argtypes = [vp, ctypes.POINTER(Op), ctypes.c_size_t, _dp, ctypes.POINTER(ctypes.c_uint),
@@ -410,6 +429,33 @@ class HipState:
         """tmfqs_hip_run: enqueue op records (see include/tmfqs_hip.h)."""
         mp, qp, mpp, qpp = _pools(mat_pool, qubit_pool)
         check(lib().tmfqs_hip_run(self._h, ops, count, mpp, qpp))
+
+    def tune(self, ops, count, mat_pool=None, qubit_pool=None, cache=None):
+        """tmfqs_hip_tune: times the gate list for every tile width and gate-run
+        composition width on a scratch copy of the state and keeps the fastest;
+        returns (tile_qubits, fuse_qubits, device ms).  With cache (a JSON
+        file path) a result stored for the same precision, qubit count and
+        circuit shape (see circuit_tune_key) is applied without timing, and a
+        new result is stored there."""
+        from .batch import tune_cache_lookup_entry, tune_cache_store
+
+        key = circuit_tune_key(self.n, self.config().precision, ops, count)
+        if cache:
+            entry = tune_cache_lookup_entry(cache, key)
+            if entry is not None:
+                check(lib().tmfqs_hip_set_tuning(self._h, int(entry["tile_qubits"]), int(entry["fuse_qubits"])))
+                return int(entry["tile_qubits"]), int(entry["fuse_qubits"]), float(entry.get("ms", 0.0))
+        mp, qp, mpp, qpp = _pools(mat_pool, qubit_pool)
+        t, f, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        check(lib().tmfqs_hip_tune(self._h, ops, count, mpp, qpp, ctypes.byref(t), ctypes.byref(f), ctypes.byref(ms)))
+        if cache:
+            tune_cache_store(cache, key, t.value, fuse_qubits=f.value, ms=ms.value)
+        return t.value, f.value, ms.value
+
+    def config(self):
+        c = Config()
+        check(lib().tmfqs_hip_get_config(self._h, ctypes.byref(c)))
+        return c
 
     def expectation(self, terms):
         arr, cnt = make_terms(terms)

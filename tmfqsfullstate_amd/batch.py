@@ -93,12 +93,17 @@ def tune_key(circuits, precision):
     return h.hexdigest()
 
 
-def tune_cache_lookup(path, key):
-    """Tile width stored under `key` in the JSON file `path`, or None."""
+def tune_cache_lookup_entry(path, key):
+    """The entry stored under `key` in the JSON file `path`, or None."""
     if not path or not os.path.exists(path):
         return None
     with open(path) as f:
-        entry = json.load(f).get(key)
+        return json.load(f).get(key)
+
+
+def tune_cache_lookup(path, key):
+    """Tile width stored under `key` in the JSON file `path`, or None."""
+    entry = tune_cache_lookup_entry(path, key)
     return None if entry is None else int(entry["tile_qubits"])
 
 

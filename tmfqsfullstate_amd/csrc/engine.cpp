@@ -12,6 +12,7 @@
 #include <chrono>
 #include <climits>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -1277,6 +1278,165 @@ static size_t qft_unit_end(const std::vector<QOp> &in, size_t s, size_t hi, unsi
 	return k;
 }
 
+// ------------------------------------------------------------ gate-run composition
+// Physical bits a gate acts on; false for operations a composed run cannot
+// hold (swaps are relabelings handled by the planner, elementwise steps touch
+// every amplitude).
+bool gate_support(const QOp &op, u64 &support) {
+	switch(op.kind) {
+		case Q_H:
+		case Q_X:
+		case Q_U1:
+		case Q_PH: support = 1ull << op.b0; return true;
+		case Q_CX:
+		case Q_CP: support = (1ull << op.b0) | (1ull << op.b1); return true;
+		case Q_UK:
+			support = 0;
+			for(unsigned j = 0; j < op.k; ++j) support |= 1ull << op.bits[j];
+			return true;
+		default: return false;
+	}
+}
+
+// Applies gate op to v, a vector over the 2^k local indices of bits
+// (bits[0] is the index MSB), in FP64: the columns of a composed matrix.
+void apply_local(const QOp &op, const std::vector<double> &qmats, const unsigned *bits, unsigned k,
+                 std::complex<double> *v) {
+	using C = std::complex<double>;
+	auto lm = [&](unsigned p) -> unsigned {
+		for(unsigned i = 0; i < k; ++i)
+			if(bits[i] == p) return 1u << (k - 1 - i);
+		return 0;
+	};
+	const unsigned D = 1u << k;
+	switch(op.kind) {
+		case Q_H: {
+			const unsigned m = lm(op.b0);
+			const double c = 1.0 / std::sqrt(2.0);
+			for(unsigned r = 0; r < D; ++r)
+				if(!(r & m)) {
+					const C a = v[r], b = v[r | m];
+					v[r] = (a + b) * c;
+					v[r | m] = (a - b) * c;
+				}
+			break;
+		}
+		case Q_X:
+		case Q_CX: {
+			const unsigned m = lm(op.kind == Q_X ? op.b0 : op.b1), cm = op.kind == Q_X ? 0u : lm(op.b0);
+			for(unsigned r = 0; r < D; ++r)
+				if(!(r & m) && (r & cm) == cm) std::swap(v[r], v[r | m]);
+			break;
+		}
+		case Q_CP:
+		case Q_PH: {
+			const unsigned m = op.kind == Q_PH ? lm(op.b0) : (lm(op.b0) | lm(op.b1));
+			const C f(op.cr, op.ci);
+			for(unsigned r = 0; r < D; ++r)
+				if((r & m) == m) v[r] *= f;
+			break;
+		}
+		case Q_U1:
+		case Q_UK: {
+			const unsigned kg = op.kind == Q_U1 ? 1u : op.k;
+			const unsigned Dg = 1u << kg;
+			unsigned gm[5];
+			for(unsigned j = 0; j < kg; ++j) gm[j] = lm(op.kind == Q_U1 ? op.b0 : op.bits[j]);
+			unsigned all = 0;
+			for(unsigned j = 0; j < kg; ++j) all |= gm[j];
+			const double *g = qmats.data() + op.mat;
+			std::vector<C> x(Dg);
+			std::vector<unsigned> idx(Dg);
+			for(unsigned base = 0; base < D; ++base) {
+				if(base & all) continue;
+				for(unsigned r = 0; r < Dg; ++r) {
+					unsigned off = 0;
+					for(unsigned j = 0; j < kg; ++j)
+						if((r >> (kg - 1 - j)) & 1u) off |= gm[j];
+					idx[r] = base | off;
+					x[r] = v[idx[r]];
+				}
+				for(unsigned r = 0; r < Dg; ++r) {
+					C acc(0.0, 0.0);
+					for(unsigned c = 0; c < Dg; ++c) acc += C(g[2 * (r * Dg + c)], g[2 * (r * Dg + c) + 1]) * x[c];
+					v[idx[r]] = acc;
+				}
+			}
+			break;
+		}
+		default: break;
+	}
+}
+
+// Replaces every run of consecutive gates that acts on at most
+// cfg.fuse_qubits qubits and holds at least cfg.fuse_min_gates gates (one of
+// them not diagonal) by one dense block: the product of the run's matrices,
+// formed column by column in FP64 on the host (the matrix product of
+// src/tmfqs/gates/quantum_gate.cpp:59-77, gate after gate).  Five-qubit
+// composition pads smaller runs with low bits up to five qubits, so blocks
+// take the MFMA path.  Returns false when no run qualifies.
+bool compose_gate_runs(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_t hi, std::vector<QOp> &out) {
+	const unsigned kmax = (unsigned)h->cfg.fuse_qubits;
+	const size_t minGates = h->cfg.fuse_min_gates > 0 ? (size_t)h->cfg.fuse_min_gates : 24;
+	const bool pad5 = kmax == 5 && h->n >= (h->prec == 0 ? 11u : 10u);
+	out.clear();
+	bool any = false;
+	size_t s = lo;
+	while(s < hi) {
+		u64 sup = 0, Q = 0;
+		if(!gate_support(in[s], sup)) {
+			out.push_back(in[s++]);
+			continue;
+		}
+		size_t e = s;
+		bool nondiag = false;
+		while(e < hi && gate_support(in[e], sup) && (unsigned)__builtin_popcountll(Q | sup) <= kmax) {
+			Q |= sup;
+			nondiag = nondiag || !(in[e].kind == Q_CP || in[e].kind == Q_PH);
+			++e;
+		}
+		if(e - s < minGates || !nondiag) {
+			out.insert(out.end(), in.begin() + s, in.begin() + e);
+			s = e;
+			continue;
+		}
+		for(unsigned b = 0; pad5 && __builtin_popcountll(Q) < 5 && b < h->n; ++b) Q |= 1ull << b;
+		unsigned bits[5], k = 0;
+		for(int b = 63; b >= 0; --b)
+			if((Q >> b) & 1ull) bits[k++] = (unsigned)b;
+		const unsigned D = 1u << k;
+		std::vector<std::complex<double>> col(D);
+		std::vector<double> M(2 * (size_t)D * D);
+		for(unsigned c = 0; c < D; ++c) {
+			std::fill(col.begin(), col.end(), std::complex<double>(0.0, 0.0));
+			col[c] = 1.0;
+			for(size_t i = s; i < e; ++i) apply_local(in[i], h->qmats, bits, k, col.data());
+			for(unsigned r = 0; r < D; ++r) {
+				M[2 * ((size_t)r * D + c)] = col[r].real();
+				M[2 * ((size_t)r * D + c) + 1] = col[r].imag();
+			}
+		}
+		QOp op;
+		op.mat = h->qmats.size();
+		h->qmats.insert(h->qmats.end(), M.begin(), M.end());
+		if(k == 1) {
+			op.kind = Q_U1;
+			op.b0 = bits[0];
+		} else {
+			op.kind = Q_UK;
+			op.k = k;
+			for(unsigned j = 0; j < k; ++j) op.bits[j] = bits[j];
+		}
+		out.push_back(op);
+		any = true;
+		if(h->profiling) h->prof["gate_runs_composed"].launches += 1;  // a host step: no device time or bytes
+		s = e;
+	}
+	return any;
+}
+
+void exec_planned(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_t hi, const GateTarget *tgt);
+
 void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_t hi, const GateTarget *tgt) {
 	const unsigned n = h->n;
 	if(!tgt && !h->cfg.exact_phases && n >= 12 && n <= 40) {
@@ -1294,6 +1454,21 @@ void exec_gates(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_
 				return;
 			}
 	}
+	if((!tgt || !tgt->b) && !h->cfg.exact_phases && h->cfg.fuse_qubits >= 2 &&
+	   !(n >= 12 && n <= 40 && qft_unit_end(in, lo, hi, n) == hi)) {
+		std::vector<QOp> fused;
+		if(compose_gate_runs(h, in, lo, hi, fused)) {
+			exec_planned(h, fused, 0, fused.size(), tgt);
+			return;
+		}
+	}
+	exec_planned(h, in, lo, hi, tgt);
+}
+
+// Plans a gate run (no complete QFT cut out, runs already composed) into
+// tiled passes and launches them.
+void exec_planned(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, size_t hi, const GateTarget *tgt) {
+	const unsigned n = h->n;
 	void *const A = tgt ? tgt->a : h->amp;
 	void *const B = tgt ? tgt->b : nullptr;
 	const bool dual = B != nullptr;
@@ -2016,6 +2191,9 @@ int tmfqs_hip_create(const tmfqs_hip_config *cfg, tmfqs_hip_state **out) {
 		if(h->cfg.tile_qubits < 0 || h->cfg.tile_qubits > maxT) throw std::invalid_argument("tmfqs_hip_create: tile_qubits out of range");
 		if(h->cfg.coalesce_qubits < 0 || h->cfg.coalesce_qubits > 14)
 			throw std::invalid_argument("tmfqs_hip_create: coalesce_qubits out of range");
+		if(h->cfg.fuse_qubits < 0 || h->cfg.fuse_qubits > 5 || h->cfg.fuse_qubits == 1)
+			throw std::invalid_argument("tmfqs_hip_create: fuse_qubits must be 0 or 2..5");
+		if(h->cfg.fuse_min_gates < 0) throw std::invalid_argument("tmfqs_hip_create: fuse_min_gates out of range");
 		h->prec = h->cfg.precision;
 		h->esz = h->prec == 0 ? 16 : 8;
 		int count = 0;
@@ -2578,6 +2756,89 @@ int tmfqs_hip_run(tmfqs_hip_state *h, const tmfqs_hip_op *ops, size_t count, con
 		}
 		throw_status(rc);
 	}
+	ABI_END
+}
+
+int tmfqs_hip_tune(tmfqs_hip_state *h, const tmfqs_hip_op *ops, size_t count, const double *mat_pool, const unsigned *qubit_pool,
+                   int *tile_qubits, int *fuse_qubits, double *best_ms) {
+	ABI_BEGIN
+	checked(h);
+	require_init(h, "tuning");
+	if(count && !ops) throw std::invalid_argument("tmfqs_hip_tune: null operation list");
+	if(!tile_qubits || !fuse_qubits) throw std::invalid_argument("tmfqs_hip_tune: null output pointer");
+	HIPC(hipSetDevice(h->cfg.device));
+	sync_all(h);
+	h->qmats.clear();
+	const std::vector<QOp> q = convert_ops(h, ops, count, mat_pool, qubit_pool, false, nullptr);
+	const size_t qm = h->qmats.size();
+	ensure_scratch(h);
+	const tmfqs_hip_config saved = h->cfg;
+	const int defT = h->prec == 0 ? 12 : 13;
+	GateTarget tg;
+	tg.a = h->scratch;
+	hipEvent_t ev[2] = {nullptr, nullptr};
+	HIPC(hipEventCreate(&ev[0]));
+	HIPC(hipEventCreate(&ev[1]));
+	float best = 0;
+	int bestT = saved.tile_qubits, bestF = saved.fuse_qubits;
+	bool have = false;
+	try {
+		// tile width: automatic (default, or one bit narrower when that needs no
+		// more passes), or either width forced; composition off or on 3..5 qubits
+		for(int T : {0, defT - 1, defT})
+			for(int F : {0, 3, 4, 5}) {
+				if(T > (int)h->n) continue;
+				h->cfg.tile_qubits = T;
+				h->cfg.fuse_qubits = F;
+				h->cfg.exact_phases = 0;
+				float cand = 0;
+				for(int rep = 0; rep < 2; ++rep) {
+					HIPC(launch_copy(h->L(), h->amp, h->scratch, h->n));
+					HIPC(hipEventRecord(ev[0], h->stream));
+					exec_gates(h, q, 0, q.size(), &tg);
+					HIPC(hipEventRecord(ev[1], h->stream));
+					HIPC(hipEventSynchronize(ev[1]));
+					float ms = 0;
+					HIPC(hipEventElapsedTime(&ms, ev[0], ev[1]));
+					cand = rep == 0 ? ms : std::min(cand, ms);
+					h->qmats.resize(qm);  // composed blocks of this run
+				}
+				if(!have || cand < best) {
+					best = cand;
+					bestT = T;
+					bestF = F;
+					have = true;
+				}
+			}
+	} catch(...) {
+		h->cfg = saved;
+		h->qmats.clear();
+		hipEventDestroy(ev[0]);
+		hipEventDestroy(ev[1]);
+		throw;
+	}
+	hipEventDestroy(ev[0]);
+	hipEventDestroy(ev[1]);
+	h->cfg = saved;
+	h->cfg.tile_qubits = bestT;
+	h->cfg.fuse_qubits = saved.exact_phases ? saved.fuse_qubits : bestF;
+	h->qmats.clear();
+	*tile_qubits = bestT;
+	*fuse_qubits = h->cfg.fuse_qubits;
+	if(best_ms) *best_ms = best;
+	ABI_END
+}
+
+int tmfqs_hip_set_tuning(tmfqs_hip_state *h, int tile_qubits, int fuse_qubits) {
+	ABI_BEGIN
+	checked(h);
+	const int maxT = h->prec == 0 ? 13 : 14;
+	if(tile_qubits < 0 || tile_qubits > maxT) throw std::invalid_argument("tmfqs_hip_set_tuning: tile_qubits out of range");
+	if(fuse_qubits < 0 || fuse_qubits > 5 || fuse_qubits == 1)
+		throw std::invalid_argument("tmfqs_hip_set_tuning: fuse_qubits must be 0 or 2..5");
+	flush(h);
+	h->cfg.tile_qubits = tile_qubits;
+	h->cfg.fuse_qubits = fuse_qubits;
 	ABI_END
 }
 

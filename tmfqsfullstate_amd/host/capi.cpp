@@ -53,6 +53,8 @@ tmfqs::RegisterConfig regConfig(const tmfqs_hip_config *c) {
 	cfg.hip.coalesceQubits = static_cast<unsigned>(c->coalesce_qubits);
 	cfg.hip.groverBlock = static_cast<unsigned>(c->grover_block);
 	cfg.hip.exactPhases = c->exact_phases != 0;
+	cfg.hip.fuseQubits = static_cast<unsigned>(c->fuse_qubits);
+	cfg.hip.fuseMinGates = static_cast<unsigned>(c->fuse_min_gates);
 	return cfg;
 }
 

@@ -43,6 +43,8 @@ tmfqs_hip_config toHipConfig(const HipConfig &c) {
 	h.coalesce_qubits = static_cast<int>(c.coalesceQubits);
 	h.grover_block = static_cast<int>(c.groverBlock);
 	h.exact_phases = c.exactPhases ? 1 : 0;
+	h.fuse_qubits = static_cast<int>(c.fuseQubits);
+	h.fuse_min_gates = static_cast<int>(c.fuseMinGates);
 	return h;
 }
 
