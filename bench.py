@@ -296,6 +296,84 @@ def time_blocks(T, n=30, gates=8, seed=5):
     return out
 
 
+def composed_circuit(T, n, seed=9, depth=400, width=5):
+    """A deep run of gates on `width` random qubits (random 1-qubit unitaries,
+    CX, H, controlled phases, phases) after a Hadamard layer on all n qubits:
+    the shape gate-run composition is for."""
+    rng = np.random.default_rng(seed)
+    sub = [int(q) for q in rng.choice(n, width, replace=False)]
+    seq, mats, quals = [], [], []
+    for q in range(n):
+        seq.append((T.OP_H, q, 0, 0.0, 0, 0))
+    for _ in range(depth):
+        k = int(rng.integers(0, 5))
+        a, b = (int(x) for x in rng.choice(sub, 2, replace=False))
+        if k == 0:
+            z = rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))
+            u, _ = np.linalg.qr(z)
+            seq.append((T.OP_MATRIX, len(quals), 0, 0.0, 2 * 4 * len(mats), 1))
+            quals.append(a)
+            mats.append(u)
+        elif k == 1:
+            seq.append((T.OP_CX, a, b, 0.0, 0, 0))
+        elif k == 2:
+            seq.append((T.OP_H, a, 0, 0.0, 0, 0))
+        elif k == 3:
+            seq.append((T.OP_CPHASE, a, b, float(rng.uniform(0, 2 * np.pi)), 0, 0))
+        else:
+            seq.append((T.OP_PHASE, a, 0, float(rng.uniform(0, 2 * np.pi)), 0, 0))
+    arr = (T.Op * len(seq))()
+    for i, (kind, q0, q1, th, st, k) in enumerate(seq):
+        arr[i].kind, arr[i].q0, arr[i].q1, arr[i].theta, arr[i].state, arr[i].k = kind, q0, q1, th, st, k
+    pool = np.zeros(8 * len(mats))
+    for i, u in enumerate(mats):
+        pool[8 * i:8 * i + 8:2] = u.real.ravel()
+        pool[8 * i + 1:8 * i + 8:2] = u.imag.ravel()
+    return arr, len(seq), pool, np.array(quals, dtype=np.uint32)
+
+
+def time_composition(T, n=30, reps=2):
+    """Gate-run composition on the deep five-qubit run: device time for each
+    composition width (0 = every gate applied in register stages), the
+    autotuner's choice, and the error of the composed result against the
+    uncomposed complex128 one."""
+    arr, cnt, pool, quals = composed_circuit(T, n)
+    out = {"qubits": n, "gates": cnt}
+    L = T.lib()
+    for prec in (T.C128, T.C64):
+        name = "complex128" if prec == T.C128 else "complex64"
+        sweep = {}
+        states = {}
+        with T.HipState(precision=prec) as s:
+            for fq in (0, 3, 4, 5):
+                T.check(L.tmfqs_hip_set_tuning(s._h, 0, fq))
+                best = None
+                for _ in range(reps):
+                    s.init_basis(n, 0)
+                    s.synchronize()
+                    s.profile(True)
+                    s.profile_reset()
+                    s.run(arr, cnt, pool, quals)
+                    s.synchronize()
+                    p = s.profile_read()
+                    s.profile(False)
+                    ms = sum(v["ms"] for v in p.values())
+                    best = ms if best is None else min(best, ms)
+                sweep[fq] = {"device_ms": best, "passes": sum(v["launches"] for k, v in p.items() if k.startswith("k_")),
+                             "composed_runs": p.get("gate_runs_composed", {}).get("launches", 0)}
+                if fq in (0, 5):
+                    states[fq] = s.clone()
+            s.init_basis(n, 0)
+            t, f, ms = s.tune(arr, cnt, pool, quals)
+        cmp = states[0].compare(states[5])
+        for st in states.values():
+            st.close()
+        out[name] = {"device_ms_by_fuse_qubits": sweep, "tuned": {"tile_qubits": t, "fuse_qubits": f, "device_ms": ms},
+                     "speedup_fuse5_vs_off": sweep[0]["device_ms"] / sweep[5]["device_ms"],
+                     "max_abs_diff_fuse5_vs_off": cmp["max_abs_amplitude_error"]}
+    return out
+
+
 def time_batch(T, precision, steps=3):
     """The committed seed-42 workload (256 circuits, 10-22 qubits) as one
     batch (autotuned tile width, one graph launch per run) against the same
@@ -611,6 +689,16 @@ def main():
         log(f"blocks: complex64 {blk['complex64']['seconds'] * 1e3:.1f} ms, complex128 "
             f"{blk['complex128']['seconds'] * 1e3:.1f} ms, fidelity {blk['fidelity_complex64_vs_complex128']:.12f}")
         result["five_qubit_blocks"] = blk
+
+    # ---- gate-run composition and the circuit autotuner
+    if not args.skip_c64:
+        log(f"gate-run composition: a deep five-qubit run on {n} qubits")
+        comp = time_composition(T, n)
+        log("composition: " + "; ".join(
+            f"{p}: " + ", ".join(f"k={k} {v['device_ms']:.1f} ms" for k, v in comp[p]["device_ms_by_fuse_qubits"].items()) +
+            f" (tuned k={comp[p]['tuned']['fuse_qubits']}, tile {comp[p]['tuned']['tile_qubits']})"
+            for p in ("complex128", "complex64")))
+        result["composition"] = comp
 
     # ---- adjoint gradients: n = 24, 8 ring layers, 192 angles
     if not args.skip_adjoint:
