@@ -263,6 +263,15 @@ int tmfqs_hip_tune(tmfqs_hip_state *h, const tmfqs_hip_op *ops, size_t count, co
 /* Sets the two tuned parameters directly (a cached tuning result). */
 int tmfqs_hip_set_tuning(tmfqs_hip_state *h, int tile_qubits, int fuse_qubits);
 
+/* Per-operation device timing (the traced RunReport of sim::Simulator, whose
+ * reference counterpart times host calls, src/tmfqs/sim/simulator.cpp:71-82):
+ * tmfqs_hip_flush_timed executes everything queued, bracketed by a pair of
+ * stream events, without waiting; tmfqs_hip_timed_read waits for the stream
+ * and returns the device milliseconds of every bracket since the last read,
+ * in order (the first cap of them; count receives how many there were). */
+int tmfqs_hip_flush_timed(tmfqs_hip_state *h);
+int tmfqs_hip_timed_read(tmfqs_hip_state *h, double *ms, size_t cap, size_t *count);
+
 /* <psi|O|psi> of the current state (FP64 accumulation in both precisions). */
 int tmfqs_hip_expectation(tmfqs_hip_state *h, const tmfqs_hip_diag_term *terms, size_t nterms, double *out);
 

@@ -40,10 +40,16 @@ int tmfqs_reg_free(tmfqs_reg *r);
 /* The timed region of Simulator::run (simulator.cpp:66-93) on this register:
  * ops (see tmfqs_hip_op; MATRIX ops take their matrix from mat_pool at
  * op.state doubles and their k targets from mat_qubits at op.q0) inside one
- * operation batch.  seconds receives RunReport.executionSeconds; trace waits
- * for every operation (ObservabilityConfig.traceOperations). */
+ * operation batch.  seconds receives RunReport.executionSeconds; trace runs
+ * every operation on its own (ObservabilityConfig.traceOperations). */
 int tmfqs_reg_run(tmfqs_reg *r, const tmfqs_hip_op *ops, size_t count, const double *mat_pool, const unsigned *mat_qubits,
                   int trace, double *seconds);
+
+/* As tmfqs_reg_run with the per-operation trace on: op_seconds receives
+ * RunReport.operations[i].seconds for each of the count operations (their
+ * device time on a Hip register). */
+int tmfqs_reg_run_traced(tmfqs_reg *r, const tmfqs_hip_op *ops, size_t count, const double *mat_pool, const unsigned *mat_qubits,
+                         double *op_seconds, double *seconds);
 
 /* Full Simulator::run from QuantumState::basis(n, basis) (init_amps == NULL)
  * or QuantumState::fromAmplitudes(n, init_amps, AllowUnnormalized); the result
@@ -84,6 +90,29 @@ int tmfqs_reg_adjoint(tmfqs_reg *r, const tmfqs_hip_op *ops, size_t count, const
                       double *grads, size_t grad_count, double *seconds);
 /* sim::expectation of the register's state. */
 int tmfqs_reg_expectation(tmfqs_reg *r, const tmfqs_hip_diag_term *terms, size_t nterms, double *out);
+
+/* experiment::compareStates (src/tmfqs/experiment/comparison.cpp:12-63) of
+ * two simulations of one circuit: ops run through sim::Simulator from
+ * basis(n, basis) (or init_amps when given) once with ref_cfg and once with
+ * cand_cfg, and the two result states compared (on the device when both are
+ * Hip registers).  out's overlap and fidelity fields are left zero. */
+int tmfqs_sim_compare(unsigned n, uint64_t basis, const double *init_amps, const tmfqs_hip_config *ref_cfg,
+                      const tmfqs_hip_config *cand_cfg, const tmfqs_hip_op *ops, size_t count, const double *mat_pool,
+                      const unsigned *mat_qubits, tmfqs_hip_comparison *out);
+/* experiment::compareStates of two host amplitude vectors (2 * 2^n doubles
+ * each), as QuantumState::fromAmplitudes values. */
+int tmfqs_compare_amplitudes(unsigned n, const double *ref_amps, const double *cand_amps, tmfqs_hip_comparison *out);
+
+/* QuantumGate factories (src/tmfqs/gates/quantum_gate.cpp:96-210 in the
+ * reference), by index: 0 Identity(dim), 1 Hadamard, 2 PauliX, 3 PauliY,
+ * 4 PauliZ, 5 PhaseShift(theta), 6 PiOverEight, 7 ControlledNot, 8 Toffoli,
+ * 9 ControlledPhaseShift(theta), 10 Swap, 11 Ising(theta).  out receives
+ * the row-major interleaved matrix (2 * dim^2 doubles, at most cap), *dim
+ * its dimension. */
+int tmfqs_gate_matrix(int which, double theta, unsigned identity_dim, double *out, size_t cap, unsigned *dim);
+/* QuantumGate products: out = A * B (matrix) or A * (s_re + i s_im) (scalar
+ * when b == NULL); A, B, out row-major interleaved dim x dim. */
+int tmfqs_gate_product(unsigned dim, const double *a, const double *b, double s_re, double s_im, double *out);
 
 #ifdef __cplusplus
 }

@@ -217,4 +217,63 @@ int ref_time_ops_each(unsigned n, const orc_op *ops, size_t count, double *secon
 	}
 }
 
+// QuantumGate factories and products of the reference, in the index order of
+// tmfqs_gate_matrix / tmfqs_gate_product (include/tmfqs_sim.h).
+static tmfqs::QuantumGate refGateFactory(int which, double theta, unsigned identityDim) {
+	using tmfqs::QuantumGate;
+	switch(which) {
+		case 0: return QuantumGate::Identity(identityDim);
+		case 1: return QuantumGate::Hadamard();
+		case 2: return QuantumGate::PauliX();
+		case 3: return QuantumGate::PauliY();
+		case 4: return QuantumGate::PauliZ();
+		case 5: return QuantumGate::PhaseShift(theta);
+		case 6: return QuantumGate::PiOverEight();
+		case 7: return QuantumGate::ControlledNot();
+		case 8: return QuantumGate::Toffoli();
+		case 9: return QuantumGate::ControlledPhaseShift(theta);
+		case 10: return QuantumGate::Swap();
+		case 11: return QuantumGate::Ising(theta);
+		default: throw std::invalid_argument("unknown gate factory");
+	}
+}
+
+static tmfqs::QuantumGate refGateFrom(unsigned dim, const double *m) {
+	tmfqs::QuantumGate g(dim);
+	for(unsigned r = 0; r < dim; ++r)
+		for(unsigned c = 0; c < dim; ++c) g[r][c] = tmfqs::Amplitude{m[2 * (r * dim + c)], m[2 * (r * dim + c) + 1]};
+	return g;
+}
+
+static void refGateTo(const tmfqs::QuantumGate &g, double *out) {
+	const unsigned d = g.dimension();
+	for(unsigned r = 0; r < d; ++r)
+		for(unsigned c = 0; c < d; ++c) {
+			out[2 * (r * d + c)] = g[r][c].real;
+			out[2 * (r * d + c) + 1] = g[r][c].imag;
+		}
+}
+
+int ref_gate_matrix(int which, double theta, unsigned identityDim, double *out, size_t cap, unsigned *dim) {
+	try {
+		const tmfqs::QuantumGate g = refGateFactory(which, theta, identityDim);
+		if(cap < 2 * static_cast<size_t>(g.dimension()) * g.dimension()) return ORC_INVALID_ARGUMENT;
+		refGateTo(g, out);
+		*dim = g.dimension();
+		return 0;
+	} catch(...) {
+		return mapException();
+	}
+}
+
+int ref_gate_product(unsigned dim, const double *a, const double *b, double sre, double sim, double *out) {
+	try {
+		const tmfqs::QuantumGate A = refGateFrom(dim, a);
+		refGateTo(b ? A * refGateFrom(dim, b) : A * tmfqs::Amplitude{sre, sim}, out);
+		return 0;
+	} catch(...) {
+		return mapException();
+	}
+}
+
 } // extern "C"

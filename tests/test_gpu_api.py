@@ -194,3 +194,86 @@ def test_trace_report(tmp_path, monkeypatch):
         assert k["fraction_of_hbm_peak"] == pytest.approx(k["gb_per_s"] / 8000, rel=1e-3, abs=1e-5)
     assert names["k_qft_tail_reverse"]["bytes"] == 2 * amp_bytes
     assert rec["total_ms"] == pytest.approx(sum(k["ms"] for k in rec["kernels"]), rel=1e-5)
+
+
+def sequential_metrics(ref, cand):
+    """experiment::compareStates (reference comparison.cpp:12-63) in state
+    order, restated in Python on interleaved vectors."""
+    r = ref[0::2] + 1j * ref[1::2]
+    c = cand[0::2] + 1j * cand[1::2]
+    m = dict(bitwise_equal=int(np.array_equal(ref, cand)), max_abs_amplitude_error=0.0, max_abs_component_error=0.0,
+             max_abs_probability_error=0.0, worst_state=0)
+    d2 = r2 = rt = ct = 0.0
+    for s in range(r.size):
+        dr, di = c[s].real - r[s].real, c[s].imag - r[s].imag
+        sd = dr * dr + di * di
+        ae = sd ** 0.5
+        if ae > m["max_abs_amplitude_error"]:
+            m["max_abs_amplitude_error"], m["worst_state"] = ae, s
+        m["max_abs_component_error"] = max(m["max_abs_component_error"], abs(dr), abs(di))
+        d2 += sd
+        rp = r[s].real * r[s].real + r[s].imag * r[s].imag
+        cp = c[s].real * c[s].real + c[s].imag * c[s].imag
+        r2 += rp
+        rt += rp
+        ct += cp
+        m["max_abs_probability_error"] = max(m["max_abs_probability_error"], abs(cp - rp))
+    m["rmse_amplitude"] = (d2 / r.size) ** 0.5
+    m["rel_l2"] = 0.0 if r2 == 0 else (d2 / r2) ** 0.5
+    m["total_probability_diff"] = abs(ct - rt)
+    return m
+
+
+FIELDS = ["bitwise_equal", "max_abs_amplitude_error", "max_abs_component_error", "rmse_amplitude", "rel_l2",
+          "max_abs_probability_error", "total_probability_diff", "worst_state"]
+
+
+def test_compare_states_all_metrics():
+    # every StateComparison field and worstState: the device path (two Hip
+    # simulation results) against a sequential restatement (maxima and the
+    # worst state exactly, sums to rounding); the host path (vectors)
+    # bit for bit
+    n = 12
+    init = O.random_phase(n, 21)
+    ops, cnt = S.qft_ops(n)
+    a, _ = S.simulate(n, ops, cnt, init=init)
+    b, _ = S.simulate(n, ops, cnt, init=init, precision=T.C64)
+    ra, rb = a.export(), b.export()
+    want = sequential_metrics(ra, rb)
+    assert want["bitwise_equal"] == 0 and want["max_abs_amplitude_error"] > 0
+    dev = S.compare_runs(n, ops, cnt, ref={}, cand={"precision": T.C64}, init=init)
+    host = S.compare_amplitudes(n, ra, rb)
+    for f in FIELDS:
+        assert host[f] == want[f], f
+        if f in ("bitwise_equal", "worst_state", "max_abs_amplitude_error", "max_abs_component_error",
+                 "max_abs_probability_error"):
+            assert dev[f] == want[f], f
+        else:
+            assert abs(dev[f] - want[f]) <= 1e-13 * max(1.0, abs(want[f])), (f, dev[f], want[f])
+    same = S.compare_runs(n, ops, cnt, ref={}, cand={}, init=init)
+    assert same["bitwise_equal"] == 1 and all(same[f] == 0 for f in FIELDS[1:])
+    # a single perturbed amplitude: it is the worst state
+    rc = ra.copy()
+    rc[2 * 777] += 3e-9
+    host = S.compare_amplitudes(n, ra, rc)
+    assert host["worst_state"] == 777 and host["bitwise_equal"] == 0
+    assert host == {**host, **sequential_metrics(ra, rc)}
+
+
+def test_traced_run_times_each_operation_on_the_device():
+    # per-operation RunReport seconds from stream events: one entry per
+    # operation, all positive, their sum within the run's execution seconds
+    n = 22
+    ops, cnt = S.qft_ops(n)
+    reg = S.Register.basis(n, 3)
+    per, secs = reg.run_traced(ops, cnt)
+    assert per.shape == (cnt,) and np.all(per > 0)
+    assert per.sum() <= secs
+    want = O.run_circuit(n, O.qft_ops(n), init=_basis(n, 3)).amplitudes()
+    assert np.abs(reg.export() - want).max() <= 1e-12  # traced runs compute the same state
+
+
+def _basis(n, s):
+    v = np.zeros(2 << n)
+    v[2 * s] = 1.0
+    return v

@@ -191,6 +191,7 @@ struct tmfqs_hip_state {
 	std::string trace_path;  // TMFQS_HIP_TRACE: where the handle's kernel report goes when it is destroyed
 	std::vector<ProfRec> recs;
 	std::vector<hipEvent_t> event_pool;
+	std::vector<std::pair<hipEvent_t, hipEvent_t>> timed;  // tmfqs_hip_flush_timed brackets
 	std::map<std::string, ProfAgg> prof;
 	// adjoint: checkpoint/lambda buffers (kept between calls), gradient partials and outputs
 	std::vector<void *> adj_bufs;
@@ -2098,6 +2099,10 @@ void destroy_handle(tmfqs_hip_state *h) {
 		hipEventDestroy(r.start);
 		hipEventDestroy(r.stop);
 	}
+	for(auto &p : h->timed) {
+		hipEventDestroy(p.first);
+		hipEventDestroy(p.second);
+	}
 	for(auto e : h->event_pool) hipEventDestroy(e);
 	hipFree(h->amp);
 	hipFree(h->scratch);
@@ -2839,6 +2844,39 @@ int tmfqs_hip_set_tuning(tmfqs_hip_state *h, int tile_qubits, int fuse_qubits) {
 	flush(h);
 	h->cfg.tile_qubits = tile_qubits;
 	h->cfg.fuse_qubits = fuse_qubits;
+	ABI_END
+}
+
+int tmfqs_hip_flush_timed(tmfqs_hip_state *h) {
+	ABI_BEGIN
+	checked(h);
+	HIPC(hipSetDevice(h->cfg.device));
+	const hipEvent_t a = take_event(h), b = take_event(h);
+	h->timed.emplace_back(a, b);
+	HIPC(hipEventRecord(a, h->stream));
+	flush(h);
+	HIPC(hipEventRecord(b, h->stream));
+	ABI_END
+}
+
+int tmfqs_hip_timed_read(tmfqs_hip_state *h, double *ms, size_t cap, size_t *count) {
+	ABI_BEGIN
+	checked(h);
+	if(!count || (cap && !ms)) throw std::invalid_argument("tmfqs_hip_timed_read: null output");
+	HIPC(hipSetDevice(h->cfg.device));
+	HIPC(hipStreamSynchronize(h->stream));
+	const size_t n = std::min(cap, h->timed.size());
+	for(size_t i = 0; i < h->timed.size(); ++i) {
+		if(i < n) {
+			float t = 0;
+			HIPC(hipEventElapsedTime(&t, h->timed[i].first, h->timed[i].second));
+			ms[i] = t;
+		}
+		h->event_pool.push_back(h->timed[i].first);
+		h->event_pool.push_back(h->timed[i].second);
+	}
+	*count = h->timed.size();
+	h->timed.clear();
 	ABI_END
 }
 

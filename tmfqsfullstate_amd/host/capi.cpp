@@ -222,6 +222,19 @@ int tmfqs_reg_run(tmfqs_reg *r, const tmfqs_hip_op *ops, size_t count, const dou
 	CAPI_END
 }
 
+int tmfqs_reg_run_traced(tmfqs_reg *r, const tmfqs_hip_op *ops, size_t count, const double *pool, const unsigned *mq,
+                         double *op_seconds, double *seconds) {
+	CAPI_BEGIN
+	if(!r || (count && !op_seconds)) throw std::invalid_argument("null register or output");
+	const tmfqs::circuit::Circuit c = toCircuit(r->reg.qubitCount(), ops, count, pool, mq);
+	tmfqs::sim::ExecutionConfig ec;
+	ec.observability.traceOperations = true;
+	const tmfqs::experiment::RunReport rep = tmfqs::sim::Simulator(ec).runOn(c, r->reg);
+	for(size_t i = 0; i < count; ++i) op_seconds[i] = i < rep.operations.size() ? rep.operations[i].seconds : 0.0;
+	if(seconds) *seconds = rep.executionSeconds;
+	CAPI_END
+}
+
 int tmfqs_sim_run(unsigned n, uint64_t basis, const double *init, const tmfqs_hip_config *cfg, const tmfqs_hip_op *ops, size_t count,
                   const double *pool, const unsigned *mq, double *seconds, tmfqs_reg **result) {
 	CAPI_BEGIN
@@ -239,6 +252,105 @@ int tmfqs_sim_run(unsigned n, uint64_t basis, const double *init, const tmfqs_hi
 	sim::RunResult rr = sim::Simulator(ec).run(toCircuit(n, ops, count, pool, mq), input);
 	if(seconds) *seconds = rr.report.executionSeconds;
 	*result = new tmfqs_reg{*rr.state.registerPtr()};
+	CAPI_END
+}
+
+namespace {
+tmfqs::state::QuantumState hostState(unsigned n, const double *amps) {
+	tmfqs::state::StateValidation v;
+	v.normalization = tmfqs::state::NormalizationPolicy::AllowUnnormalized;
+	return tmfqs::state::QuantumState::fromAmplitudes(n, tmfqs::AmplitudesVector(amps, amps + tmfqs::checkedAmplitudeElementCount(n)), v);
+}
+
+void fillComparison(const tmfqs::experiment::StateComparison &m, tmfqs_hip_comparison *out) {
+	*out = tmfqs_hip_comparison{};
+	out->bitwise_equal = m.bitwiseEqual ? 1 : 0;
+	out->max_abs_amplitude_error = m.maxAbsAmplitudeError;
+	out->max_abs_component_error = m.maxAbsComponentError;
+	out->rmse_amplitude = m.rmseAmplitude;
+	out->rel_l2 = m.relL2;
+	out->max_abs_probability_error = m.maxAbsProbabilityError;
+	out->total_probability_diff = m.totalProbabilityDiff;
+	out->worst_state = m.worstState;
+}
+}  // namespace
+
+int tmfqs_sim_compare(unsigned n, uint64_t basis, const double *init, const tmfqs_hip_config *ref_cfg,
+                      const tmfqs_hip_config *cand_cfg, const tmfqs_hip_op *ops, size_t count, const double *pool,
+                      const unsigned *mq, tmfqs_hip_comparison *out) {
+	CAPI_BEGIN
+	using namespace tmfqs;
+	if(!out) throw std::invalid_argument("tmfqs_sim_compare: null output");
+	const state::QuantumState input = init ? hostState(n, init) : state::QuantumState::basis(n, basis);
+	const circuit::Circuit c = toCircuit(n, ops, count, pool, mq);
+	sim::ExecutionConfig er, ec;
+	er.backend = regConfig(ref_cfg);
+	ec.backend = regConfig(cand_cfg);
+	const sim::RunResult a = sim::Simulator(er).run(c, input);
+	const sim::RunResult b = sim::Simulator(ec).run(c, input);
+	fillComparison(experiment::compareStates(a.state, b.state), out);
+	CAPI_END
+}
+
+int tmfqs_compare_amplitudes(unsigned n, const double *ref, const double *cand, tmfqs_hip_comparison *out) {
+	CAPI_BEGIN
+	if(!ref || !cand || !out) throw std::invalid_argument("tmfqs_compare_amplitudes: null pointer");
+	fillComparison(tmfqs::experiment::compareStates(hostState(n, ref), hostState(n, cand)), out);
+	CAPI_END
+}
+
+namespace {
+tmfqs::QuantumGate gateFactory(int which, double theta, unsigned identityDim) {
+	using tmfqs::QuantumGate;
+	switch(which) {
+		case 0: return QuantumGate::Identity(identityDim);
+		case 1: return QuantumGate::Hadamard();
+		case 2: return QuantumGate::PauliX();
+		case 3: return QuantumGate::PauliY();
+		case 4: return QuantumGate::PauliZ();
+		case 5: return QuantumGate::PhaseShift(theta);
+		case 6: return QuantumGate::PiOverEight();
+		case 7: return QuantumGate::ControlledNot();
+		case 8: return QuantumGate::Toffoli();
+		case 9: return QuantumGate::ControlledPhaseShift(theta);
+		case 10: return QuantumGate::Swap();
+		case 11: return QuantumGate::Ising(theta);
+		default: throw std::invalid_argument("unknown gate factory");
+	}
+}
+
+tmfqs::QuantumGate gateFrom(unsigned dim, const double *m) {
+	tmfqs::QuantumGate g(dim);
+	for(unsigned r = 0; r < dim; ++r)
+		for(unsigned c = 0; c < dim; ++c) g[r][c] = tmfqs::Amplitude{m[2 * (r * dim + c)], m[2 * (r * dim + c) + 1]};
+	return g;
+}
+
+void gateTo(const tmfqs::QuantumGate &g, double *out) {
+	const unsigned d = g.dimension();
+	for(unsigned r = 0; r < d; ++r)
+		for(unsigned c = 0; c < d; ++c) {
+			out[2 * (r * d + c)] = g[r][c].real;
+			out[2 * (r * d + c) + 1] = g[r][c].imag;
+		}
+}
+}  // namespace
+
+int tmfqs_gate_matrix(int which, double theta, unsigned identityDim, double *out, size_t cap, unsigned *dim) {
+	CAPI_BEGIN
+	const tmfqs::QuantumGate g = gateFactory(which, theta, identityDim);
+	const size_t need = 2 * static_cast<size_t>(g.dimension()) * g.dimension();
+	if(!out || !dim || cap < need) throw std::invalid_argument("tmfqs_gate_matrix: output too small");
+	gateTo(g, out);
+	*dim = g.dimension();
+	CAPI_END
+}
+
+int tmfqs_gate_product(unsigned dim, const double *a, const double *b, double sre, double sim, double *out) {
+	CAPI_BEGIN
+	if(!a || !out) throw std::invalid_argument("tmfqs_gate_product: null matrix");
+	const tmfqs::QuantumGate A = gateFrom(dim, a);
+	gateTo(b ? A * gateFrom(dim, b) : A * tmfqs::Amplitude{sre, sim}, out);
 	CAPI_END
 }
 

@@ -2,12 +2,15 @@
 #include "tmfqs/sim/simulator.h"
 
 #include <chrono>
+#include <vector>
 #include <stdexcept>
 #include <type_traits>
 #include <utility>
 
 #include "tmfqs/register/quantum_register.h"
 #include "tmfqs/storage/factory/storage_strategy_registry.h"
+#include "tmfqs/storage/hip/hip_state_backend.h"
+#include "tmfqs_hip.h"
 
 namespace tmfqs {
 namespace sim {
@@ -50,9 +53,10 @@ Simulator::Simulator(ExecutionConfig config) : config_(std::move(config)) {}
 
 const ExecutionConfig &Simulator::config() const noexcept { return config_; }
 
-// reference :66-93: one batch around the op loop, closed on exceptions; with
-// tracing on, each operation is waited for so that its seconds cover the
-// device work too.
+// reference :66-93: one batch around the op loop, closed on exceptions.  With
+// tracing on, each operation's queued work is executed right after it,
+// bracketed by stream events on a device backend (its seconds are the
+// device time of its kernels, read once at the end) or waited for on others.
 experiment::RunReport Simulator::runOn(const circuit::Circuit &circuit, QuantumRegister &reg) const {
 	experiment::RunReport report;
 	report.qubits = circuit.qubitCount();
@@ -63,16 +67,27 @@ experiment::RunReport Simulator::runOn(const circuit::Circuit &circuit, QuantumR
 	bool open = true;
 	try {
 		const auto &ops = circuit.operations();
+		tmfqs_hip_state *device = config_.observability.traceOperations ? hipBackendHandle(reg.backend()) : nullptr;
 		for(size_t i = 0; i < ops.size(); ++i) {
 			const auto t0 = Clock::now();
 			executeOperation(reg, ops[i]);
-			if(config_.observability.traceOperations) {
+			if(!config_.observability.traceOperations) continue;
+			if(device) {
+				raiseHipStatus(tmfqs_hip_flush_timed(device));
+				report.operations.push_back({i, circuit::operationName(ops[i]), 0.0});
+			} else {
 				reg.backend()->synchronize();
 				report.operations.push_back({i, circuit::operationName(ops[i]), std::chrono::duration<double>(Clock::now() - t0).count()});
 			}
 		}
 		reg.endOperationBatch();
 		open = false;
+		if(device) {
+			std::vector<double> ms(report.operations.size());
+			size_t got = 0;
+			raiseHipStatus(tmfqs_hip_timed_read(device, ms.data(), ms.size(), &got));
+			for(size_t i = 0; i < report.operations.size() && i < got; ++i) report.operations[i].seconds = ms[i] * 1e-3;
+		}
 	} catch(...) {
 		if(open) {
 			try {

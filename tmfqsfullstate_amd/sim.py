@@ -161,6 +161,23 @@ class Register:
         check(lib().tmfqs_reg_run(self._h, ops, count, mp, mq, int(trace), ctypes.byref(secs)))
         return secs.value
 
+    def run_traced(self, ops, count=None, mat_pool=None, mat_qubits=None):
+        """Simulator::run with the per-operation trace; returns (per-op seconds, executionSeconds)."""
+        if count is None:
+            ops, count = make_ops(ops)
+        per = np.zeros(max(1, count))
+        secs = ctypes.c_double()
+        mp = None if mat_pool is None else np.ascontiguousarray(mat_pool, dtype=np.float64).ctypes.data_as(_dp)
+        mq = None
+        if mat_qubits is not None:
+            mqa = np.ascontiguousarray(mat_qubits, dtype=np.uint32)
+            mq = mqa.ctypes.data_as(ctypes.POINTER(ctypes.c_uint))
+        L = lib()
+        L.tmfqs_reg_run_traced.argtypes = [_vp, ctypes.POINTER(Op), ctypes.c_size_t, _dp, ctypes.POINTER(ctypes.c_uint), _dp,
+                                           _dp]
+        check(L.tmfqs_reg_run_traced(self._h, ops, count, mp, mq, per.ctypes.data_as(_dp), ctypes.byref(secs)))
+        return per[:count], secs.value
+
     def export(self):
         out = np.empty(2 << self.n, dtype=np.float64)
         check(lib().tmfqs_reg_export(self._h, out.ctypes.data_as(_dp)))
@@ -241,6 +258,43 @@ def simulate(n, ops, count=None, init=None, basis=0, precision=0, mat_pool=None,
         mq = mqa.ctypes.data_as(ctypes.POINTER(ctypes.c_uint))
     check(lib().tmfqs_sim_run(n, basis, ip, ctypes.byref(c), ops, count, mp, mq, ctypes.byref(secs), ctypes.byref(h)))
     return Register(h, n), secs.value
+
+
+def compare_runs(n, ops, count=None, ref=None, cand=None, init=None, basis=0, mat_pool=None, mat_qubits=None):
+    """experiment::compareStates of the circuit simulated with config `ref` and
+    config `cand` (dicts of tmfqs_hip_config fields); returns the metrics."""
+    from . import Comparison
+    if count is None:
+        ops, count = make_ops(ops)
+    cr, cc = _cfg(**(ref or {})), _cfg(**(cand or {}))
+    ip = None
+    if init is not None:
+        init = np.ascontiguousarray(init, dtype=np.float64)
+        ip = init.ctypes.data_as(_dp)
+    mp = None if mat_pool is None else np.ascontiguousarray(mat_pool, dtype=np.float64).ctypes.data_as(_dp)
+    mq = None
+    if mat_qubits is not None:
+        mqa = np.ascontiguousarray(mat_qubits, dtype=np.uint32)
+        mq = mqa.ctypes.data_as(ctypes.POINTER(ctypes.c_uint))
+    out = Comparison()
+    L = lib()
+    L.tmfqs_sim_compare.argtypes = [ctypes.c_uint, ctypes.c_uint64, _dp, ctypes.POINTER(Config), ctypes.POINTER(Config),
+                                    ctypes.POINTER(Op), ctypes.c_size_t, _dp, ctypes.POINTER(ctypes.c_uint),
+                                    ctypes.POINTER(Comparison)]
+    check(L.tmfqs_sim_compare(n, basis, ip, ctypes.byref(cr), ctypes.byref(cc), ops, count, mp, mq, ctypes.byref(out)))
+    return out.as_dict()
+
+
+def compare_amplitudes(n, ref, cand):
+    """experiment::compareStates of two host amplitude vectors."""
+    from . import Comparison
+    a = np.ascontiguousarray(ref, dtype=np.float64)
+    b = np.ascontiguousarray(cand, dtype=np.float64)
+    out = Comparison()
+    L = lib()
+    L.tmfqs_compare_amplitudes.argtypes = [ctypes.c_uint, _dp, _dp, ctypes.POINTER(Comparison)]
+    check(L.tmfqs_compare_amplitudes(n, a.ctypes.data_as(_dp), b.ctypes.data_as(_dp), ctypes.byref(out)))
+    return out.as_dict()
 
 
 def random_phase(n, seed):
