@@ -22,6 +22,7 @@ ROOT = os.path.dirname(PKG)
 INCLUDE = os.path.join(ROOT, "include")
 CSRC = os.path.join(PKG, "csrc")
 HOST = os.path.join(PKG, "host")
+CLI = os.path.join(PKG, "cli")
 LIB = os.path.join(PKG, "lib")
 BIN = os.path.join(PKG, "bin")
 
@@ -82,12 +83,14 @@ def build_host():
            f"-I{INCLUDE}", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
     _run([*cxx, "-shared", *sorted(srcs), "-o", tmp, f"-L{LIB}", "-ltmfqs_hip", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, out)
-    tdir = os.path.join(HOST, "tests")
-    if os.path.isdir(tdir):
-        for f in sorted(os.listdir(tdir)):
+    # The C++ test drivers and the command-line programs, one executable each.
+    for d in (os.path.join(HOST, "tests"), CLI):
+        if not os.path.isdir(d):
+            continue
+        for f in sorted(os.listdir(d)):
             if f.endswith(".cpp"):
                 exe = os.path.join(BIN, f[:-4])
-                _run([*cxx, os.path.join(tdir, f), "-o", exe + ".tmp", f"-L{LIB}", "-ltmfqs", "-ltmfqs_hip",
+                _run([*cxx, os.path.join(d, f), "-o", exe + ".tmp", f"-L{LIB}", "-ltmfqs", "-ltmfqs_hip",
                       "-Wl,-rpath,$ORIGIN/../lib"])
                 os.replace(exe + ".tmp", exe)
     return out
