@@ -125,6 +125,15 @@ int tmfqs_hip_init_basis(tmfqs_hip_state *h, unsigned num_qubits, uint64_t state
 int tmfqs_hip_init_uniform(tmfqs_hip_state *h, unsigned num_qubits, const uint64_t *states, size_t count);
 /* IStateBackend::loadAmplitudes (i_state_backend.h:56): elem_count = 2 * 2^n doubles. */
 int tmfqs_hip_load(tmfqs_hip_state *h, unsigned num_qubits, const double *interleaved, uint64_t elem_count);
+/* QuantumState::randomPhase(n, seed) materialized on this backend
+ * (src/tmfqs/state/quantum_state.cpp:264-277 and materialize, :290-307),
+ * generated in device memory: std::mt19937(seed) split into jump-ahead
+ * streams, uniform_real_distribution(0, 2 acos(-1)) per state, and
+ * norm * (cos, sin) of the phase, bit for bit the values glibc's sincos gives
+ * (near-midpoint values are recomputed on the host with glibc).  Validates
+ * the probability mass as fromAmplitudes does (TMFQS_HIP_ERR_INVALID beyond
+ * 1e-9).  Replaces the host construction plus loadAmplitudes. */
+int tmfqs_hip_init_random_phase(tmfqs_hip_state *h, unsigned num_qubits, unsigned seed);
 /* IStateBackend::exportAmplitudes / forEachAmplitudeChunk (i_state_backend.h:61,66):
  * copies states [first, first + count) as interleaved FP64 into out. */
 int tmfqs_hip_export(tmfqs_hip_state *h, uint64_t first, uint64_t count, double *out);
@@ -352,6 +361,16 @@ int tmfqs_hip_batch_info(tmfqs_hip_batch *b, tmfqs_hip_batch_stats *out);
 int tmfqs_hip_batch_adjoint(tmfqs_hip_batch *b, const tmfqs_hip_diag_term *terms, size_t nterms, double *expvals, double *grads,
                             size_t grad_count);
 int tmfqs_hip_batch_destroy(tmfqs_hip_batch *b);
+
+/* ---- host-side checks of the randomPhase arithmetic -------------------
+ * No counterpart in the reference; for tests of the device path, no device
+ * needed.  tmfqs_hip_mt_jump_outputs: the count outputs of std::mt19937(seed)
+ * that follow its first jump_words outputs, reached with the jump-ahead
+ * polynomials the device streams use.  tmfqs_hip_phase_eval: the device's
+ * double-double cos/sin of x (out = cos hi, cos lo, sin hi, sin lo) and its
+ * near-midpoint flags (flags[0] cos, flags[1] sin). */
+int tmfqs_hip_mt_jump_outputs(uint32_t seed, uint64_t jump_words, size_t count, uint32_t *out);
+int tmfqs_hip_phase_eval(double x, double out[4], int flags[2]);
 
 #ifdef __cplusplus
 }

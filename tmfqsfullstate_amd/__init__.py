@@ -154,6 +154,10 @@ def lib():
         L.tmfqs_hip_init_basis.argtypes = [vp, ctypes.c_uint, ctypes.c_uint64, ctypes.c_double, ctypes.c_double]
         L.tmfqs_hip_init_uniform.argtypes = [vp, ctypes.c_uint, _u64p, ctypes.c_size_t]
         L.tmfqs_hip_load.argtypes = [vp, ctypes.c_uint, _dp, ctypes.c_uint64]
+        L.tmfqs_hip_init_random_phase.argtypes = [vp, ctypes.c_uint, ctypes.c_uint]
+        L.tmfqs_hip_mt_jump_outputs.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_size_t,
+                                                ctypes.POINTER(ctypes.c_uint32)]
+        L.tmfqs_hip_phase_eval.argtypes = [ctypes.c_double, _dp, ctypes.POINTER(ctypes.c_int)]
         L.tmfqs_hip_export.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, _dp]
         L.tmfqs_hip_amplitude.argtypes = [vp, ctypes.c_uint64, _dp, _dp]
         L.tmfqs_hip_set_amplitude.argtypes = [vp, ctypes.c_uint64, ctypes.c_double, ctypes.c_double]
@@ -216,6 +220,24 @@ def default_config(**kw):
     for k, v in kw.items():
         setattr(cfg, k, int(v))
     return cfg
+
+
+def mt_jump_outputs(seed, jump_words, count):
+    """The count std::mt19937(seed) outputs after its first jump_words, through
+    the jump-ahead polynomials (host computation)."""
+    out = np.empty(count, dtype=np.uint32)
+    check(lib().tmfqs_hip_mt_jump_outputs(seed, jump_words, count,
+                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+    return out
+
+
+def phase_eval(x):
+    """The device's double-double (cos hi, cos lo, sin hi, sin lo) of x and its
+    near-midpoint flags (cos, sin), evaluated on the host."""
+    out = np.empty(4, dtype=np.float64)
+    fl = (ctypes.c_int * 2)()
+    check(lib().tmfqs_hip_phase_eval(x, out.ctypes.data_as(_dp), fl))
+    return out, (bool(fl[0]), bool(fl[1]))
 
 
 def expectation_device(amps_ptr, n, terms, out_ptr, precision=C128, device=0, stream=None):
@@ -281,6 +303,12 @@ class HipState:
     def init_uniform(self, n, states):
         a = np.ascontiguousarray(states, dtype=np.uint64)
         check(lib().tmfqs_hip_init_uniform(self._h, n, a.ctypes.data_as(_u64p), len(a)))
+        self.n = n
+        return self
+
+    def init_random_phase(self, n, seed):
+        """QuantumState::randomPhase(n, seed) generated in device memory."""
+        check(lib().tmfqs_hip_init_random_phase(self._h, n, seed))
         self.n = n
         return self
 

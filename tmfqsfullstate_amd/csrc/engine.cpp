@@ -20,11 +20,14 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <unordered_set>
 #include <vector>
 
 #include "engine.h"
+#include "mt_jump.h"
+#include "phase_math.h"
 
 using namespace tmfqs_hip;
 
@@ -2318,6 +2321,184 @@ int tmfqs_hip_load(tmfqs_hip_state *h, unsigned n, const double *amps, uint64_t 
 	if(elem_count != 2ull * (1ull << n)) throw std::invalid_argument("HipStateBackend: amplitudes size mismatch");
 	allocate_state(h, n);
 	load_host(h, amps);
+	ABI_END
+}
+
+// ------------------------------------------------------------ randomPhase
+// QuantumState::randomPhase(n, seed) (quantum_state.cpp:264-277) generated
+// in device memory.  The seed's 2^(n+1) words are split into P = 2^levels
+// streams of L words; stream 0 starts at the seeded state and level i's
+// kernel jumps streams [0, 2^i) by L 2^i words into [2^i, 2^(i+1)).  Each
+// workgroup then generates its stream's states; values whose double-double
+// cos/sin lie near a rounding midpoint are listed, recomputed on the host
+// with glibc (the call the reference's compiler emits for its cos/sin pair)
+// and patched back.
+namespace {
+
+struct DeviceBlock {
+	void *p = nullptr;
+	explicit DeviceBlock(size_t bytes) { HIPC(hipMalloc(&p, std::max<size_t>(bytes, 256))); }
+	~DeviceBlock() {
+		if(p) hipFree(p);
+	}
+	DeviceBlock(const DeviceBlock &) = delete;
+	DeviceBlock &operator=(const DeviceBlock &) = delete;
+};
+
+// glibc's value of each flagged entry, in place: x -> sin x or cos x.
+void glibc_phase_values(PhaseFlag *f, u64 count) {
+	auto work = [f](u64 lo, u64 hi) {
+		for(u64 i = lo; i < hi; ++i) {
+			double sn, cs;
+			__builtin_sincos(f[i].x, &sn, &cs);
+			f[i].x = (f[i].key & 1u) ? sn : cs;
+		}
+	};
+	const unsigned hw = std::thread::hardware_concurrency();
+	const unsigned workers = std::max(1u, std::min(hw ? hw : 1u, 16u));
+	if(workers == 1 || count < (1u << 16)) {
+		work(0, count);
+		return;
+	}
+	std::vector<std::thread> pool;
+	const u64 per = (count + workers - 1) / workers;
+	for(u64 lo = 0; lo < count; lo += per) pool.emplace_back(work, lo, std::min(count, lo + per));
+	for(auto &t : pool) t.join();
+}
+
+// Round trip of `count` flag entries through the two pinned staging slots:
+// the copy down of chunk c + 1 runs while the host evaluates chunk c.
+void patch_flags_via_host(tmfqs_hip_state *h, PhaseFlag *dflags, u64 count) {
+	if(count == 0) return;
+	const size_t stage = 64u << 20;
+	ensure_stage(h, 2 * stage);
+	const u64 per = stage / sizeof(PhaseFlag);
+	const u64 chunks = (count + per - 1) / per;
+	SlotEvents slots;
+	slots.create();
+	auto down = [&](u64 c) {
+		const u64 k = std::min(per, count - c * per);
+		char *slot = h->h_stage + (c & 1) * stage;
+		HIPC(hipMemcpyAsync(slot, dflags + c * per, k * sizeof(PhaseFlag), hipMemcpyDeviceToHost, h->stream));
+		HIPC(hipEventRecord(slots.ev[c & 1], h->stream));
+	};
+	down(0);
+	for(u64 c = 0; c < chunks; ++c) {
+		if(c + 1 < chunks) down(c + 1);  // after the copy up of chunk c - 1 from that slot, in stream order
+		HIPC(hipEventSynchronize(slots.ev[c & 1]));
+		const u64 k = std::min(per, count - c * per);
+		PhaseFlag *slot = (PhaseFlag *)(h->h_stage + (c & 1) * stage);
+		glibc_phase_values(slot, k);
+		HIPC(hipMemcpyAsync(dflags + c * per, slot, k * sizeof(PhaseFlag), hipMemcpyHostToDevice, h->stream));
+	}
+	HIPC(hipStreamSynchronize(h->stream));
+}
+
+void random_phase_device(tmfqs_hip_state *h, unsigned seed) {
+	using tmfqs_phase::kMtN;
+	using tmfqs_phase::kPolyWords;
+	const u64 words = 2 * h->N;
+	const u64 L = words <= (1u << 12) ? words : std::max<u64>(1u << 12, words >> 11);
+	const unsigned P = (unsigned)(words / L);
+	unsigned logL = 0, levels = 0;
+	while((u64{1} << logL) < L) ++logL;
+	while((1u << levels) < P) ++levels;
+	// Stream states and the jump polynomials of the levels.
+	std::vector<uint32_t> st0(kMtN);
+	tmfqs_mt::seed_state(seed, st0.data());
+	std::vector<uint64_t> polys((size_t)std::max(levels, 1u) * kPolyWords, 0);
+	for(unsigned i = 0; i < levels; ++i) {
+		const tmfqs_mt::Poly &g = tmfqs_mt::pow2_mod(logL + i);
+		// The jump kernel reads x_{j+m} for set bits j < 19937 only.
+		if(g.size() != (size_t)kPolyWords || (g.back() >> (tmfqs_phase::kMtDegree - 64 * (kPolyWords - 1))) != 0)
+			throw std::logic_error("randomPhase: jump polynomial of unexpected degree");
+		std::copy(g.begin(), g.end(), polys.begin() + (size_t)i * kPolyWords);
+	}
+	DeviceBlock dstates((size_t)P * kMtN * sizeof(uint32_t)), dpoly(polys.size() * sizeof(uint64_t));
+	HIPC(hipMemcpy(dstates.p, st0.data(), kMtN * sizeof(uint32_t), hipMemcpyHostToDevice));
+	HIPC(hipMemcpy(dpoly.p, polys.data(), polys.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+	for(unsigned i = 0; i < levels; ++i)
+		HIPC(launch_mt_jump(h->stream, (uint32_t *)dstates.p, 0, 1u << i, 1u << i, (const uint64_t *)dpoly.p + (size_t)i * kPolyWords));
+	// About 9% of values fall in the flag bands: a round of R streams lists at
+	// most a quarter of its values (plus slack for tiny states), 8 GiB at most.
+	const u64 per_stream = L / 2;
+	const u64 slack = 1u << 16, max_entries = u64{1} << 29;
+	unsigned R = P;
+	while(R > 1 && (u64)R * per_stream / 2 > max_entries) R /= 2;
+	const u64 cap = (u64)R * per_stream / 2 + slack;
+	DeviceBlock dflags(cap * sizeof(PhaseFlag)), dcount(sizeof(unsigned long long));
+	const double norm = 1.0 / std::sqrt((double)h->N);
+	const double range = 2.0 * std::acos(-1.0);
+	for(unsigned s0 = 0; s0 < P; s0 += R) {
+		const unsigned cnt = std::min(R, P - s0);
+		HIPC(hipMemsetAsync(dcount.p, 0, sizeof(unsigned long long), h->stream));
+		HIPC(launch_random_phase(h->L(), (const uint32_t *)dstates.p + (size_t)s0 * kMtN, s0, cnt, L, norm, range, h->amp,
+		                         (PhaseFlag *)dflags.p, (unsigned long long *)dcount.p, cap));
+		unsigned long long flagged = 0;
+		HIPC(hipMemcpyAsync(&flagged, dcount.p, sizeof(flagged), hipMemcpyDeviceToHost, h->stream));
+		HIPC(hipStreamSynchronize(h->stream));
+		if(flagged > cap) throw std::runtime_error("randomPhase: more near-midpoint values than the flag list holds");
+		patch_flags_via_host(h, (PhaseFlag *)dflags.p, flagged);
+		HIPC(launch_phase_patch(h->L(), (const PhaseFlag *)dflags.p, flagged, norm, h->amp));
+	}
+	HIPC(hipStreamSynchronize(h->stream));
+}
+
+}  // namespace
+
+int tmfqs_hip_init_random_phase(tmfqs_hip_state *h, unsigned n, unsigned seed) {
+	ABI_BEGIN
+	checked(h);
+	HIPC(hipSetDevice(h->cfg.device));
+	if(n > 63u) throw std::invalid_argument("Number of qubits exceeds supported range for state indexing");
+	allocate_state(h, n);
+	try {
+		random_phase_device(h, seed);
+		// fromAmplitudes' default validation: the probability mass, summed in
+		// state order, within 1e-9 of one.  The reference checks the FP64
+		// vector before storage rounds it; complex64 holds only the rounded
+		// values, whose mass lies within a factor (1 +- 2^-24)^2 of it, so
+		// the bound widens by that much (it never rejects an accepted vector).
+		const double tol = h->prec == 0 ? 1e-9 : 1e-9 + 0x1p-22;
+		if(std::abs(chain_walk(h, 0, false, 0.0, nullptr).total - 1.0) > tol)
+			throw std::invalid_argument("Amplitude vector is not normalized");
+	} catch(...) {
+		h->init = false;
+		throw;
+	}
+	ABI_END
+}
+
+int tmfqs_hip_mt_jump_outputs(uint32_t seed, uint64_t jump_words, size_t count, uint32_t *out) {
+	ABI_BEGIN
+	if(count && !out) throw std::invalid_argument("tmfqs_hip_mt_jump_outputs: null output");
+	uint32_t st[tmfqs_phase::kMtN];
+	tmfqs_mt::seed_state(seed, st);
+	for(unsigned m = 0; m < 64; ++m)
+		if((jump_words >> m) & 1u) tmfqs_mt::jump_host(st, tmfqs_mt::pow2_mod(m));
+	std::vector<uint32_t> x(st, st + tmfqs_phase::kMtN);
+	for(size_t k = 0; k < count; ++k) {
+		x.push_back(tmfqs_phase::mt_next(x[k], x[k + 1], x[k + tmfqs_phase::kMtM]));
+		out[k] = tmfqs_phase::mt_temper(x.back());
+	}
+	ABI_END
+}
+
+int tmfqs_hip_phase_eval(double x, double out[4], int flags[2]) {
+	ABI_BEGIN
+	if(!out || !flags) throw std::invalid_argument("tmfqs_hip_phase_eval: null output");
+	if(!(x >= 0.0 && x < 7.0)) throw std::invalid_argument("tmfqs_hip_phase_eval: x outside [0, 2 pi)");
+	tmfqs_phase::DD s, c;
+	tmfqs_phase::dd_sincos(x, s, c);
+	bool fc, fs;
+	tmfqs_phase::round_flagged(c, &fc);
+	tmfqs_phase::round_flagged(s, &fs);
+	out[0] = c.hi;
+	out[1] = c.lo;
+	out[2] = s.hi;
+	out[3] = s.lo;
+	flags[0] = fc;
+	flags[1] = fs;
 	ABI_END
 }
 

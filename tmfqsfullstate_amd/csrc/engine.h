@@ -212,6 +212,26 @@ struct ChunkTicks {
 hipError_t launch_chunk_ticks(const Launch &L, const void *a, unsigned n, const double *affine4, int sel,
                               const int32_t *dulog, ChunkTicks *dout);
 
+// ------------------------------------------------------------ randomPhase
+// A value whose double-double cos/sin lay too close to a rounding midpoint:
+// key = state << 1 | (0 cos, 1 sin); x is the phase, replaced on the host by
+// glibc's value before the patch kernel stores norm * x.
+struct PhaseFlag {
+	u64 key;
+	double x;
+};
+// Jump states[src0 + b] -> states[dst0 + b], b < count, by the polynomial
+// dpoly (tmfqs_phase::kPolyWords words); states are 624-word buffers.
+hipError_t launch_mt_jump(hipStream_t stream, uint32_t *dstates, unsigned src0, unsigned dst0, unsigned count,
+                          const uint64_t *dpoly);
+// Streams [stream0, stream0 + streams) of `words` words each (dstates points
+// at stream0's state) write their words / 2 amplitudes norm * (cos, sin) of
+// canonical * range and append near-midpoint values to dflags (*dcount
+// counts every one; entries at or past cap are dropped).
+hipError_t launch_random_phase(const Launch &L, const uint32_t *dstates, unsigned stream0, unsigned streams, u64 words,
+                               double norm, double range, void *amp, PhaseFlag *dflags, unsigned long long *dcount, u64 cap);
+hipError_t launch_phase_patch(const Launch &L, const PhaseFlag *dflags, u64 count, double norm, void *amp);
+
 // ------------------------------------------------------ batched circuits
 // Batched tile passes hold at most 2^batch_max_tile_bits(prec) amplitudes per
 // tile (64 KiB of amplitudes: two workgroups per CU).

@@ -5,6 +5,7 @@
 #include <string>
 #include <variant>
 
+#include "tmfqs/storage/hip/hip_state_backend.h"
 #include "tmfqs_sim.h"
 #include "tmfqsfs.h"
 
@@ -193,8 +194,13 @@ int tmfqs_reg_create_amplitudes(unsigned n, const double *amps, const tmfqs_hip_
 
 int tmfqs_reg_create_random_phase(unsigned n, unsigned seed, const tmfqs_hip_config *cfg, tmfqs_reg **out) {
 	CAPI_BEGIN
-	tmfqs::AmplitudesVector a = tmfqs::state::QuantumState::randomPhase(n, seed).amplitudes();
-	*out = new tmfqs_reg{tmfqs::QuantumRegister(n, std::move(a), regConfig(cfg))};
+	// What QuantumState::randomPhase(n, seed) materializes to on a Hip
+	// register, generated in device memory.
+	tmfqs::QuantumRegister reg(n, regConfig(cfg));
+	tmfqs_hip_state *h = tmfqs::hipBackendHandle(reg.backend());
+	if(!h) throw std::logic_error("tmfqs_reg_create_random_phase: the register has no device backend");
+	tmfqs::raiseHipStatus(tmfqs_hip_init_random_phase(h, n, seed));
+	*out = new tmfqs_reg{std::move(reg)};
 	CAPI_END
 }
 

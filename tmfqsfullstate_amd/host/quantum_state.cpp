@@ -4,7 +4,8 @@
 // are needed.  Queries on host-described states are answered on the host;
 // materialize() builds the register on the requested backend, writing a
 // vector straight from the recipe into device memory (no intermediate copy)
-// when the backend is the Hip one.
+// when the backend is the Hip one; a random-phase state is generated there
+// (tmfqs_hip_init_random_phase) without a host vector at all.
 //
 // Semantics follow src/tmfqs/state/quantum_state.cpp:44-315: normalization
 // policies and their messages, measure() of a basis state draws nothing,
@@ -342,13 +343,15 @@ QuantumRegister QuantumState::materialize(const RegisterConfig &config) const {
 		        return reg;
 	        },
 	        [&](const PhaseRecipe &p) {
+		        QuantumRegister reg(q, config);
+		        if(tmfqs_hip_state *h = deviceHandle(reg)) {
+			        // drawn, validated and stored in device memory; no host vector
+			        raiseHipStatus(tmfqs_hip_init_random_phase(h, q, p.seed));
+			        return reg;
+		        }
 		        AmplitudesVector a = phaseVector(q, p.seed);
 		        applyValidation(a, StateValidation{});
-		        QuantumRegister reg(q, config);
-		        if(tmfqs_hip_state *h = deviceHandle(reg))
-			        raiseHipStatus(tmfqs_hip_load(h, q, a.data(), a.size()));
-		        else
-			        reg.loadAmplitudes(std::move(a));
+		        reg.loadAmplitudes(std::move(a));
 		        return reg;
 	        },
 	        [&](const ResultRecipe &g) { return QuantumRegister(*g.reg); },
