@@ -65,19 +65,6 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def kernel_sources_sha():
-    """SHA-256 (first 16 hex digits) of the HIP kernel sources."""
-    import hashlib
-
-    csrc = os.path.join(ROOT, "tmfqsfullstate_amd", "csrc")
-    h = hashlib.sha256()
-    for f in sorted(os.listdir(csrc)):
-        if f.endswith((".hip", ".h")):
-            h.update(f.encode())
-            h.update(open(os.path.join(csrc, f), "rb").read())
-    return h.hexdigest()[:16]
-
-
 def read_traffic(kernel, workload):
     """HBM bytes per launch from the committed PMC summary (two rocprofv3
     --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel's
@@ -86,6 +73,8 @@ def read_traffic(kernel, workload):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         d = json.load(open(path))
+        from tmfqsfullstate_amd.pmc_traffic import kernel_sources_sha
+
         if d.get("kernel_sources_sha") != kernel_sources_sha():
             log("profiles/traffic.json was measured on other kernel sources: traffic reported as null")
             return None
