@@ -87,6 +87,11 @@ struct HipConfig {
 	unsigned int fuseQubits = 0;
 	/** Fewest gates a run needs to be composed (0: 24). */
 	unsigned int fuseMinGates = 0;
+	/** Complete QFTs: 0 runs ten-target passes and an out-of-place tail fused
+	 *  with the bit reversal (three passes at 30 qubits) when device memory
+	 *  has room for a second state buffer; 1 keeps the in-place eight-target
+	 *  plan (four passes at 30 qubits, no second buffer). */
+	unsigned int qftPlan = 0;
 };
 
 struct HipTuningOverrides {

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define TMFQS_HIP_ABI_VERSION 2
+#define TMFQS_HIP_ABI_VERSION 3
 
 typedef struct tmfqs_hip_state tmfqs_hip_state;
 
@@ -72,6 +72,12 @@ typedef struct tmfqs_hip_config {
 	                        applied as one k-qubit block (five-qubit blocks on MFMA); 0: off.  Not in
 	                        the exact mode.  tmfqs_hip_tune chooses it per circuit. */
 	int fuse_min_gates;  /* fewest gates a run needs to be composed (0: 24) */
+	int qft_plan;        /* complete QFTs (makeQft's op order, fused ladders): 0 automatic: passes of
+	                        ten targets and the last ten targets fused with the bit reversal out of
+	                        place (three full-state passes at 30 qubits; the handle keeps a second
+	                        state buffer, used when device memory has room for it, else 1);
+	                        1: in place, passes of eight targets and the last six targets exchanged
+	                        with the reversal in place (four passes at 30 qubits, no second buffer) */
 } tmfqs_hip_config;
 
 /* One circuit operation (the reference's circuit::Operation variant,

@@ -42,12 +42,13 @@ def test_record_layouts():
     assert ctypes.sizeof(T.Op) == ctypes.sizeof(O.Op)
     for (a, *_), (b, *_) in zip(T.Op._fields_, O.Op._fields_):
         assert getattr(T.Op, a).offset == getattr(O.Op, b).offset
-    assert ctypes.sizeof(T.Config) == 9 * 4
+    assert ctypes.sizeof(T.Config) == 10 * 4
+    assert [f for f, _ in T.Config._fields_][-1] == "qft_plan"
 
 
 def test_abi_version_and_errors_without_device():
     lib = T.lib()
-    assert lib.tmfqs_hip_abi_version() == 2
+    assert lib.tmfqs_hip_abi_version() == 3
     # Creating a handle on a machine without a GPU fails cleanly (device error),
     # never by crashing; on a GPU machine it succeeds.
     h = ctypes.c_void_p()

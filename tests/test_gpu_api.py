@@ -187,12 +187,15 @@ def test_trace_report(tmp_path, monkeypatch):
     rec = json.loads(lines[0])
     assert rec["qubits"] == n and rec["precision"] == "complex128"
     names = {k["name"]: k for k in rec["kernels"]}
-    assert "k_qft_tail_reverse" in names
+    # the default QFT plan at n = 16: one six-target block pass and the
+    # ten-target tail fused with the bit reversal
+    assert set(names) == {"k_qft_wide_block", "k_qft_wide_tail"}
+    assert names["k_qft_wide_block"]["launches"] == 1
     amp_bytes = 16 << n
     for k in rec["kernels"]:
         assert k["launches"] >= 1 and k["ms"] > 0 and k["bytes"] > 0
         assert k["fraction_of_hbm_peak"] == pytest.approx(k["gb_per_s"] / 8000, rel=1e-3, abs=1e-5)
-    assert names["k_qft_tail_reverse"]["bytes"] == 2 * amp_bytes
+    assert names["k_qft_wide_tail"]["bytes"] == 2 * amp_bytes
     assert rec["total_ms"] == pytest.approx(sum(k["ms"] for k in rec["kernels"]), rel=1e-5)
 
 

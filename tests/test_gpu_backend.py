@@ -86,8 +86,11 @@ def test_random_circuits(mode):
         assert np.array_equal(got, want), (mode, n)
 
 
-@pytest.mark.parametrize("mode", MODES, ids=["unfused", "fused-exact", "fused"])
-@pytest.mark.parametrize("n", [1, 2, 3, 9, 12, 16])
+QFT_MODES = MODES + [dict(fusion=1, qft_plan=1)]
+
+
+@pytest.mark.parametrize("mode", QFT_MODES, ids=["unfused", "fused-exact", "fused", "fused-in-place"])
+@pytest.mark.parametrize("n", [1, 2, 3, 9, 12, 13, 14, 16, 20])
 def test_qft(mode, n):
     init = rand_state(n, 7)
     ops, cnt = O.qft_ops(n)
@@ -114,7 +117,7 @@ def test_qft_tail_pair_shapes(precision):
     # kernels alone.
     for n in (12, 13, 14, 15, 17, 19, 20, 21):
         init = rand_state(n, 3)
-        s = st(precision).load(n, init)
+        s = st(precision, qft_plan=1).load(n, init)
         ops, cnt = O.qft_ops(n)
         s.profile(True)
         Register(s, n).run(ops, cnt)
@@ -130,12 +133,53 @@ def test_qft_tail_pair_shapes(precision):
             assert 1 - abs(np.vdot(want, got)) ** 2 <= 1e-6, n
 
 
+@pytest.mark.parametrize("precision", [T.C128, T.C64], ids=["c128", "c64"])
+def test_qft_ten_target_plan_shapes(precision):
+    # The default plan: a leading pass of (n - 10) mod 10 targets (K = 3..9
+    # here, none at n = 20), ten-target passes down to bit 10, and the last ten
+    # targets fused with the bit reversal out of place.  n = 13 (complex128)
+    # and 14 (complex64) have no bits between the tail's passengers and its
+    # targets; complex64 n = 13 keeps the in-place plan.
+    lo = 13 if precision == T.C128 else 14
+    for n in (13, 14, 15, 16, 19, 20, 21, 23, 25):
+        init = rand_state(n, 5)
+        s = st(precision).load(n, init)
+        ops, cnt = O.qft_ops(n)
+        s.profile(True)
+        Register(s, n).run(ops, cnt)
+        got = s.amplitudes()
+        prof = s.profile_read()
+        if n >= lo:
+            assert set(prof) == {"k_qft_wide_block", "k_qft_wide_tail"}, (n, set(prof))
+            assert prof["k_qft_wide_block"]["launches"] == (n - 10 + 9) // 10, n
+            assert prof["k_qft_wide_tail"]["launches"] == 1
+        want = np.fft.ifft(cplx(init), norm="ortho")
+        if precision == T.C128:
+            assert np.abs(got - want).max() <= TOL, n
+            if n <= 16:
+                ref = cplx(O.run_circuit(n, (ops, cnt), init=init).amplitudes())
+                assert np.abs(got - ref).max() <= TOL, n
+        else:
+            assert np.abs(got - want).max() < 1e-5, n
+            assert 1 - abs(np.vdot(want, got)) ** 2 <= 1e-6, n
+        # the state moved into the handle's second buffer: later operations
+        # and a second QFT (back into the first buffer) see it there
+        s.h(0)
+        Register(s, n).run(ops, cnt)
+        got2 = s.amplitudes()
+        h0 = want.reshape(2, -1)
+        h0 = np.concatenate([(h0[0] + h0[1]), (h0[0] - h0[1])]) / np.sqrt(2)
+        want2 = np.fft.ifft(h0, norm="ortho")
+        assert np.abs(got2 - want2).max() <= (TOL if precision == T.C128 else 1e-5), n
+
+
 def _op_tuples(ops, cnt):
     return [(ops[i].kind, ops[i].q0, ops[i].q1, ops[i].theta) for i in range(cnt)]
 
 
+@pytest.mark.parametrize("plan", [0, 1], ids=["ten-target", "in-place"])
 @pytest.mark.parametrize("precision", [T.C128, T.C64], ids=["c128", "c64"])
-def test_qft_inside_longer_gate_runs(precision):
+def test_qft_inside_longer_gate_runs(precision, plan):
     # Two QFTs with other gates before, between and after them, all in one
     # flush: each QFT is still planned as a unit and runs on the QFT kernels.
     n = 16
@@ -152,13 +196,17 @@ def test_qft_inside_longer_gate_runs(precision):
     qft = _op_tuples(*O.qft_ops(n))
     seq = others(12) + qft + others(7) + qft + others(5)
     init = rand_state(n, 9)
-    s = st(precision).load(n, init)
+    s = st(precision, qft_plan=plan).load(n, init)
     s.profile(True)
     s.run_gates(seq)
     got = s.amplitudes()
     prof = s.profile_read()
-    assert prof["k_qft_tail_reverse"]["launches"] == 2
-    assert prof["k_qft_block"]["launches"] == 4  # a partial (two targets) and a full block per QFT
+    if plan:
+        assert prof["k_qft_tail_reverse"]["launches"] == 2
+        assert prof["k_qft_block"]["launches"] == 4  # a partial (two targets) and a full block per QFT
+    else:
+        assert prof["k_qft_wide_tail"]["launches"] == 2
+        assert prof["k_qft_wide_block"]["launches"] == 2  # one six-target pass per QFT
     want = cplx(O.run_circuit(n, seq, init=init).amplitudes())
     if precision == T.C128:
         assert np.abs(got - want).max() <= TOL

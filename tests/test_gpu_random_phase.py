@@ -130,14 +130,18 @@ def test_qft32_complex64_on_random_phase_against_complex128():
     assert abs(p128 - 1.0) < 1e-12 and abs(p64 - 1.0) < 1e-5
 
 
-@pytest.mark.parametrize("precision,n", [(T.C128, 33), (T.C64, 34)], ids=["c128-n33", "c64-n34"])
-def test_capacity_qft_basis_closed_form(precision, n):
+@pytest.mark.parametrize(
+    "precision,n,plan", [(T.C128, 33, 0), (T.C64, 34, 0), (T.C64, 34, 1)], ids=["c128-n33", "c64-n34", "c64-n34-in-place"]
+)
+def test_capacity_qft_basis_closed_form(precision, n, plan):
     # 128 GiB states: QFT|s> = sum_y e^{2 pi i s y / N} |y> / sqrt(N) on three
-    # slices (the reference's dense backend stops at 31 qubits).
+    # slices (the reference's dense backend stops at 31 qubits).  The default
+    # plan's out-of-place tail takes a second 128 GiB buffer; the in-place
+    # plan needs none.
     N = 1 << n
     src = 0x13579BDF1 % N
-    print(f"QFT-{n} of |{src}> ({'c128' if precision == T.C128 else 'c64'})", flush=True)
-    with T.HipState(precision=precision) as s:
+    print(f"QFT-{n} of |{src}> ({'c128' if precision == T.C128 else 'c64'}, plan {plan})", flush=True)
+    with T.HipState(precision=precision, qft_plan=plan) as s:
         s.init_basis(n, src)
         ops, cnt = S.qft_ops(n)
         s.run(ops, cnt)

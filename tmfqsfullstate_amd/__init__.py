@@ -43,6 +43,7 @@ class Config(ctypes.Structure):
         ("exact_phases", ctypes.c_int),
         ("fuse_qubits", ctypes.c_int),
         ("fuse_min_gates", ctypes.c_int),
+        ("qft_plan", ctypes.c_int),
     ]
 
 

@@ -324,6 +324,33 @@ void ensure_stage(tmfqs_hip_state *h, size_t bytes) {
 	h->stage_bytes = bytes;
 }
 
+// A second state buffer for out-of-place passes, allocated when device
+// memory keeps at least 1 GiB free beside it; false (nothing allocated) when
+// there is no room.
+bool reserve_scratch(tmfqs_hip_state *h) {
+	if(h->scratch && h->scratch_bytes >= h->amp_bytes) return true;
+	size_t freeb = 0, total = 0;
+	if(hipMemGetInfo(&freeb, &total) != hipSuccess) {
+		(void)hipGetLastError();
+		return false;
+	}
+	const size_t reserve = (size_t)1 << 30;
+	if(h->scratch) {
+		freeb += h->scratch_bytes;
+		hipFree(h->scratch);
+		h->scratch = nullptr;
+		h->scratch_bytes = 0;
+	}
+	if(freeb < h->amp_bytes + reserve) return false;
+	if(hipMalloc(&h->scratch, h->amp_bytes) != hipSuccess) {
+		(void)hipGetLastError();
+		h->scratch = nullptr;
+		return false;
+	}
+	h->scratch_bytes = h->amp_bytes;
+	return true;
+}
+
 void ensure_scratch(tmfqs_hip_state *h) {
 	if(h->scratch_bytes >= h->amp_bytes) return;
 	if(h->scratch) hipFree(h->scratch);
@@ -915,8 +942,10 @@ std::vector<TileStage> plan_stages(TileOp *ops, size_t cnt, unsigned T, unsigned
 
 // One launch of a planned gate run.
 struct PassLaunch {
-	enum Kind { TILE, QFT_BLOCK, QFT_TAIL } kind = TILE;
-	unsigned pbot = 0, qc = 0, qk = 0;  // QFT_BLOCK: block bits pbot+7 .. pbot, the top qk of them targets
+	enum Kind { TILE, QFT_BLOCK, QFT_TAIL, QFT_WBLOCK, QFT_WTAIL } kind = TILE;
+	// QFT_BLOCK: block bits pbot+7 .. pbot, the top qk of them targets;
+	// QFT_WBLOCK: block bits pbot+9 .. pbot, the top qk of them targets
+	unsigned pbot = 0, qc = 0, qk = 0;
 	TilePass tp{};              // TILE
 	unsigned W = 0;
 	std::vector<TileOp> ops;
@@ -940,6 +969,7 @@ struct PlanCfg {
 	unsigned T = 12, C = 4, qft_c = 5;
 	unsigned W = 0;  // stage width of tile passes; 0 picks tile_stage_width(prec, T, dual)
 	bool mfma_blocks = true;  // complex128 five-qubit blocks on FP64 MFMA (not the reference's accumulation order)
+	unsigned wide_qft_min = 0;  // complete QFTs of at least this many qubits take the ten-target plan (0: never)
 };
 
 GatePlan plan_gates(const PlanCfg &cfg, unsigned n, const std::vector<QOp> &in, size_t lo, size_t hi,
@@ -982,6 +1012,31 @@ GatePlan plan_gates(const PlanCfg &cfg, unsigned n, const std::vector<QOp> &in, 
 	if(!cur.ops.empty()) passes.push_back(cur);
 	// a whole QFT: eight-target block passes and the tail fused with the reversal
 	const bool qft_fused = ladders && plan_qft(ops, map, n, passes);
+	if(qft_fused && cfg.wide_qft_min && n >= cfg.wide_qft_min && n <= 40) {
+		// ten-target plan: a leading pass of (n - 10) mod 10 targets, passes of
+		// ten targets down to bit 10, the last ten targets with the reversal
+		unsigned top = n - 1;
+		const unsigned r = (n - 10) % 10;
+		auto block = [&](unsigned pbot, unsigned K) {
+			PassLaunch L;
+			L.kind = PassLaunch::QFT_WBLOCK;
+			L.pbot = pbot;
+			L.qk = K;
+			plan.launches.push_back(std::move(L));
+		};
+		if(r) {
+			block(n - 10, r);
+			top -= r;
+		}
+		for(; top > 9; top -= 10) block(top - 9, 10);
+		PassLaunch L;
+		L.kind = PassLaunch::QFT_WTAIL;
+		plan.launches.push_back(std::move(L));
+		plan.map = map;
+		for(unsigned b = 0; b < n; ++b) plan.map[b] = b;
+		plan.permute = false;
+		return plan;
+	}
 
 	// 3. emit each pass as a tile kernel.
 	for(size_t pi = 0; pi < passes.size(); ++pi) {
@@ -1485,7 +1540,12 @@ void exec_planned(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, siz
 	if(tgt && tgt->tile_delta) cfg.T = (unsigned)std::max<int>(1, std::min<int>((int)n, (int)cfg.T + tgt->tile_delta));
 	cfg.C = coalesce_bits(h, cfg.T);
 	cfg.qft_c = qft_block_coalesce_bits(h->prec, h->cfg.coalesce_qubits);
+	if(!tgt && h->cfg.qft_plan == 0) cfg.wide_qft_min = qft_wide_min_qubits(h->prec);
 	GatePlan plan = plan_gates(cfg, n, in, lo, hi, h->qmats);
+	if(!plan.launches.empty() && plan.launches.back().kind == PassLaunch::QFT_WTAIL && !reserve_scratch(h)) {
+		cfg.wide_qft_min = 0;  // no room for the second state buffer: in-place plan
+		plan = plan_gates(cfg, n, in, lo, hi, h->qmats);
+	}
 	if(!dual && !tgt && h->cfg.tile_qubits == 0 && cfg.T > 8 && cfg.T == (h->prec == 0 ? 12u : 13u)) {
 		// One bit narrower tiles run about a fifth faster per pass (four
 		// workgroups per CU, three-bit stages); take them when they need no
@@ -1497,6 +1557,22 @@ void exec_planned(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, siz
 		if(p2.launches.size() <= plan.launches.size() && (plan.permute || !p2.permute)) plan = std::move(p2);
 	}
 	for(const PassLaunch &L : plan.launches) {
+		if(L.kind == PassLaunch::QFT_WBLOCK) {
+			ProfScope ps(h, "k_qft_wide_block", pass_bytes(h));
+			HIPC(launch_qft_wide_block(h->L(), A, n, L.pbot, L.qk, h->d_phase, h->d_cin, 0));
+			continue;
+		}
+		if(L.kind == PassLaunch::QFT_WTAIL) {
+			// the last launch of the plan: the state moves into the second buffer,
+			// which becomes the state buffer
+			{
+				ProfScope ps(h, "k_qft_wide_tail", pass_bytes(h));
+				HIPC(launch_qft_wide_tail(h->L(), A, h->scratch, n, h->d_phase, h->d_cin, 0));
+			}
+			std::swap(h->amp, h->scratch);
+			std::swap(h->amp_bytes, h->scratch_bytes);
+			continue;
+		}
 		if(L.kind == PassLaunch::QFT_TAIL) {
 			ProfScope ps(h, "k_qft_tail_reverse", pass_bytes(h));
 			HIPC(launch_qft_tail_reverse(h->L(), A, n, h->d_cin));
@@ -2202,6 +2278,7 @@ int tmfqs_hip_create(const tmfqs_hip_config *cfg, tmfqs_hip_state **out) {
 		if(h->cfg.fuse_qubits < 0 || h->cfg.fuse_qubits > 5 || h->cfg.fuse_qubits == 1)
 			throw std::invalid_argument("tmfqs_hip_create: fuse_qubits must be 0 or 2..5");
 		if(h->cfg.fuse_min_gates < 0) throw std::invalid_argument("tmfqs_hip_create: fuse_min_gates out of range");
+		if(h->cfg.qft_plan < 0 || h->cfg.qft_plan > 1) throw std::invalid_argument("tmfqs_hip_create: qft_plan must be 0 or 1");
 		h->prec = h->cfg.precision;
 		h->esz = h->prec == 0 ? 16 : 8;
 		int count = 0;

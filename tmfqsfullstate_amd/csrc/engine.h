@@ -140,6 +140,17 @@ unsigned qft_block_coalesce_bits(int prec, int requested);
 // bit reversal, in place, for 12 <= n <= 40; dcq holds the 63 constants
 // e^{i pi v / 2^t}, t = 0..5, v < 2^t (at 2^t - 1 + v).
 hipError_t launch_qft_tail_reverse(const Launch &L, void *a, unsigned n, const double *dcq);
+// Ten-target QFT passes (see k_qft.hip).  Block pass, in place, over the
+// block bits pbot+9 .. pbot whose top K (1..10) bits are targets, each with
+// its whole ladder; needs C <= pbot and pbot + 10 <= n (C = 3 complex128,
+// 4 complex64).  Tail: targets 9..0 and the full bit reversal, from src into
+// dst (distinct buffers of 2^n amplitudes), for qft_wide_min_qubits(prec) <=
+// n <= 40.  grid_cap bounds the workgroups (0: one tile each).
+hipError_t launch_qft_wide_block(const Launch &L, void *a, unsigned n, unsigned pbot, unsigned K, const double *dphase,
+                                 const double *dcin, u64 grid_cap);
+hipError_t launch_qft_wide_tail(const Launch &L, const void *src, void *dst, unsigned n, const double *dphase, const double *dcin,
+                                u64 grid_cap);
+unsigned qft_wide_min_qubits(int prec);
 // Chain of diffusions only: dm2 holds nsteps pairs (2 Re m, 2 Im m).  When
 // zero_ok (every 2 Im m is +-0), zp and zm are the chain applied to an
 // imaginary part +0 and -0; wavefronts of real amplitudes use them.

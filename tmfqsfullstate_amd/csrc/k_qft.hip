@@ -2,6 +2,8 @@
 // with the bit reversal.
 #include "device_common.h"
 
+#include <type_traits>
+
 namespace tmfqs_hip {
 
 // ------------------------------------------------------------------------
@@ -509,6 +511,372 @@ static hipError_t launch_qft_tail_t(const Launch &L, void *a, unsigned n, const 
 hipError_t launch_qft_tail_reverse(const Launch &L, void *a, unsigned n, const double *dcq) {
 	if(n < 12 || n > 40) return hipErrorInvalidValue;
 	return L.prec == 0 ? launch_qft_tail_t<double>(L, a, n, dcq) : launch_qft_tail_t<float>(L, a, n, dcq);
+}
+
+// ------------------------------------------------------------------------
+// Ten-target QFT passes: (n - 10) / 10 rounded up block passes plus one tail
+// pass fused with the bit reversal, three full-state passes at n = 30.
+//
+// A workgroup of 512 lanes owns a tile of 2^(10 + C) amplitudes: ten
+// transform bits y and C passenger bits whose 2^C values make one 128 B run
+// (C = 3 in complex128, 4 in complex64).  The ten-bit transform runs in two
+// register stages joined by one LDS transpose:
+//   complex128: 16 amplitudes per lane; stage A holds y bits 9..6 in
+//   registers and bit 5 across lane bit 3 (DPP row rotation by 8), stage B
+//   bits 4..1 in registers and bit 0 across lane bit 3;
+//   complex64: 32 amplitudes per lane; stage A bits 9..5, stage B bits 4..0.
+// The ladder factor e^{i pi (y mod 2^j) / 2^j} of a register target j is a
+// constant of the register bits below j (a uniform operand) times a factor of
+// the lane's other bits.  That lane factor multiplies only registers with bit
+// j set, so it commutes with the stage's remaining register butterflies and
+// is applied once per register after them: in stage A, register ra takes
+// omega^(yA rev(ra)), omega = e^{2 pi i / 1024}, yA the lane's other y bits
+// (one chain of products per lane and tile); in stage B the lane bit is y
+// bit 0 and the factors are constants e^{i pi rev(rb) / 16}.  The phases of
+// the bits r below the block become one twiddle per state, w^(r rev10(y)),
+// w = e^{2 pi i / 2^(pb+10)}, as in the eight-target pass.  Butterflies are
+// unscaled and (1/sqrt(2))^K goes into the twiddle.
+template <typename R> struct WideQft;
+template <> struct WideQft<double> {
+	static constexpr unsigned C = 3, RB = 4, DL = 1;
+};
+template <> struct WideQft<float> {
+	static constexpr unsigned C = 4, RB = 5, DL = 0;
+};
+template <typename R> using WV = typename V2T<R>::t;
+template <typename R> using WP = typename LanePhase<WV<R>>::t;
+
+__host__ __device__ constexpr unsigned rev_bits(unsigned x, unsigned b) {
+	unsigned r = 0;
+	for(unsigned i = 0; i < b; ++i) r |= ((x >> i) & 1u) << (b - 1u - i);
+	return r;
+}
+
+// f(std::integral_constant<int, K>) for K = K0 .. N-1: register indices
+// derived from K (bit reversals) are then constants, so register arrays stay
+// in registers.
+template <int K0, int N, typename F> __device__ __forceinline__ void static_for(F &&f) {
+	if constexpr(K0 < N) {
+		f(std::integral_constant<int, K0>{});
+		static_for<K0 + 1, N>(f);
+	}
+}
+
+// Butterflies over register bit B, then the in-block constants of its
+// target, c[B][low] = e^{i pi low / 2^B} (cst[2^B - 1 + low]), on the
+// registers with bit B set (low = the register bits below B).
+template <int B, int NR, typename V>
+__device__ __forceinline__ void wide_reg_target(V *v, const typename LanePhase<V>::t *cst) {
+#pragma unroll
+	for(int r = 0; r < NR; ++r)
+		if(!(r & (1 << B))) butterfly(v[r], v[r | (1 << B)]);
+#pragma unroll
+	for(int r = 0; r < NR; ++r) {
+		if(!(r & (1 << B))) continue;
+		const int low = r & ((1 << B) - 1);
+		if(low) v[r] = rotp(v[r], cst[(1 << B) - 1 + low]);
+	}
+}
+
+// Register targets B, B-1, ..., lowest (uniform; lowest <= 0: all).
+template <int B, int NR, typename V>
+__device__ __forceinline__ void wide_reg_targets(V *v, int lowest, const typename LanePhase<V>::t *cst) {
+	if constexpr(B >= 0) {
+		if(B >= lowest) {
+			wide_reg_target<B, NR>(v, cst);
+			wide_reg_targets<B - 1, NR>(v, lowest, cst);
+		}
+	}
+}
+
+// Stage A on a lane's registers (register ra = y bits 9 .. 10-RB), the top K
+// y bits being targets (PART: K < 10).  stepA = omega^yA; lfd is the ladder
+// factor of the lane target 9-RB, e^{i pi (yA mod 2^(9-RB)) / 2^(9-RB)} on
+// lanes with that bit set, 1 elsewhere.
+template <typename R, bool PART>
+__device__ __forceinline__ void wide_stage_a(WV<R> *v, unsigned K, double2 stepA, double2 lfd, const WP<R> *cst) {
+	using W = WideQft<R>;
+	constexpr int RB = (int)W::RB, NA = 1 << RB;
+	wide_reg_targets<RB - 1, NA>(v, PART ? RB - (int)K : 0, cst);
+	const unsigned km = (!PART || K >= (unsigned)RB) ? (unsigned)NA - 1u : (1u << K) - 1u;
+	double2 ph = make_double2(1.0, 0.0);
+	static_for<1, NA>([&](auto kc) {
+		constexpr int k = decltype(kc)::value;
+		constexpr int ra = (int)rev_bits((unsigned)k, (unsigned)RB);
+		if(!PART || (k & km)) {
+			ph = cmul(ph, stepA);
+			v[ra] = rot<WV<R>>(v[ra], ph);
+		} else
+			ph = make_double2(1.0, 0.0);
+	});
+	if constexpr(W::DL != 0) {
+		if(!PART || K > (unsigned)RB) {
+			const unsigned lane = threadIdx.x;
+#pragma unroll
+			for(int r = 0; r < NA; ++r) v[r] = rot<WV<R>>(lane_bfly<(1u << W::C)>(v[r], lane), lfd);
+		}
+	}
+}
+
+// Stage B (register rb = y bits RB+DL-1 .. DL; dB = y bit 0 across lane bit
+// C when DL).  The lane factor of target j = b + DL is e^{i pi dB / 2^j};
+// over the register bits it is e^{i pi dB k / 2^RB} on register rev(k)
+// (cst[2^RB - 1 + k]), restricted to the target bits in a partial pass.
+template <typename R, bool PART>
+__device__ __forceinline__ void wide_stage_b(WV<R> *w, unsigned K, unsigned dB, const WP<R> *cst) {
+	using W = WideQft<R>;
+	constexpr int RB = (int)W::RB, NA = 1 << RB, DL = (int)W::DL;
+	wide_reg_targets<RB - 1, NA>(w, PART ? 10 - (int)K - DL : 0, cst);
+	if constexpr(DL != 0) {
+		const int e = PART ? (int)K + DL + RB - 10 : RB;
+		const unsigned mB = e <= 0 ? 0u : (e >= RB ? (unsigned)NA - 1u : (1u << e) - 1u);
+		static_for<1, NA>([&](auto kc) {
+			constexpr int k = decltype(kc)::value;
+			constexpr int rb = (int)rev_bits((unsigned)k, (unsigned)RB);
+			if(PART && !(k & mB)) return;
+			const WV<R> t = rotp(w[rb], cst[(1 << RB) - 1 + (PART ? (k & (int)mB) : k)]);
+			// component selects: a select of the whole vector struct becomes a
+			// select of addresses, which puts the register array in scratch
+			w[rb].x = dB ? t.x : w[rb].x;
+			w[rb].y = dB ? t.y : w[rb].y;
+		});
+		if(!PART || K == 10u) {
+			const unsigned lane = threadIdx.x;
+#pragma unroll
+			for(int r = 0; r < NA; ++r) w[r] = lane_bfly<(1u << W::C)>(w[r], lane);
+		}
+	}
+}
+
+struct WideBlockParams {
+	u64 ntiles;  // 2^(n - 10 - C)
+	uint32_t n, pbot, K, xcd;  // xcd: tiles in XCD-contiguous order (ntiles a multiple of 8)
+	double scale;              // (1/sqrt(2))^K
+};
+
+// Workgroup v's tile when tiles are dealt in XCD-contiguous order: workgroups
+// are dispatched round-robin over the eight XCDs, so v mod 8 names the XCD
+// and each XCD walks its own contiguous eighth of the tiles.
+__device__ __forceinline__ u64 xcd_tile(u64 v, u64 ntiles, unsigned xcd) {
+	return xcd ? ((v & 7ull) * (ntiles >> 3) + (v >> 3)) : v;
+}
+
+// In-place pass over the block bits pbot+9 .. pbot, whose top K bits are
+// targets (K = 10, or fewer in a leading pass); C passenger bits lo at the
+// bottom of the index (pbot >= C).  One tile per workgroup (the grid walks
+// the tiles in XCD-contiguous order), its 128 KiB LDS tile one per CU.
+// Measured at n = 30 against other forms: tiles exchanged through LDS in
+// two 64 KiB halves (two barriers more, nothing gained without a second
+// workgroup per CU, which 16 complex128 amplitudes per lane cannot fit in
+// 128 registers) were 4-8% slower; prefetching the next tile in a
+// persistent grid gained nothing.
+template <typename R, bool PART>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
+k_qft_wide_block(WV<R> *__restrict__ a, WideBlockParams P, const double2 *__restrict__ ptab, const WP<R> *cst) {
+	using V = WV<R>;
+	using W = WideQft<R>;
+	constexpr unsigned C = W::C, RB = W::RB, DL = W::DL, NA = 1u << RB;
+	constexpr unsigned SB = RB + DL;  // stage-B register bits start at y bit DL; the lane bits at y bit SB
+	extern __shared__ char smem[];
+	V *tile = (V *)smem;  // slot (y << C) | lo
+	const unsigned t = threadIdx.x;
+	const unsigned lo = t & ((1u << C) - 1u), u = t >> C;
+	// stage A: y = (ra << (10 - RB)) | yA; with DL, lane bit C is y bit 9-RB
+	const unsigned yA = DL ? (((u & 1u) << (9u - RB)) | (u >> 1)) : u;
+	// stage B: y = (ub << SB) | (rb << DL) | dB
+	const unsigned dB = DL ? (u & 1u) : 0u, ub = u >> DL;
+	const unsigned pb = P.pbot, K = P.K;
+	const u64 lowmask = (1ull << pb) - 1ull;
+	const double2 stepA = phase_of_range((u64)yA << 54, ptab, 0, 1);  // omega^yA
+	double2 lfd = make_double2(1.0, 0.0);
+	if(DL && (yA >> (9u - RB))) lfd = phase_of_range((u64)(yA & ((1u << (9u - RB)) - 1u)) << (54u + RB), ptab, 0, 1);
+	// twiddle exponent rev10(y) restricted to the target bits: the lane's part
+	// mt, and per register 2^(10 - SB) (rev(rb) & m2)
+	const unsigned kmask = (1u << K) - 1u;
+	const unsigned mt = (rev_bits(ub, 10u - SB) | (dB << 9)) & kmask;
+	const unsigned m2 = kmask >> (10u - SB);
+	V v[NA];
+	for(u64 vt = blockIdx.x; vt < P.ntiles; vt += gridDim.x) {
+		const u64 tid = xcd_tile(vt, P.ntiles, P.xcd);
+		const u64 x = tid << C;
+		V *const base = a + ((((x >> pb) << (pb + 10)) | (x & lowmask)) + lo);
+		{
+			const V *src = base + ((u64)yA << pb);
+#pragma unroll
+			for(unsigned ra = 0; ra < NA; ++ra) v[ra] = nt_load(src + ((u64)ra << (pb + 10 - RB)));
+		}
+		wide_stage_a<R, PART>(v, K, stepA, lfd, cst);
+		__syncthreads();  // the previous tile's stage-B reads are complete
+#pragma unroll
+		for(unsigned ra = 0; ra < NA; ++ra) tile[(((ra << (10u - RB)) | yA) << C) | lo] = v[ra];
+		__syncthreads();
+#pragma unroll
+		for(unsigned rb = 0; rb < NA; ++rb) v[rb] = tile[(((ub << SB) | (rb << DL) | dB) << C) | lo];
+		wide_stage_b<R, PART>(v, K, dB, cst);
+		// twiddle w^(r m) (1/sqrt(2))^K, r = the state's bits below the block
+		const u64 r = (x & lowmask) | lo;
+		double2 bph = phase_of((r * mt) << (54u - pb), ptab);
+		bph.x *= P.scale;
+		bph.y *= P.scale;
+		const double2 us = phase_of(r << (64u - pb - SB), ptab);  // w^(2^(10-SB) r)
+		double2 ph = bph;
+		static_for<0, (int)NA>([&](auto kc) {
+			constexpr unsigned k = (unsigned)decltype(kc)::value;
+			constexpr unsigned rb = rev_bits(k, RB);
+			if(k) ph = (!PART || (k & m2)) ? cmul(ph, us) : bph;
+			v[rb] = rot<V>(v[rb], ph);
+		});
+#pragma unroll
+		for(unsigned rb = 0; rb < NA; ++rb) nt_store(v[rb], base + ((u64)((ub << SB) | (rb << DL) | dB) << pb));
+	}
+}
+
+struct WideTailParams {
+	u64 ntiles;  // 2^(n - 10 - C)
+	uint32_t n, f, xcd, pad;  // f = n - 10 - C: bits between the passengers and the targets
+};
+
+// QFT targets 9..0 followed by the full bit reversal, out of place.  A tile
+// holds the ten target bits Lo and the top C bits h of the index (the
+// passengers: after the reversal they are the bottom C bits, one 128 B run
+// per output row); tile p has the middle bits F = rev_f(p), so its output
+// index is (rev10(Lo), p, rev_C(h)) and consecutive tiles write consecutive
+// runs.  Stage A lanes: h bit 0 on lane bit C-1, the low 10 - RB bits of Lo
+// (Lo bit 9-RB on lane bit C in complex128) and the other h bits on top;
+// each wavefront reads runs of 256 B or more (measured 7% faster than 1 KiB
+// runs with all of h on top).  Stage B lanes: rev_C(h) on the bottom C lane
+// bits, then Lo bit 0 (complex128) and Lo bits 9..5.
+template <typename R>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
+k_qft_wide_tail(const WV<R> *__restrict__ a, WV<R> *__restrict__ o, WideTailParams P, const double2 *__restrict__ ptab,
+                const WP<R> *cst) {
+	using V = WV<R>;
+	using W = WideQft<R>;
+	constexpr unsigned C = W::C, RB = W::RB, DL = W::DL, NA = 1u << RB, CM = (1u << C) - 1u, HM = (1u << (C - 1u)) - 1u;
+	extern __shared__ char smem[];
+	V *tile = (V *)smem;  // slot (Lo << C) | (h ^ (Lo mod 2^C)): conflict-free both ways
+	const unsigned t = threadIdx.x;
+	const unsigned h0 = (t >> (C - 1u)) & 1u;  // h bit 0: lane bit C-1 in both stages
+	// stage A: lane bits below C-1 are Lo bits C-2..0; with DL lane bit C is
+	// Lo bit 9-RB; the next lane bits hold the remaining low Lo bits, then h.
+	unsigned yA, hA;
+	if constexpr(DL != 0) {
+		constexpr unsigned NL = 10u - RB - (C - 1u) - 1u;  // Lo bits above C-2 besides bit 9-RB
+		const unsigned rest = t >> (C + 1u);
+		yA = (t & HM) | ((rest & ((1u << NL) - 1u)) << (C - 1u)) | (((t >> C) & 1u) << (9u - RB));
+		hA = h0 | ((rest >> NL) << 1);
+	} else {
+		constexpr unsigned NL = 10u - RB - (C - 1u);
+		const unsigned rest = t >> C;
+		yA = (t & HM) | ((rest & ((1u << NL) - 1u)) << (C - 1u));
+		hA = h0 | ((rest >> NL) << 1);
+	}
+	const unsigned g = t & CM, u = t >> C;
+	const unsigned dB = DL ? (u & 1u) : 0u, ub = u >> DL;
+	const unsigned hB = rev_bits(g, C);
+	const double2 stepA = phase_of_range((u64)yA << 54, ptab, 0, 1);
+	double2 lfd = make_double2(1.0, 0.0);
+	if(DL && (yA >> (9u - RB))) lfd = phase_of_range((u64)(yA & ((1u << (9u - RB)) - 1u)) << (54u + RB), ptab, 0, 1);
+	const unsigned n = P.n, f = P.f;
+	V v[NA];
+	for(u64 vt = blockIdx.x; vt < P.ntiles; vt += gridDim.x) {
+		const u64 tid = xcd_tile(vt, P.ntiles, P.xcd);
+		{
+			const u64 F = f ? (u64)(__brev((unsigned)tid) >> (32u - f)) : 0ull;
+			const V *src = a + (((u64)hA << (n - C)) | (F << 10) | yA);
+#pragma unroll
+			for(unsigned ra = 0; ra < NA; ++ra) v[ra] = nt_load(src + ((u64)ra << (10u - RB)));
+		}
+		wide_stage_a<R, false>(v, 10u, stepA, lfd, cst);
+		__syncthreads();
+#pragma unroll
+		for(unsigned ra = 0; ra < NA; ++ra) {
+			const unsigned L = (ra << (10u - RB)) | yA;
+			tile[(L << C) | (hA ^ (L & CM))] = v[ra];
+		}
+		__syncthreads();
+#pragma unroll
+		for(unsigned rb = 0; rb < NA; ++rb) {
+			const unsigned L = (ub << (RB + DL)) | (rb << DL) | dB;
+			v[rb] = tile[(L << C) | (hB ^ (L & CM))];
+		}
+		wide_stage_b<R, false>(v, 10u, dB, cst);
+		V *dst = o + ((tid << C) | g);
+#pragma unroll
+		for(unsigned rb = 0; rb < NA; ++rb) {
+			const unsigned L = (ub << (RB + DL)) | (rb << DL) | dB;
+			const V x = v[rb];
+			nt_store(mk<V>(x.x * 0.03125f, x.y * 0.03125f), dst + ((u64)rev_bits(L, 10u) << (n - 10u)));
+		}
+	}
+}
+
+unsigned qft_wide_min_qubits(int prec) { return 10u + (prec == 0 ? WideQft<double>::C : WideQft<float>::C); }
+
+template <typename R, bool PART>
+static hipError_t launch_qft_wide_block_t(const Launch &L, void *a, unsigned n, unsigned pbot, unsigned K, const double *dphase,
+                                          const double *dcin, u64 grid_cap) {
+	using V = WV<R>;
+	WideBlockParams P;
+	P.n = n;
+	P.pbot = pbot;
+	P.K = K;
+	P.ntiles = 1ull << (n - 10 - WideQft<R>::C);
+	P.xcd = P.ntiles >= 8 ? 1u : 0u;
+	P.scale = std::ldexp(K & 1u ? 1.0 / std::sqrt(2.0) : 1.0, -(int)(K / 2));  // (1/sqrt(2))^K, one rounding at most
+	const u64 cap = grid_cap ? grid_cap : P.ntiles;
+	const unsigned grid = (unsigned)(P.ntiles < cap ? P.ntiles : cap);
+	const size_t shmem = ((size_t)1 << (10 + WideQft<R>::C)) * sizeof(V);
+	static bool attr = false;
+	if(!attr) {
+		(void)hipFuncSetAttribute((const void *)k_qft_wide_block<R, PART>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+		attr = true;
+	}
+	const WP<R> *cinp = sizeof(R) == 8 ? (const WP<R> *)dcin : (const WP<R> *)(dcin + 126);
+	hipLaunchKernelGGL((k_qft_wide_block<R, PART>), dim3(grid), dim3(512), shmem, L.stream, (V *)a, P, (const double2 *)dphase, cinp);
+	return hipGetLastError();
+}
+
+hipError_t launch_qft_wide_block(const Launch &L, void *a, unsigned n, unsigned pbot, unsigned K, const double *dphase,
+                                 const double *dcin, u64 grid_cap) {
+	const unsigned C = L.prec == 0 ? WideQft<double>::C : WideQft<float>::C;
+	if(n > 40 || pbot < C || pbot + 10 > n || K < 1 || K > 10) return hipErrorInvalidValue;
+	if(L.prec == 0)
+		return K == 10 ? launch_qft_wide_block_t<double, false>(L, a, n, pbot, K, dphase, dcin, grid_cap)
+		               : launch_qft_wide_block_t<double, true>(L, a, n, pbot, K, dphase, dcin, grid_cap);
+	return K == 10 ? launch_qft_wide_block_t<float, false>(L, a, n, pbot, K, dphase, dcin, grid_cap)
+	               : launch_qft_wide_block_t<float, true>(L, a, n, pbot, K, dphase, dcin, grid_cap);
+}
+
+template <typename R>
+static hipError_t launch_qft_wide_tail_t(const Launch &L, const void *src, void *dst, unsigned n, const double *dphase,
+                                         const double *dcin, u64 grid_cap) {
+	using V = WV<R>;
+	WideTailParams P;
+	P.n = n;
+	P.f = n - 10 - WideQft<R>::C;
+	P.pad = 0;
+	P.ntiles = 1ull << P.f;
+	P.xcd = P.ntiles >= 8 ? 1u : 0u;
+	const u64 cap = grid_cap ? grid_cap : P.ntiles;
+	const unsigned grid = (unsigned)(P.ntiles < cap ? P.ntiles : cap);
+	const size_t shmem = ((size_t)1 << (10 + WideQft<R>::C)) * sizeof(V);
+	static bool attr = false;
+	if(!attr) {
+		(void)hipFuncSetAttribute((const void *)k_qft_wide_tail<R>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+		attr = true;
+	}
+	const WP<R> *cinp = sizeof(R) == 8 ? (const WP<R> *)dcin : (const WP<R> *)(dcin + 126);
+	hipLaunchKernelGGL((k_qft_wide_tail<R>), dim3(grid), dim3(512), shmem, L.stream, (const V *)src, (V *)dst, P,
+	                   (const double2 *)dphase, cinp);
+	return hipGetLastError();
+}
+
+hipError_t launch_qft_wide_tail(const Launch &L, const void *src, void *dst, unsigned n, const double *dphase, const double *dcin,
+                                u64 grid_cap) {
+	if(n < qft_wide_min_qubits(L.prec) || n > 40 || src == dst) return hipErrorInvalidValue;
+	return L.prec == 0 ? launch_qft_wide_tail_t<double>(L, src, dst, n, dphase, dcin, grid_cap)
+	                   : launch_qft_wide_tail_t<float>(L, src, dst, n, dphase, dcin, grid_cap);
 }
 
 }  // namespace tmfqs_hip

@@ -56,6 +56,7 @@ tmfqs::RegisterConfig regConfig(const tmfqs_hip_config *c) {
 	cfg.hip.exactPhases = c->exact_phases != 0;
 	cfg.hip.fuseQubits = static_cast<unsigned>(c->fuse_qubits);
 	cfg.hip.fuseMinGates = static_cast<unsigned>(c->fuse_min_gates);
+	cfg.hip.qftPlan = static_cast<unsigned>(c->qft_plan);
 	return cfg;
 }
 

@@ -45,6 +45,7 @@ tmfqs_hip_config toHipConfig(const HipConfig &c) {
 	h.exact_phases = c.exactPhases ? 1 : 0;
 	h.fuse_qubits = static_cast<int>(c.fuseQubits);
 	h.fuse_min_gates = static_cast<int>(c.fuseMinGates);
+	h.qft_plan = static_cast<int>(c.qftPlan);
 	return h;
 }
 
