@@ -1279,6 +1279,15 @@ GatePlan plan_gates(const PlanCfg &cfg, unsigned n, const std::vector<QOp> &in, 
 			L.stages = plan_stages(L.ops.data(), L.ops.size(), Tp, W);
 			for(const TileStage &st : L.stages)
 				if(st.lds) L.ldsops = 1;
+			unsigned Wl = W;
+			if(cfg.prec == 0 && !L.ldsops && !cfg.W && W > 1 && Tp == W + 8) {
+				// complex128, register stages only: 512 lanes with stages one bit
+				// narrower hold 127 (dual: 103) VGPRs, so four wavefronts share a
+				// SIMD at the same LDS tile (measured: Hadamard layers 8-9% and the
+				// n = 24 adjoint 7% faster; the complex64 dual pass was slower)
+				Wl = W - 1;
+				L.stages = plan_stages(L.ops.data(), L.ops.size(), Tp, Wl);
+			}
 			if(L.ldsops && cfg.prec == 0 && Tp >= 11) {
 				// complex128 five-qubit blocks, all with MFMA operands: the MFMA build
 				bool k5 = false, all = true;
@@ -1294,7 +1303,7 @@ GatePlan plan_gates(const PlanCfg &cfg, unsigned n, const std::vector<QOp> &in, 
 			L.tp = tp;
 			L.tp.nops = (unsigned)L.ops.size();
 			L.tp.op0 = 0;
-			L.W = W;
+			L.W = Wl;
 			plan.launches.push_back(std::move(L));
 			s0 = e;
 		}

@@ -640,50 +640,68 @@ __host__ __device__ __forceinline__ unsigned staged_tile_elems(unsigned T, bool 
 	return ldsops ? (1u << T) : (1u << T) + ((1u << T) >> 4);
 }
 
-template <typename R, int W, bool DUAL, int LDS>
-__global__ void __launch_bounds__(256) k_tile_staged(typename V2T<R>::t *__restrict__ a, typename V2T<R>::t *__restrict__ b,
-                                                     TilePass P, const TileOp *__restrict__ ops,
-                                                     const TileStage *__restrict__ stages, unsigned nstages,
-                                                     const double *__restrict__ mats, const TwiddleTerm *__restrict__ terms,
-                                                     const double2 *__restrict__ ptab, double *__restrict__ gpart,
-                                                     unsigned nslots) {
+// TH lanes per workgroup: 256, or 512 for register-stage builds whose stages
+// hold 2^(T-9) amplitudes per lane (twice the wavefronts per CU at the same LDS
+// tile, with one-bit-narrower stages).
+template <typename R, int W, bool DUAL, int LDS, int TH>
+__global__ void __launch_bounds__(TH) k_tile_staged(typename V2T<R>::t *__restrict__ a, typename V2T<R>::t *__restrict__ b,
+                                                    TilePass P, const TileOp *__restrict__ ops,
+                                                    const TileStage *__restrict__ stages, unsigned nstages,
+                                                    const double *__restrict__ mats, const TwiddleTerm *__restrict__ terms,
+                                                    const double2 *__restrict__ ptab, double *__restrict__ gpart,
+                                                    unsigned nslots) {
 	using V = typename V2T<R>::t;
+	constexpr unsigned NW = TH / 64;
 	extern __shared__ char smem[];
 	const unsigned nlocp = staged_tile_elems(P.T, LDS != 0);
 	V *ta = (V *)smem;
 	V *tb = ta + nlocp;  // second tile (dual passes only)
 	u64 *hitab = (u64 *)(ta + (DUAL ? 2u : 1u) * nlocp);
-	double *gacc = (double *)(hitab + (1u << P.nh));  // dual: [4 waves][kMaxGradSlots]
+	double *gacc = (double *)(hitab + (1u << P.nh));  // dual: [NW wavefronts][kMaxGradSlots]
 	tile_hitab<R>(hitab, P);
 	if(DUAL)
-		for(unsigned i = threadIdx.x; i < 4 * kMaxGradSlots; i += blockDim.x) gacc[i] = 0.0;
+		for(unsigned i = threadIdx.x; i < NW * kMaxGradSlots; i += blockDim.x) gacc[i] = 0.0;
 	for(u64 tid = blockIdx.x; tid < P.ntiles; tid += gridDim.x)
 		staged_tile<R, W, DUAL, LDS>(a, b, P, tid, ops, stages, nstages, mats, terms, ptab, ta, tb, hitab, gacc);
 	if(DUAL) {
 		__syncthreads();
-		for(unsigned s = threadIdx.x; s < nslots; s += blockDim.x)
-			gpart[(u64)s * gridDim.x + blockIdx.x] =
-			    ((gacc[s] + gacc[kMaxGradSlots + s]) + gacc[2 * kMaxGradSlots + s]) + gacc[3 * kMaxGradSlots + s];
+		for(unsigned s = threadIdx.x; s < nslots; s += blockDim.x) {
+			double sum = gacc[s];
+#pragma unroll
+			for(unsigned w = 1; w < NW; ++w) sum += gacc[w * kMaxGradSlots + s];  // fixed order
+			gpart[(u64)s * gridDim.x + blockIdx.x] = sum;
+		}
 	}
+}
+
+template <typename R, int W, bool DUAL, int LDS, int TH>
+static hipError_t launch_staged_th(const Launch &L, void *a, void *b, const TilePass &P, const TileOp *dops,
+                                   const TileStage *dstages, unsigned nstages, const double *dmats, const TwiddleTerm *dterms,
+                                   const double *dphase, double *gpart, unsigned nslots) {
+	using V = typename V2T<R>::t;
+	const size_t nlocp = staged_tile_elems(P.T, LDS != 0);
+	const size_t shmem =
+	    (DUAL ? 2 : 1) * sizeof(V) * nlocp + (8ull << P.nh) + (DUAL ? (TH / 64) * kMaxGradSlots * sizeof(double) : 0);
+	if(shmem > 160 * 1024) return hipErrorInvalidValue;
+	static bool attr = false;
+	if(!attr) {
+		(void)hipFuncSetAttribute((const void *)k_tile_staged<R, W, DUAL, LDS, TH>, hipFuncAttributeMaxDynamicSharedMemorySize,
+		                          160 * 1024);
+		attr = true;
+	}
+	TMFQS_LAUNCH((k_tile_staged<R, W, DUAL, LDS, TH>), tile_grid(P, DUAL, L.prec), TH, shmem, L.stream, (V *)a, (V *)b, P, dops,
+	             dstages, nstages, dmats, dterms, (const double2 *)dphase, gpart, nslots);
+	return hipGetLastError();
 }
 
 template <typename R, int W, bool DUAL, int LDS>
 static hipError_t launch_staged_t(const Launch &L, void *a, void *b, const TilePass &P, const TileOp *dops, const TileStage *dstages,
                                   unsigned nstages, const double *dmats, const TwiddleTerm *dterms, const double *dphase,
                                   double *gpart, unsigned nslots) {
-	using V = typename V2T<R>::t;
-	const size_t nlocp = staged_tile_elems(P.T, LDS != 0);
-	const size_t shmem = (DUAL ? 2 : 1) * sizeof(V) * nlocp + (8ull << P.nh) + (DUAL ? 4 * kMaxGradSlots * sizeof(double) : 0);
-	if(shmem > 160 * 1024) return hipErrorInvalidValue;
-	static bool attr = false;
-	if(!attr) {
-		(void)hipFuncSetAttribute((const void *)k_tile_staged<R, W, DUAL, LDS>, hipFuncAttributeMaxDynamicSharedMemorySize,
-		                          160 * 1024);
-		attr = true;
-	}
-	TMFQS_LAUNCH((k_tile_staged<R, W, DUAL, LDS>), tile_grid(P, DUAL, L.prec), 256, shmem, L.stream, (V *)a, (V *)b, P, dops, dstages, nstages,
-	             dmats, dterms, (const double2 *)dphase, gpart, nslots);
-	return hipGetLastError();
+	if constexpr(LDS == 0 && W <= 3)
+		if(P.T >= 9 + (unsigned)W)  // 512 lanes x 2^W amplitudes cover the tile
+			return launch_staged_th<R, W, DUAL, LDS, 512>(L, a, b, P, dops, dstages, nstages, dmats, dterms, dphase, gpart, nslots);
+	return launch_staged_th<R, W, DUAL, LDS, 256>(L, a, b, P, dops, dstages, nstages, dmats, dterms, dphase, gpart, nslots);
 }
 
 template <typename R, bool DUAL, int LDS>
