@@ -6,9 +6,10 @@ cannot run at that size:
 * Grover-30 complex64, marked {1, 2^29 + 5, 2^30 - 1}, 14858 iterations:
   marked-state probability against the closed form sin^2((2k+1) asin
   sqrt(M/N)), fidelity against the complex128 run, measured index.
-* adjoint n = 24 complex64 (8 ring layers, 192 angles): against the
-  complex128 adjoint, which tests/test_gpu_adjoint.py pins against FD and
-  parameter shift.
+* adjoint n = 24 complex128 (8 ring layers, 192 angles): the expectation
+  against the oracle and four gradients against the oracle's parameter-shift
+  rule (eight 24-qubit oracle circuits); complex64 against the complex128
+  adjoint.
 * the seed-42 batch of 256 circuits: complex128 bit-exact against the
   oracle for every circuit; complex64 fidelity against the oracle; batched
   adjoint gradients against the single-circuit adjoint and, on a subset,
@@ -85,6 +86,32 @@ def test_grover30_complex64_three_marked():
     assert 1.0 - cmp["fidelity"] <= 1e-6, cmp
     assert cmp["max_abs_amplitude_error"] <= 1e-6
     assert r128.measure_seed(123456) == m64
+
+
+def test_adjoint_n24_complex128_vs_oracle_parameter_shift():
+    # E(theta + pi/2) - E(theta - pi/2) over 2 for four of the 192 angles (the
+    # first and last layer's ends and two in between), each energy a full
+    # 24-qubit oracle circuit (about 3 s per circuit on the host).
+    n = 24
+    ops, cnt = S.ring_ops(n, 8, 3)
+    with T.HipState() as s:
+        s.init_basis(n, 0)
+        e, g = s.adjoint(ops, cnt, Z0_PLUS_P0)
+    oops, ocnt = O.ring_ops(n, 8, 3)
+    assert ocnt == cnt
+    for i in range(cnt):
+        assert oops[i].kind == ops[i].kind and oops[i].theta == ops[i].theta
+    terms = [(O.DIAG_Z, 1.0, 1 << 0), (O.DIAG_PROJECTOR, 1.0, 0)]
+    print("adjoint n=24: oracle energy", flush=True)
+    assert abs(e - O.energy(n, oops, ocnt, terms)) <= 1e-12
+    params = O.param_indices(oops, ocnt)
+    assert len(params) == g.size == 192
+    for j in (0, 61, 130, 191):
+        print(f"adjoint n=24: oracle parameter shift of angle {j}", flush=True)
+        i = params[j]
+        ep = O.energy(n, O._shifted(oops, ocnt, i, np.pi / 2), ocnt, terms)
+        em = O.energy(n, O._shifted(oops, ocnt, i, -np.pi / 2), ocnt, terms)
+        assert abs(g[j] - 0.5 * (ep - em)) <= 1e-10, j
 
 
 def test_adjoint_n24_complex64_vs_complex128():
