@@ -145,11 +145,13 @@ hipError_t launch_qft_tail_reverse(const Launch &L, void *a, unsigned n, const d
 // its whole ladder; needs C <= pbot and pbot + 10 <= n (C = 3 complex128,
 // 4 complex64).  Tail: targets 9..0 and the full bit reversal, from src into
 // dst (distinct buffers of 2^n amplitudes), for qft_wide_min_qubits(prec) <=
-// n <= 40.  grid_cap bounds the workgroups (0: one tile each).
+// n <= 40.  Persistent workgroups that load their next tile into LDS during
+// the current one; grid_cap sets the workgroup count (0: the measured
+// default for the pass).
 hipError_t launch_qft_wide_block(const Launch &L, void *a, unsigned n, unsigned pbot, unsigned K, const double *dphase,
-                                 const double *dcin, u64 grid_cap);
+                                 const double *dcin, u64 grid_cap = 0);
 hipError_t launch_qft_wide_tail(const Launch &L, const void *src, void *dst, unsigned n, const double *dphase, const double *dcin,
-                                u64 grid_cap);
+                                u64 grid_cap = 0);
 unsigned qft_wide_min_qubits(int prec);
 // Chain of diffusions only: dm2 holds nsteps pairs (2 Re m, 2 Im m).  When
 // zero_ok (every 2 Im m is +-0), zp and zm are the chain applied to an

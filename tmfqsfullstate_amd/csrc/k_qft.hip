@@ -2,6 +2,7 @@
 // with the bit reversal.
 #include "device_common.h"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace tmfqs_hip {
@@ -661,64 +662,110 @@ __device__ __forceinline__ u64 xcd_tile(u64 v, u64 ntiles, unsigned xcd) {
 	return xcd ? ((v & 7ull) * (ntiles >> 3) + (v >> 3)) : v;
 }
 
+// LDS-direct loads (gfx950 global_load_lds_dwordx4): lane l of a wavefront's
+// instruction writes its 16 bytes at the instruction's LDS base + 16 l; no
+// registers are involved, and the vector-memory counter covers them in issue
+// order with every other vector memory access of the wave.
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void glb_void_t;
+
+template <unsigned N> __device__ __forceinline__ void wait_vmcnt() {
+	static_assert(N == 0 || N == 16 || N == 32, "wait_vmcnt");
+	if constexpr(N == 0) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+	else if constexpr(N == 16) asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
+	else asm volatile("s_waitcnt vmcnt(32)" ::: "memory");
+}
+
 // In-place pass over the block bits pbot+9 .. pbot, whose top K bits are
 // targets (K = 10, or fewer in a leading pass); C passenger bits lo at the
-// bottom of the index (pbot >= C).  One tile per workgroup (the grid walks
-// the tiles in XCD-contiguous order), its 128 KiB LDS tile one per CU.
-// Measured at n = 30 against other forms: tiles exchanged through LDS in
-// two 64 KiB halves (two barriers more, nothing gained without a second
-// workgroup per CU, which 16 complex128 amplitudes per lane cannot fit in
-// 128 registers) were 4-8% slower; prefetching the next tile in a
-// persistent grid gained nothing.
+// bottom of the index (pbot >= C).  A 128 KiB LDS tile per workgroup (one
+// workgroup per CU), tiles dealt in XCD-contiguous order to a persistent
+// grid.  The next tile is loaded straight into LDS while the current one
+// finishes: once stage B has read its tile out of LDS, the next tile's rows
+// are requested into it (16 LDS-direct loads per lane), so they are in flight
+// during stage B, the twiddle and the stores; those stores drain during the
+// next tile's stage A.  The LDS tile holds the rows as they sit in memory
+// (slot (y << C) | lo), which is also the exchange layout.  Waiting for the
+// next tile takes vmcnt(NA): the NA stores issued after its loads may stay
+// outstanding.  Nothing else may sit on that counter between the loads and
+// the wait: the stage constants come through scalar loads and the twiddle's
+// phase tables are copied into LDS beside the tile.
+// Measured at n = 30 against other forms: one tile per workgroup with
+// register loads 8-16% slower (complex128 9.0 / 7.2 ms against 8.3 / 6.1 ms
+// at pbot 20 / 10); prefetching into registers spilled; exchanging the tile
+// through LDS in two 64 KiB halves (without a second workgroup per CU,
+// which 16 complex128 amplitudes per lane cannot fit in 128 registers) was
+// 4-8% slower.
 template <typename R, bool PART>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
-k_qft_wide_block(WV<R> *__restrict__ a, WideBlockParams P, const double2 *__restrict__ ptab, const WP<R> *cst) {
+k_qft_wide_block(WV<R> *__restrict__ a, WideBlockParams P, const double2 *__restrict__ ptab, const WP<R> *__restrict__ cst) {
 	using V = WV<R>;
 	using W = WideQft<R>;
-	constexpr unsigned C = W::C, RB = W::RB, DL = W::DL, NA = 1u << RB;
-	constexpr unsigned SB = RB + DL;  // stage-B register bits start at y bit DL; the lane bits at y bit SB
+	constexpr unsigned C = W::C, RB = W::RB, DL = W::DL, NA = 1u << RB, CM = (1u << C) - 1u;
+	constexpr unsigned SB = RB + DL;
+	constexpr unsigned EPL = 16u / (unsigned)sizeof(V);            // slots per lane and load
+	constexpr unsigned NDMA = (1u << (10u + C)) / (512u * EPL);   // loads per lane and tile
+	static_assert(NDMA * 512u * EPL == (1u << (10u + C)), "tile");
 	extern __shared__ char smem[];
-	V *tile = (V *)smem;  // slot (y << C) | lo
+	V *tile = (V *)smem;
 	const unsigned t = threadIdx.x;
-	const unsigned lo = t & ((1u << C) - 1u), u = t >> C;
-	// stage A: y = (ra << (10 - RB)) | yA; with DL, lane bit C is y bit 9-RB
+	const unsigned lo = t & CM, u = t >> C;
 	const unsigned yA = DL ? (((u & 1u) << (9u - RB)) | (u >> 1)) : u;
-	// stage B: y = (ub << SB) | (rb << DL) | dB
 	const unsigned dB = DL ? (u & 1u) : 0u, ub = u >> DL;
 	const unsigned pb = P.pbot, K = P.K;
 	const u64 lowmask = (1ull << pb) - 1ull;
-	const double2 stepA = phase_of_range((u64)yA << 54, ptab, 0, 1);  // omega^yA
+	const double2 stepA = phase_of_range((u64)yA << 54, ptab, 0, 1);
 	double2 lfd = make_double2(1.0, 0.0);
 	if(DL && (yA >> (9u - RB))) lfd = phase_of_range((u64)(yA & ((1u << (9u - RB)) - 1u)) << (54u + RB), ptab, 0, 1);
-	// twiddle exponent rev10(y) restricted to the target bits: the lane's part
-	// mt, and per register 2^(10 - SB) (rev(rb) & m2)
 	const unsigned kmask = (1u << K) - 1u;
 	const unsigned mt = (rev_bits(ub, 10u - SB) | (dB << 9)) & kmask;
 	const unsigned m2 = kmask >> (10u - SB);
-	V v[NA];
-	for(u64 vt = blockIdx.x; vt < P.ntiles; vt += gridDim.x) {
-		const u64 tid = xcd_tile(vt, P.ntiles, P.xcd);
-		const u64 x = tid << C;
-		V *const base = a + ((((x >> pb) << (pb + 10)) | (x & lowmask)) + lo);
-		{
-			const V *src = base + ((u64)yA << pb);
+	const unsigned wv = __builtin_amdgcn_readfirstlane(t >> 6), ln = t & 63u;
+	// the twiddle's phase tables in LDS after the tile: their reads do not
+	// queue behind the next tile's loads on the vector-memory counter
+	double2 *ltab = (double2 *)(smem + ((size_t)1 << (10u + C)) * sizeof(V));
+	for(unsigned i = t; i < 5u * 256u; i += 512u) ltab[i] = ptab[i];
+	// load i of lane ln fills slot s = (i * 8 + wv) * 64 EPL + ln EPL = (y << C) | lo
+	// with y = 64 i + ((wv * 64 + ln) EPL >> C) (2^C = 8 EPL): one lane offset
+	// plus i rows of 64 << pb (kept as one base, not sixteen offsets in registers)
+	const unsigned s_lane = (wv * 64u + ln) * EPL;
+	const u64 lane_off = ((u64)(s_lane >> C) << pb) + (s_lane & CM);
+	auto request = [&](u64 vt) {
+		const u64 x = xcd_tile(vt, P.ntiles, P.xcd) << C;
+		const V *src = a + ((((x >> pb) << (pb + 10)) | (x & lowmask)) + lane_off);
 #pragma unroll
-			for(unsigned ra = 0; ra < NA; ++ra) v[ra] = nt_load(src + ((u64)ra << (pb + 10 - RB)));
-		}
+		for(unsigned i = 0; i < NDMA; ++i)
+			__builtin_amdgcn_global_load_lds((glb_void_t *)(src + ((u64)i << (pb + 6))), (lds_void_t *)(tile + (i * 8u + wv) * 64u * EPL),
+			                                 16, 0, 2);
+	};
+	V v[NA];
+	u64 vt = blockIdx.x;
+	if(vt < P.ntiles) request(vt);
+	bool first = true;
+	for(; vt < P.ntiles; vt += gridDim.x) {
+		const u64 x = xcd_tile(vt, P.ntiles, P.xcd) << C;
+		V *const base = a + ((((x >> pb) << (pb + 10)) | (x & lowmask)) + lo);
+		if(first) wait_vmcnt<0>();
+		else wait_vmcnt<NA>();
+		first = false;
+		__syncthreads();  // every wavefront's part of the tile is in LDS
+#pragma unroll
+		for(unsigned ra = 0; ra < NA; ++ra) v[ra] = tile[(((ra << (10u - RB)) | yA) << C) | lo];
 		wide_stage_a<R, PART>(v, K, stepA, lfd, cst);
-		__syncthreads();  // the previous tile's stage-B reads are complete
+		__syncthreads();  // every lane has read its stage-A amplitudes
 #pragma unroll
 		for(unsigned ra = 0; ra < NA; ++ra) tile[(((ra << (10u - RB)) | yA) << C) | lo] = v[ra];
 		__syncthreads();
 #pragma unroll
 		for(unsigned rb = 0; rb < NA; ++rb) v[rb] = tile[(((ub << SB) | (rb << DL) | dB) << C) | lo];
+		__syncthreads();  // the tile is in registers: the LDS takes the next one
+		if(vt + gridDim.x < P.ntiles) request(vt + gridDim.x);
 		wide_stage_b<R, PART>(v, K, dB, cst);
-		// twiddle w^(r m) (1/sqrt(2))^K, r = the state's bits below the block
 		const u64 r = (x & lowmask) | lo;
-		double2 bph = phase_of((r * mt) << (54u - pb), ptab);
+		double2 bph = phase_of((r * mt) << (54u - pb), ltab);
 		bph.x *= P.scale;
 		bph.y *= P.scale;
-		const double2 us = phase_of(r << (64u - pb - SB), ptab);  // w^(2^(10-SB) r)
+		const double2 us = phase_of(r << (64u - pb - SB), ltab);
 		double2 ph = bph;
 		static_for<0, (int)NA>([&](auto kc) {
 			constexpr unsigned k = (unsigned)decltype(kc)::value;
@@ -736,32 +783,14 @@ struct WideTailParams {
 	uint32_t n, f, xcd, pad;  // f = n - 10 - C: bits between the passengers and the targets
 };
 
-// QFT targets 9..0 followed by the full bit reversal, out of place.  A tile
-// holds the ten target bits Lo and the top C bits h of the index (the
-// passengers: after the reversal they are the bottom C bits, one 128 B run
-// per output row); tile p has the middle bits F = rev_f(p), so its output
-// index is (rev10(Lo), p, rev_C(h)) and consecutive tiles write consecutive
-// runs.  Stage A lanes: h bit 0 on lane bit C-1, the low 10 - RB bits of Lo
-// (Lo bit 9-RB on lane bit C in complex128) and the other h bits on top;
-// each wavefront reads runs of 256 B or more (measured 7% faster than 1 KiB
-// runs with all of h on top).  Stage B lanes: rev_C(h) on the bottom C lane
-// bits, then Lo bit 0 (complex128) and Lo bits 9..5.
+// Stage-A coordinates of lane t in the tail pass: its Lo bits yA and its h.
 template <typename R>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
-k_qft_wide_tail(const WV<R> *__restrict__ a, WV<R> *__restrict__ o, WideTailParams P, const double2 *__restrict__ ptab,
-                const WP<R> *cst) {
-	using V = WV<R>;
+__device__ __forceinline__ void wide_tail_lane(unsigned t, unsigned &yA, unsigned &hA) {
 	using W = WideQft<R>;
-	constexpr unsigned C = W::C, RB = W::RB, DL = W::DL, NA = 1u << RB, CM = (1u << C) - 1u, HM = (1u << (C - 1u)) - 1u;
-	extern __shared__ char smem[];
-	V *tile = (V *)smem;  // slot (Lo << C) | (h ^ (Lo mod 2^C)): conflict-free both ways
-	const unsigned t = threadIdx.x;
-	const unsigned h0 = (t >> (C - 1u)) & 1u;  // h bit 0: lane bit C-1 in both stages
-	// stage A: lane bits below C-1 are Lo bits C-2..0; with DL lane bit C is
-	// Lo bit 9-RB; the next lane bits hold the remaining low Lo bits, then h.
-	unsigned yA, hA;
+	constexpr unsigned C = W::C, RB = W::RB, DL = W::DL, HM = (1u << (C - 1u)) - 1u;
+	const unsigned h0 = (t >> (C - 1u)) & 1u;
 	if constexpr(DL != 0) {
-		constexpr unsigned NL = 10u - RB - (C - 1u) - 1u;  // Lo bits above C-2 besides bit 9-RB
+		constexpr unsigned NL = 10u - RB - (C - 1u) - 1u;
 		const unsigned rest = t >> (C + 1u);
 		yA = (t & HM) | ((rest & ((1u << NL) - 1u)) << (C - 1u)) | (((t >> C) & 1u) << (9u - RB));
 		hA = h0 | ((rest >> NL) << 1);
@@ -771,6 +800,39 @@ k_qft_wide_tail(const WV<R> *__restrict__ a, WV<R> *__restrict__ o, WideTailPara
 		yA = (t & HM) | ((rest & ((1u << NL) - 1u)) << (C - 1u));
 		hA = h0 | ((rest >> NL) << 1);
 	}
+}
+
+// QFT targets 9..0 followed by the full bit reversal, out of place.  A tile
+// holds the ten target bits Lo and the top C bits h of the index (the
+// passengers: after the reversal they are the bottom C bits, one 128 B run
+// per output row); tile p has the middle bits F = rev_f(p), so its output
+// index is (rev10(Lo), p, rev_C(h)) and consecutive tiles write consecutive
+// runs.  Stage A lanes: h bit 0 on lane bit C-1, the low 10 - RB bits of Lo
+// (Lo bit 9-RB on lane bit C in complex128) and the other h bits on top;
+// each wavefront reads runs of 256 B or more (measured 7% faster than 1 KiB
+// runs with all of h on top).  Stage B lanes: rev_C(h) on the bottom C lane
+// bits, then Lo bit 0 (complex128) and Lo bits 9..5.  Persistent grid with
+// the next tile loaded straight into LDS during stage B and the stores, as
+// in k_qft_wide_block.  Prefetched tile layout: slot ra * 512 + t holds the
+// amplitude lane t takes into register ra, so each load instruction fetches
+// what one stage-A register load of the wavefront would (complex64: a
+// lane's 16 bytes are the amplitudes of lanes t and t + 1, t even, which are
+// adjacent in memory).  Measured at n = 30: 6.5 / 3.6 ms against 6.9 / 3.8
+// ms with one tile per workgroup and register loads.
+template <typename R>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
+k_qft_wide_tail(const WV<R> *__restrict__ a, WV<R> *__restrict__ o, WideTailParams P, const double2 *__restrict__ ptab,
+                const WP<R> *__restrict__ cst) {
+	using V = WV<R>;
+	using W = WideQft<R>;
+	constexpr unsigned C = W::C, RB = W::RB, DL = W::DL, NA = 1u << RB, CM = (1u << C) - 1u;
+	constexpr unsigned EPL = 16u / (unsigned)sizeof(V), NDMA = NA / EPL;
+	static_assert(NDMA * 512u * EPL == (1u << (10u + C)), "tile");
+	extern __shared__ char smem[];
+	V *tile = (V *)smem;
+	const unsigned t = threadIdx.x;
+	unsigned yA, hA;
+	wide_tail_lane<R>(t, yA, hA);
 	const unsigned g = t & CM, u = t >> C;
 	const unsigned dB = DL ? (u & 1u) : 0u, ub = u >> DL;
 	const unsigned hB = rev_bits(g, C);
@@ -778,17 +840,34 @@ k_qft_wide_tail(const WV<R> *__restrict__ a, WV<R> *__restrict__ o, WideTailPara
 	double2 lfd = make_double2(1.0, 0.0);
 	if(DL && (yA >> (9u - RB))) lfd = phase_of_range((u64)(yA & ((1u << (9u - RB)) - 1u)) << (54u + RB), ptab, 0, 1);
 	const unsigned n = P.n, f = P.f;
-	V v[NA];
-	for(u64 vt = blockIdx.x; vt < P.ntiles; vt += gridDim.x) {
+	const unsigned wv = __builtin_amdgcn_readfirstlane(t >> 6), ln = t & 63u;
+	auto request = [&](u64 vt) {
 		const u64 tid = xcd_tile(vt, P.ntiles, P.xcd);
-		{
-			const u64 F = f ? (u64)(__brev((unsigned)tid) >> (32u - f)) : 0ull;
-			const V *src = a + (((u64)hA << (n - C)) | (F << 10) | yA);
+		const u64 F = f ? (u64)(__brev((unsigned)tid) >> (32u - f)) : 0ull;
 #pragma unroll
-			for(unsigned ra = 0; ra < NA; ++ra) v[ra] = nt_load(src + ((u64)ra << (10u - RB)));
+		for(unsigned q = 0; q < NDMA; ++q) {
+			const unsigned s0 = (q * 8u + wv) * 64u * EPL, s = s0 + ln * EPL;
+			const unsigned ra = s >> 9, tt = s & 511u;
+			unsigned y, h;
+			wide_tail_lane<R>(tt, y, h);
+			const V *src = a + ((((u64)h << (n - C)) | (F << 10) | y) + ((u64)ra << (10u - RB)));
+			__builtin_amdgcn_global_load_lds((glb_void_t *)src, (lds_void_t *)(tile + s0), 16, 0, 2);
 		}
+	};
+	V v[NA];
+	u64 vt = blockIdx.x;
+	if(vt < P.ntiles) request(vt);
+	bool first = true;
+	for(; vt < P.ntiles; vt += gridDim.x) {
+		const u64 tid = xcd_tile(vt, P.ntiles, P.xcd);
+		if(first) wait_vmcnt<0>();
+		else wait_vmcnt<NA>();
+		first = false;
+		__syncthreads();  // every wavefront's part of the tile is in LDS
+#pragma unroll
+		for(unsigned ra = 0; ra < NA; ++ra) v[ra] = tile[ra * 512u + t];
 		wide_stage_a<R, false>(v, 10u, stepA, lfd, cst);
-		__syncthreads();
+		__syncthreads();  // every lane has read its stage-A amplitudes
 #pragma unroll
 		for(unsigned ra = 0; ra < NA; ++ra) {
 			const unsigned L = (ra << (10u - RB)) | yA;
@@ -800,6 +879,8 @@ k_qft_wide_tail(const WV<R> *__restrict__ a, WV<R> *__restrict__ o, WideTailPara
 			const unsigned L = (ub << (RB + DL)) | (rb << DL) | dB;
 			v[rb] = tile[(L << C) | (hB ^ (L & CM))];
 		}
+		__syncthreads();  // the tile is in registers: the LDS takes the next one
+		if(vt + gridDim.x < P.ntiles) request(vt + gridDim.x);
 		wide_stage_b<R, false>(v, 10u, dB, cst);
 		V *dst = o + ((tid << C) | g);
 #pragma unroll
@@ -813,6 +894,15 @@ k_qft_wide_tail(const WV<R> *__restrict__ a, WV<R> *__restrict__ o, WideTailPara
 
 unsigned qft_wide_min_qubits(int prec) { return 10u + (prec == 0 ? WideQft<double>::C : WideQft<float>::C); }
 
+// Grids of the ten-target passes, measured at n = 30 (HIP events; rocprof
+// agrees): blocks whose rows lie at least 2 MiB apart (pbot 20) are fastest
+// with 512-1024 workgroups (complex128 8.3 ms, complex64 4.1 ms; 8.5-8.8 /
+// 4.4-5.0 ms at 32768-65536); nearer rows (pbot 10) with two tiles per
+// workgroup in complex128 (6.1 ms; 6.3-8.1 ms at 512-32768) and four in
+// complex64 (3.05 ms; 4.0 ms at one tile each); the tail with four tiles per
+// workgroup (6.5 / 3.6 ms).
+static u64 wide_grid_floor(u64 ntiles) { return ntiles < 1024 ? ntiles : 1024; }
+
 template <typename R, bool PART>
 static hipError_t launch_qft_wide_block_t(const Launch &L, void *a, unsigned n, unsigned pbot, unsigned K, const double *dphase,
                                           const double *dcin, u64 grid_cap) {
@@ -824,9 +914,14 @@ static hipError_t launch_qft_wide_block_t(const Launch &L, void *a, unsigned n, 
 	P.ntiles = 1ull << (n - 10 - WideQft<R>::C);
 	P.xcd = P.ntiles >= 8 ? 1u : 0u;
 	P.scale = std::ldexp(K & 1u ? 1.0 / std::sqrt(2.0) : 1.0, -(int)(K / 2));  // (1/sqrt(2))^K, one rounding at most
-	const u64 cap = grid_cap ? grid_cap : P.ntiles;
+	u64 cap = grid_cap;
+	if(!cap) {
+		const bool far = ((u64)sizeof(V) << pbot) >= (2ull << 20);
+		const u64 per = sizeof(R) == 8 ? 2 : 4;  // tiles per workgroup when the rows are near
+		cap = far ? wide_grid_floor(P.ntiles) : std::max(wide_grid_floor(P.ntiles), P.ntiles / per);
+	}
 	const unsigned grid = (unsigned)(P.ntiles < cap ? P.ntiles : cap);
-	const size_t shmem = ((size_t)1 << (10 + WideQft<R>::C)) * sizeof(V);
+	const size_t shmem = ((size_t)1 << (10 + WideQft<R>::C)) * sizeof(V) + 5 * 256 * sizeof(double2);  // tile, phase tables
 	static bool attr = false;
 	if(!attr) {
 		(void)hipFuncSetAttribute((const void *)k_qft_wide_block<R, PART>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
@@ -858,7 +953,7 @@ static hipError_t launch_qft_wide_tail_t(const Launch &L, const void *src, void 
 	P.pad = 0;
 	P.ntiles = 1ull << P.f;
 	P.xcd = P.ntiles >= 8 ? 1u : 0u;
-	const u64 cap = grid_cap ? grid_cap : P.ntiles;
+	const u64 cap = grid_cap ? grid_cap : std::max(wide_grid_floor(P.ntiles), P.ntiles / 4);
 	const unsigned grid = (unsigned)(P.ntiles < cap ? P.ntiles : cap);
 	const size_t shmem = ((size_t)1 << (10 + WideQft<R>::C)) * sizeof(V);
 	static bool attr = false;
