@@ -368,6 +368,52 @@ int tmfqs_hip_batch_adjoint(tmfqs_hip_batch *b, const tmfqs_hip_diag_term *terms
                             size_t grad_count);
 int tmfqs_hip_batch_destroy(tmfqs_hip_batch *b);
 
+/* ---- chunked host-resident states --------------------------------------
+ * A register of 2^n amplitudes kept as 2^(n-c) chunks of 2^c amplitudes in
+ * pinned host memory, uncompressed, for states beyond one GPU's HBM: the
+ * reference's chunked storage (CompressedStateBackend,
+ * include/tmfqs/storage/common/compressed_state_backend.h:103-747, with its
+ * ChunkLayout, chunk_layout.h:22-90, and decompressed-chunk cache,
+ * chunk_cache.h:83-122) without the codecs.  Operation lists stream through
+ * device windows of 2^g chunks, each an engine handle of c + g qubits: the
+ * list is cut into segments whose gates mix at most g chunk bits (the index
+ * bits above the low c), and a segment visits every window holding those
+ * bits, two windows in flight on two streams.  A gate inside a chunk is the
+ * reference's IntraChunk pair mode, one across chunks its InterChunk mode
+ * (pair_kernel_executor.h:26-103).  Controls and phases on chunk bits
+ * outside a window are constant over it and become a condition or a phase
+ * per window.  Both diffusion kinds reduce the amplitude sum over every
+ * window, then update each.  Kinds: H, X, CX, CPHASE, PHASE, SWAP, MATRIX,
+ * PHASE_FLIP, DIFFUSE_*.  Device memory: two windows of 2^(c+g) amplitudes
+ * (and their handles' working buffers).  Results equal the device register's
+ * up to the rounding of phases and sums that a window regroups. */
+typedef struct tmfqs_hip_chunked tmfqs_hip_chunked;
+
+typedef struct tmfqs_hip_chunked_stats {
+	uint64_t segments;  /* window sweeps run (one per segment, two per diffusion) */
+	uint64_t windows;   /* window visits */
+	double bytes_h2d, bytes_d2h;
+} tmfqs_hip_chunked_stats;
+
+/* 1 <= chunk_qubits <= num_qubits <= 40; window_bits >= 1 unless
+ * chunk_qubits == num_qubits; chunk_qubits + window_bits <= num_qubits.
+ * The state starts as |0...0>. */
+int tmfqs_hip_chunked_create(const tmfqs_hip_config *cfg, unsigned num_qubits, unsigned chunk_qubits, unsigned window_bits,
+                             tmfqs_hip_chunked **out);
+int tmfqs_hip_chunked_init_basis(tmfqs_hip_chunked *c, uint64_t state, double re, double im);
+/* elem_count = 2 * 2^n interleaved doubles. */
+int tmfqs_hip_chunked_load(tmfqs_hip_chunked *c, const double *interleaved, uint64_t elem_count);
+/* States [first, first + count) as interleaved FP64. */
+int tmfqs_hip_chunked_export(tmfqs_hip_chunked *c, uint64_t first, uint64_t count, double *out);
+/* As tmfqs_hip_run; a record that mixes more than g chunk bits is refused
+ * (TMFQS_HIP_ERR_INVALID_ARGUMENT) before anything runs. */
+int tmfqs_hip_chunked_run(tmfqs_hip_chunked *c, const tmfqs_hip_op *ops, size_t count, const double *mat_pool,
+                          const unsigned *qubit_pool);
+/* Sum of |a|^2, window by window. */
+int tmfqs_hip_chunked_total_probability(tmfqs_hip_chunked *c, double *out);
+int tmfqs_hip_chunked_info(tmfqs_hip_chunked *c, tmfqs_hip_chunked_stats *out);
+int tmfqs_hip_chunked_destroy(tmfqs_hip_chunked *c);
+
 /* ---- host-side checks of the randomPhase arithmetic -------------------
  * No counterpart in the reference; for tests of the device path, no device
  * needed.  tmfqs_hip_mt_jump_outputs: the count outputs of std::mt19937(seed)

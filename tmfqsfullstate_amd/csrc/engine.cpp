@@ -2219,6 +2219,27 @@ ObsParams observable_params(unsigned n, const tmfqs_hip_diag_term *terms, size_t
 }
 void check_device(hipError_t e, const char *what) { hip_check(e, what); }
 int status_of_current_exception() { return map_exception(); }
+void *window_reset(tmfqs_hip_state *h, hipStream_t *stream) {
+	require_init(h, "chunk window");
+	HIPC(hipSetDevice(h->cfg.device));
+	flush(h);
+	h->queue.clear();
+	h->qmats.clear();
+	h->qscatter.clear();
+	h->ew_steps_pending = 0;
+	h->tracked.clear();
+	h->dirty.clear();
+	h->dirty_set.clear();
+	*stream = h->stream;
+	return h->amp;
+}
+void *window_flush(tmfqs_hip_state *h, hipStream_t *stream) {
+	require_init(h, "chunk window");
+	HIPC(hipSetDevice(h->cfg.device));
+	flush(h);
+	*stream = h->stream;
+	return h->amp;
+}
 }  // namespace tmfqs_hip
 
 // =========================================================================

@@ -205,6 +205,14 @@ void check_device(hipError_t e, const char *what);
 // Inside a catch block: records the message for tmfqs_hip_last_error and
 // returns the matching TMFQS_HIP_ERR_* status.
 int status_of_current_exception();
+// Device windows of chunked states (chunked.cpp) are ordinary handles whose
+// buffer the chunked layer fills and drains with copies on the handle's
+// stream.  window_reset issues any queued work and drops the handle's
+// host-side knowledge of its amplitudes (values cached for phase flips), as
+// before new contents arrive; window_flush issues every queued operation;
+// both return the device buffer (2^n amplitudes) and its stream.
+void *window_reset(tmfqs_hip_state *h, hipStream_t *stream);
+void *window_flush(tmfqs_hip_state *h, hipStream_t *stream);
 // Reductions write `blocks` partials into dpart, then reduce into dout
 // (sum: 2 doubles; norm: 1 double).
 constexpr unsigned kReduceBlocks = 1024;
