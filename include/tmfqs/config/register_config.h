@@ -15,7 +15,11 @@ enum class StorageStrategyKind {
 	Auto,
 	/** MI355X (gfx950) device state vector: the default strategy of this
 	 *  library and what Auto resolves to (see RegisterConfig). */
-	Hip
+	Hip,
+	/** Chunked register in pinned host memory run through device windows
+	 *  (HipConfig::chunkStates, HipConfig::windowChunks): the chunked
+	 *  storage of the Blosc and Zfp strategies without their codecs. */
+	HipChunked
 };
 
 enum class StorageWorkloadHint { Generic, Grover, Qft };
@@ -92,6 +96,13 @@ struct HipConfig {
 	 *  has room for a second state buffer; 1 keeps the in-place eight-target
 	 *  plan (four passes at 30 qubits, no second buffer). */
 	unsigned int qftPlan = 0;
+	/** HipChunked: amplitudes per host chunk, a power of two (0: 2^24, or the
+	 *  whole register when it is smaller), the counterpart of
+	 *  BloscConfig::chunkStates; and chunks per device window, a power of two
+	 *  >= 2 (0: 16, or all chunks when there are fewer), the counterpart of
+	 *  BloscConfig::gateCacheSlots. */
+	size_t chunkStates = 0;
+	size_t windowChunks = 0;
 };
 
 struct HipTuningOverrides {

@@ -10,6 +10,7 @@
 #include "tmfqs/storage/i_state_backend.h"
 
 struct tmfqs_hip_state;
+struct tmfqs_hip_config;
 
 namespace tmfqs {
 
@@ -21,6 +22,11 @@ std::unique_ptr<IStateBackend> createHipStateBackend(const RegisterConfig &cfg);
 tmfqs_hip_state *hipBackendHandle(IStateBackend *backend);
 /** Throws the exception a tmfqs_hip_* status stands for (nothing for TMFQS_HIP_OK). */
 void raiseHipStatus(int status);
+/** The C-ABI configuration of HipConfig's device settings. */
+tmfqs_hip_config toHipConfig(const HipConfig &c);
+/** A chunked register in host memory behind the same interface
+ *  (StorageStrategyKind::HipChunked), or nullptr when no HIP device is present. */
+std::unique_ptr<IStateBackend> createHipChunkedStateBackend(const RegisterConfig &cfg);
 
 }  // namespace tmfqs
 

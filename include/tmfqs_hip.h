@@ -411,6 +411,19 @@ int tmfqs_hip_chunked_run(tmfqs_hip_chunked *c, const tmfqs_hip_op *ops, size_t 
                           const unsigned *qubit_pool);
 /* Sum of |a|^2, window by window. */
 int tmfqs_hip_chunked_total_probability(tmfqs_hip_chunked *c, double *out);
+/* The register-level operations of IStateBackend on a chunked register
+ * (i_state_backend.h:37-120; the reference's CompressedStateBackend
+ * implements them over its chunk cache, compressed_state_backend.h:330-747):
+ * a deep copy; setAmplitude; applyAffineTransform (a -> s a + b);
+ * inversionAboutMean(mean) (a -> 2 mean - a); the amplitude sum (window
+ * sums added in window order); sampleMeasurement (the cumulative walk
+ * window by window in state order, each window's walk exact). */
+int tmfqs_hip_chunked_clone(tmfqs_hip_chunked *c, tmfqs_hip_chunked **out);
+int tmfqs_hip_chunked_set_amplitude(tmfqs_hip_chunked *c, uint64_t state, double re, double im);
+int tmfqs_hip_chunked_affine(tmfqs_hip_chunked *c, double s_re, double s_im, double b_re, double b_im);
+int tmfqs_hip_chunked_inversion_about_mean_value(tmfqs_hip_chunked *c, double mean_re, double mean_im);
+int tmfqs_hip_chunked_amplitude_sum(tmfqs_hip_chunked *c, double *re, double *im);
+int tmfqs_hip_chunked_sample(tmfqs_hip_chunked *c, double rnd, uint64_t *out);
 int tmfqs_hip_chunked_info(tmfqs_hip_chunked *c, tmfqs_hip_chunked_stats *out);
 int tmfqs_hip_chunked_destroy(tmfqs_hip_chunked *c);
 

@@ -33,6 +33,12 @@ int tmfqs_reg_create_basis(unsigned n, uint64_t state, const tmfqs_hip_config *c
 int tmfqs_reg_create_amplitudes(unsigned n, const double *amps, const tmfqs_hip_config *cfg, tmfqs_reg **out);
 /* QuantumState::randomPhase(n, seed) materialised on the device (quantum_state.cpp:264-277). */
 int tmfqs_reg_create_random_phase(unsigned n, unsigned seed, const tmfqs_hip_config *cfg, tmfqs_reg **out);
+/* QuantumRegister(n, state, cfg{strategy = HipChunked, hip.chunkStates =
+ * chunk_states, hip.windowChunks = window_chunks}): a register held as chunks
+ * in host memory (0: the defaults of HipConfig); the reference's chunked
+ * strategies are selected the same way (register_config.h, BloscConfig). */
+int tmfqs_reg_create_chunked(unsigned n, uint64_t state, const tmfqs_hip_config *cfg, uint64_t chunk_states,
+                             uint64_t window_chunks, tmfqs_reg **out);
 /* QuantumRegister copy constructor (deep copy). */
 int tmfqs_reg_clone(tmfqs_reg *r, tmfqs_reg **out);
 int tmfqs_reg_free(tmfqs_reg *r);

@@ -206,3 +206,55 @@ def test_errors():
         ops, cnt = S.make_ops([(T.OP_H, 10)])
         with pytest.raises(IndexError):
             s.run(ops, cnt)
+
+
+# ---- the C++ API on StorageStrategyKind::HipChunked (QuantumRegister, Simulator)
+
+@pytest.mark.parametrize("materialized", [True, False], ids=["materialized", "lazy"])
+def test_register_grover_through_simulator(materialized):
+    """Grover-12 on a chunked register (2^6-state chunks, windows of 4 chunks)
+    through Simulator::run: amplitudes, total probability and sampled indices
+    against the oracle register."""
+    n = 12
+    marked = [5, 3000]
+    ops, cnt = S.grover_ops(n, marked, materialized)
+    reg = S.Register.chunked(n, 0, chunk_states=1 << 6, window_chunks=4)
+    reg.run(ops, cnt)
+    arr = (O.Op * max(1, cnt))()
+    ctypes.memmove(arr, ops, ctypes.sizeof(O.Op) * cnt)
+    ref = O.run_circuit(n, (arr, cnt))
+    want = cplx(ref.amplitudes())
+    assert np.max(np.abs(reg.amplitudes() - want)) <= TOL
+    assert abs(reg.total_probability() - ref.total_probability()) <= 1e-12
+    for rnd in (0.0, 0.1, 0.49, 0.5, 0.93):
+        assert reg.measure_fixed(rnd) == ref.measure(rnd), rnd
+    reg.close()
+
+
+def test_register_qft_and_clone():
+    """QFT-14 of |10...0> through Simulator::run on a chunked register (2^8-state
+    chunks, windows of 8), and a deep copy taken before it that keeps its own
+    chunks and then gives the same result."""
+    n = 14
+    ops, cnt = S.qft_ops(n)
+    state = 1 << (n - 1)
+    init = np.zeros(2 << n)
+    init[2 * state] = 1.0
+    want = cplx(oracle_run(n, ops, cnt, init))
+    reg = S.Register.chunked(n, state, chunk_states=1 << 8, window_chunks=8)
+    twin = reg.clone()
+    reg.run(ops, cnt)
+    got = reg.amplitudes()
+    assert np.max(np.abs(got - want)) <= TOL
+    assert twin.amplitude(state) == 1.0
+    twin.run(ops, cnt)
+    assert np.array_equal(twin.amplitudes(), got)
+    twin.close()
+    reg.close()
+
+
+def test_register_chunked_config_errors():
+    with pytest.raises(ValueError):
+        S.Register.chunked(10, 0, chunk_states=100)  # not a power of two
+    with pytest.raises(ValueError):
+        S.Register.chunked(10, 0, chunk_states=1 << 4, window_chunks=3)

@@ -498,6 +498,110 @@ int tmfqs_hip_chunked_total_probability(tmfqs_hip_chunked *c, double *out) {
 	CHUNKED_END
 }
 
+int tmfqs_hip_chunked_clone(tmfqs_hip_chunked *c, tmfqs_hip_chunked **out) {
+	tmfqs_hip_chunked *d = nullptr;
+	try {
+		checked(c);
+		if(!out) throw std::invalid_argument("tmfqs_hip_chunked_clone: null output pointer");
+		*out = nullptr;
+		inner(tmfqs_hip_chunked_create(&c->cfg, c->n, c->c, c->g ? c->g : 1u, &d));
+		sync_windows(c);
+		std::memcpy(d->home, c->home, ((size_t)1 << c->n) * c->esz);
+		*out = d;
+		return TMFQS_HIP_OK;
+	} catch(const InnerStatus &e) {
+		destroy(d);
+		return e.rc;
+	} catch(...) {
+		const int rc = status_of_current_exception();
+		destroy(d);
+		return rc;
+	}
+}
+
+int tmfqs_hip_chunked_set_amplitude(tmfqs_hip_chunked *c, uint64_t state, double re, double im) {
+	CHUNKED_BEGIN
+	checked(c);
+	if(state >> c->n) throw std::out_of_range("HipChunkedState::setAmplitude state index out of range");
+	sync_windows(c);
+	if(c->esz == 16) {
+		((double *)c->home)[2 * state] = re;
+		((double *)c->home)[2 * state + 1] = im;
+	} else {
+		((float *)c->home)[2 * state] = (float)re;
+		((float *)c->home)[2 * state + 1] = (float)im;
+	}
+	CHUNKED_END
+}
+
+int tmfqs_hip_chunked_affine(tmfqs_hip_chunked *c, double s_re, double s_im, double b_re, double b_im) {
+	CHUNKED_BEGIN
+	checked(c);
+	check_device(hipSetDevice(c->cfg.device), "hipSetDevice");
+	sweep(c, {}, true, [&](const WindowMap &, u64, tmfqs_hip_state *h) { inner(tmfqs_hip_affine(h, s_re, s_im, b_re, b_im)); });
+	CHUNKED_END
+}
+
+int tmfqs_hip_chunked_inversion_about_mean_value(tmfqs_hip_chunked *c, double mean_re, double mean_im) {
+	CHUNKED_BEGIN
+	checked(c);
+	check_device(hipSetDevice(c->cfg.device), "hipSetDevice");
+	sweep(c, {}, true, [&](const WindowMap &, u64, tmfqs_hip_state *h) { inner(tmfqs_hip_inversion_about_mean_value(h, mean_re, mean_im)); });
+	CHUNKED_END
+}
+
+int tmfqs_hip_chunked_amplitude_sum(tmfqs_hip_chunked *c, double *re, double *im) {
+	CHUNKED_BEGIN
+	checked(c);
+	if(!re || !im) throw std::invalid_argument("tmfqs_hip_chunked_amplitude_sum: null output pointer");
+	check_device(hipSetDevice(c->cfg.device), "hipSetDevice");
+	double sr = 0.0, si = 0.0;
+	sweep(c, {}, false, [&](const WindowMap &, u64, tmfqs_hip_state *h) {
+		double wr, wi;
+		inner(tmfqs_hip_amplitude_sum(h, &wr, &wi));
+		sr += wr;
+		si += wi;
+	});
+	*re = sr;
+	*im = si;
+	CHUNKED_END
+}
+
+int tmfqs_hip_chunked_sample(tmfqs_hip_chunked *c, double rnd, uint64_t *out) {
+	CHUNKED_BEGIN
+	checked(c);
+	if(!out) throw std::invalid_argument("tmfqs_hip_chunked_sample: null output pointer");
+	if(rnd < 0.0 || rnd >= 1.0) throw std::invalid_argument("HipChunkedState::sampleMeasurement requires rnd in [0,1)");
+	check_device(hipSetDevice(c->cfg.device), "hipSetDevice");
+	// windows of the lowest chunk bits cover consecutive state ranges, so the
+	// cumulative walk runs window by window in state order
+	const WindowMap M(c, {});
+	const unsigned wbits = c->c + c->g;
+	double before = 0.0;
+	bool found = false;
+	for(u64 w = 0; w < M.windows() && !found; ++w) {
+		tmfqs_hip_state *h = c->win[0];
+		hipStream_t s;
+		void *dev = window_reset(h, &s);
+		move_window(c, M, w, dev, s, true);
+		++c->stats.windows;
+		double total = 0.0;
+		inner(tmfqs_hip_total_probability(h, &total));
+		if(rnd - before <= total) {
+			uint64_t idx = 0;
+			if(tmfqs_hip_sample(h, rnd - before, &idx) == TMFQS_HIP_OK) {
+				*out = (w << wbits) + idx;
+				found = true;
+			}
+		}
+		before += total;
+	}
+	++c->stats.segments;
+	sync_windows(c);
+	if(!found) throw std::runtime_error("HipChunkedState::sampleMeasurement cumulative probability did not reach sample");
+	CHUNKED_END
+}
+
 int tmfqs_hip_chunked_info(tmfqs_hip_chunked *c, tmfqs_hip_chunked_stats *out) {
 	CHUNKED_BEGIN
 	checked(c);

@@ -205,6 +205,17 @@ int tmfqs_reg_create_random_phase(unsigned n, unsigned seed, const tmfqs_hip_con
 	CAPI_END
 }
 
+int tmfqs_reg_create_chunked(unsigned n, uint64_t state, const tmfqs_hip_config *cfg, uint64_t chunk_states,
+                             uint64_t window_chunks, tmfqs_reg **out) {
+	CAPI_BEGIN
+	tmfqs::RegisterConfig rc = regConfig(cfg);
+	rc.strategy = tmfqs::StorageStrategyKind::HipChunked;
+	rc.hip.chunkStates = static_cast<size_t>(chunk_states);
+	rc.hip.windowChunks = static_cast<size_t>(window_chunks);
+	*out = new tmfqs_reg{tmfqs::QuantumRegister(n, state, rc)};
+	CAPI_END
+}
+
 int tmfqs_reg_clone(tmfqs_reg *r, tmfqs_reg **out) {
 	CAPI_BEGIN
 	if(!r) throw std::invalid_argument("null register");

@@ -29,10 +29,6 @@ void raiseHipStatus(int rc) {
 	}
 }
 
-namespace {
-
-void raise(int rc) { raiseHipStatus(rc); }
-
 tmfqs_hip_config toHipConfig(const HipConfig &c) {
 	tmfqs_hip_config h;
 	tmfqs_hip_config_default(&h);
@@ -48,6 +44,10 @@ tmfqs_hip_config toHipConfig(const HipConfig &c) {
 	h.qft_plan = static_cast<int>(c.qftPlan);
 	return h;
 }
+
+namespace {
+
+void raise(int rc) { raiseHipStatus(rc); }
 
 class HipStateBackend final : public IStateBackend {
 	public:

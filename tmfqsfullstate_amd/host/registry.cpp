@@ -26,7 +26,8 @@ StorageStrategyKind StorageStrategyRegistry::resolve(unsigned int numQubits, con
 
 bool StorageStrategyRegistry::isAvailable(StorageStrategyKind kind) {
 	switch(kind) {
-		case StorageStrategyKind::Hip: return isHipStateBackendAvailable();
+		case StorageStrategyKind::Hip:
+		case StorageStrategyKind::HipChunked: return isHipStateBackendAvailable();
 		case StorageStrategyKind::Auto: return true;
 		default: return false;
 	}
@@ -34,7 +35,10 @@ bool StorageStrategyRegistry::isAvailable(StorageStrategyKind kind) {
 
 std::vector<std::string> StorageStrategyRegistry::listAvailable() {
 	std::vector<std::string> names;
-	if(isHipStateBackendAvailable()) names.push_back("hip");
+	if(isHipStateBackendAvailable()) {
+		names.push_back("hip");
+		names.push_back("hip-chunked");
+	}
 	names.push_back("auto");
 	return names;
 }
@@ -46,6 +50,7 @@ std::string StorageStrategyRegistry::toString(StorageStrategyKind kind) {
 		case StorageStrategyKind::Zfp: return "zfp";
 		case StorageStrategyKind::Auto: return "auto";
 		case StorageStrategyKind::Hip: return "hip";
+		case StorageStrategyKind::HipChunked: return "hip-chunked";
 	}
 	return "unknown";
 }
@@ -61,6 +66,10 @@ BackendSelection StorageStrategyRegistry::createSelection(unsigned int numQubits
 		case StorageStrategyKind::Hip:
 			sel.backend = createHipStateBackend(sel.config);
 			if(!sel.backend) throw std::runtime_error("Hip backend requested but no HIP device is available");
+			break;
+		case StorageStrategyKind::HipChunked:
+			sel.backend = createHipChunkedStateBackend(sel.config);
+			if(!sel.backend) throw std::runtime_error("HipChunked backend requested but no HIP device is available");
 			break;
 		case StorageStrategyKind::Auto: throw std::runtime_error("StorageStrategyRegistry internal error: unresolved auto strategy");
 		default:

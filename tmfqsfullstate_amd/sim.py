@@ -28,6 +28,8 @@ def lib():
         L.tmfqs_reg_create_basis.argtypes = [ctypes.c_uint, ctypes.c_uint64, ctypes.POINTER(Config), ctypes.POINTER(_vp)]
         L.tmfqs_reg_create_amplitudes.argtypes = [ctypes.c_uint, _dp, ctypes.POINTER(Config), ctypes.POINTER(_vp)]
         L.tmfqs_reg_create_random_phase.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.POINTER(Config), ctypes.POINTER(_vp)]
+        L.tmfqs_reg_create_chunked.argtypes = [ctypes.c_uint, ctypes.c_uint64, ctypes.POINTER(Config), ctypes.c_uint64,
+                                               ctypes.c_uint64, ctypes.POINTER(_vp)]
         L.tmfqs_reg_clone.argtypes = [_vp, ctypes.POINTER(_vp)]
         L.tmfqs_reg_free.argtypes = [_vp]
         L.tmfqs_reg_run.argtypes = [_vp, ctypes.POINTER(Op), ctypes.c_size_t, _dp, ctypes.POINTER(ctypes.c_uint),
@@ -123,6 +125,15 @@ class Register:
         h = _vp()
         c = _cfg(precision, **cfg)
         check(lib().tmfqs_reg_create_amplitudes(n, a.ctypes.data_as(_dp), ctypes.byref(c), ctypes.byref(h)))
+        return cls(h, n)
+
+    @classmethod
+    def chunked(cls, n, state=0, chunk_states=0, window_chunks=0, precision=0, **cfg):
+        """A register on StorageStrategyKind::HipChunked: 2^n amplitudes in host
+        chunks of chunk_states, run through device windows of window_chunks."""
+        h = _vp()
+        c = _cfg(precision, **cfg)
+        check(lib().tmfqs_reg_create_chunked(n, state, ctypes.byref(c), chunk_states, window_chunks, ctypes.byref(h)))
         return cls(h, n)
 
     @classmethod
