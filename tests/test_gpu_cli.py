@@ -54,6 +54,23 @@ def test_grover_programs_measure_the_oracle_state(prog, materialized, n, marked)
         assert float(field(out, "Execution seconds")) > 0
 
 
+def test_programs_on_the_chunked_strategy():
+    """grover and qftG with --strategy hip-chunked (2^6-state host chunks,
+    windows of 4 chunks): the measured state equals the oracle's."""
+    n, marked = 10, [128, 512, 896]
+    for seed in (1, 7):
+        out = run("grover", n, ",".join(map(str, marked)), "--strategy", "hip-chunked", "--chunk-states", 64,
+                  "--cache-slots", 4, "--seed", seed, "--verbose")
+        assert field(out, "Resolved strategy") == "hip-chunked"
+        got = int(field(out, "Grover search result"))
+        assert got == oracle_measure(n, oracle.grover_ops(n, marked, True), seed)
+    out = run("qftG", 11, "--input-family", "random-phase", "--strategy", "hip-chunked", "--chunk-states", 128,
+              "--cache-slots", 2, "--seed", 3)
+    k = int(field(out, "Measured state (k)"))
+    assert k == oracle_measure(11, oracle.qft_ops(11), 3, init=oracle.random_phase(11, 123456))
+    run("grover", n, "128", "--strategy", "hip-chunked", "--chunk-states", 100, expect=2)
+
+
 def pattern_input(n):
     states = [8 * x + (x % 2) for x in range(1 << n) if 8 * x + (x % 2) < (1 << n)]
     a = np.zeros(2 << n)

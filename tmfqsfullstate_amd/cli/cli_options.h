@@ -6,7 +6,11 @@
 // (scripts/run_thesis_experiments.py in the reference) read these too.
 //
 // Options every driver takes:
-//   --strategy hip|auto|dense|blosc|zfp   (this library ships hip; auto resolves to it)
+//   --strategy hip|hip-chunked|auto|dense|blosc|zfp   (this library ships hip and
+//                                         hip-chunked; auto resolves to hip)
+//   --chunk-states N  --cache-slots N     (hip-chunked: amplitudes per host chunk and
+//                                         chunks per device window, powers of two; the
+//                                         reference's Blosc options of the same names)
 //   --precision complex128|complex64      (device storage precision)
 //   --device N  --tile-qubits N  --fuse-qubits K  --exact
 //   --seed S                              (measurement seed; default: std::random_device)
@@ -49,7 +53,8 @@ inline unsigned long long toNumber(const std::string &text, const char *what) {
 
 inline tmfqs::StorageStrategyKind strategyNamed(const std::string &name) {
 	for(tmfqs::StorageStrategyKind k : {tmfqs::StorageStrategyKind::Dense, tmfqs::StorageStrategyKind::Blosc, tmfqs::StorageStrategyKind::Zfp,
-	                                    tmfqs::StorageStrategyKind::Auto, tmfqs::StorageStrategyKind::Hip})
+	                                    tmfqs::StorageStrategyKind::Auto, tmfqs::StorageStrategyKind::Hip,
+	                                    tmfqs::StorageStrategyKind::HipChunked})
 		if(tmfqs::StorageStrategyRegistry::toString(k) == name) return k;
 	throw std::invalid_argument("Unknown strategy: " + name);
 }
@@ -72,6 +77,10 @@ inline bool takeCommon(int argc, char *argv[], int &i, Options &o) {
 			o.config.hip.precision = tmfqs::HipPrecision::Complex64;
 		else
 			throw std::invalid_argument("Unknown precision: " + p);
+	} else if(arg == "--chunk-states") {
+		o.config.hip.chunkStates = static_cast<size_t>(toNumber(value(), "--chunk-states"));
+	} else if(arg == "--cache-slots") {
+		o.config.hip.windowChunks = static_cast<size_t>(toNumber(value(), "--cache-slots"));
 	} else if(arg == "--device") {
 		o.config.hip.device = static_cast<int>(toNumber(value(), "--device"));
 	} else if(arg == "--tile-qubits") {
@@ -104,7 +113,8 @@ inline void printAvailableStrategies(std::ostream &os) {
 }
 
 inline const char *commonUsage() {
-	return "[--strategy hip|auto|dense|blosc|zfp] [--precision complex128|complex64] [--device N] [--tile-qubits N] "
+	return "[--strategy hip|hip-chunked|auto|dense|blosc|zfp] [--chunk-states N] [--cache-slots N] "
+	       "[--precision complex128|complex64] [--device N] [--tile-qubits N] "
 	       "[--fuse-qubits K] [--exact] [--seed S] [--verbose] [--report-json PATH]";
 }
 
