@@ -2,11 +2,13 @@
 and the device register.
 
 A chunked register streams its chunks through device windows that run the
-engine's own kernels; a window regroups the phases of controls outside it
-(a controlled phase becomes a phase gate, or a phase of the whole window)
-and the diffusion mean is summed window by window, so complex128 results
-agree with the oracle to max-abs 1e-12 (bit for bit when the whole register
-is one window); complex64: fidelity error <= 1e-6 against the oracle.
+engine's own kernels; a window rewrites the records that reach outside it (a
+controlled phase with a fixed control becomes a phase gate, one with both
+qubits fixed a phase of the whole window, each amplitude multiplied once as
+the reference does).  complex128: bit for bit with the oracle in the unfused
+and exact modes for every record kind but the diffusions, whose mean is
+summed window by window; max-abs 1e-12 in the fused mode (fused phase
+ladders) and with diffusions.  complex64: fidelity error <= 1e-6.
 """
 
 import ctypes
@@ -94,6 +96,20 @@ def test_random_circuits_vs_oracle(c, g, fusion):
     assert np.max(np.abs(got - cplx(want))) <= TOL, (c, g)
     if c < n:
         assert info["windows"] >= info["segments"] * (1 << (n - c - g))
+
+
+@pytest.mark.parametrize("c,g", SHAPES[:4], ids=[f"c{c}-g{g}" for c, g in SHAPES[:4]])
+@pytest.mark.parametrize("mode", [dict(fusion=0), dict(fusion=1, exact_phases=1)], ids=["unfused", "exact"])
+def test_random_circuits_bitexact_in_exact_modes(c, g, mode):
+    n = 13
+    print(f"chunked exact-mode circuit n={n} c={c} g={g} {mode}", flush=True)
+    ops, cnt, mats, mq = random_circuit(n, c, g, seed=c * 10 + g + 100)
+    init = O.random_phase(n, 3)
+    want = oracle_run(n, ops, cnt, init, mats, mq)
+    with ChunkedState(n, c, g, **mode) as s:
+        s.load(cplx(init))
+        s.run(ops, cnt, mats, mq)
+        assert np.array_equal(s.export().view(np.float64), want)
 
 
 def test_single_window_is_the_device_register_bit_for_bit():
