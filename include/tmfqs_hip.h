@@ -383,7 +383,8 @@ int tmfqs_hip_batch_destroy(tmfqs_hip_batch *b);
  * (pair_kernel_executor.h:26-103).  Controls and phases on chunk bits
  * outside a window are constant over it and become a condition or a phase
  * per window.  Both diffusion kinds reduce the amplitude sum over every
- * window, then update each.  Kinds: H, X, CX, CPHASE, PHASE, SWAP, MATRIX,
+ * window, then update each; a run of phase flips alone (Grover's oracle
+ * between two diffusions) negates the amplitudes where they are stored.  Kinds: H, X, CX, CPHASE, PHASE, SWAP, MATRIX,
  * PHASE_FLIP, DIFFUSE_*.  Device memory: two windows of 2^(c+g) amplitudes
  * (and their handles' working buffers).  Results equal the device register's
  * up to the rounding of phases and sums that a window regroups. */

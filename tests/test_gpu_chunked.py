@@ -148,9 +148,14 @@ def test_grover_vs_oracle(materialized):
     want = cplx(oracle_run(n, ops, cnt, None))
     with ChunkedState(n, 5, 2) as s:
         s.run(ops, cnt)
+        sweeps = s.info()["segments"]
         got = s.export()
         p = s.total_probability()
     assert np.max(np.abs(got - want)) <= TOL
+    # the Hadamard layer mixes the 5 chunk bits two at a time (3 sweeps, the
+    # first oracle riding along), then two sweeps per diffusion; the later
+    # oracles' flips negate amplitudes in host memory without a sweep
+    assert sweeps == 3 + 2 * O.grover_iterations(n, len(marked))
     assert abs(p - 1.0) <= 1e-12
     assert int(np.argmax(np.abs(got))) in marked
 

@@ -291,6 +291,31 @@ void diffuse(tmfqs_hip_chunked *C) {
 	sweep(C, {}, true, [&](const WindowMap &, u64, tmfqs_hip_state *h) { inner(tmfqs_hip_inversion_about_mean_value(h, mr, mi)); });
 }
 
+bool only_flips(const tmfqs_hip_op *ops, size_t lo, size_t hi) {
+	for(size_t i = lo; i < hi; ++i)
+		if(ops[i].kind != TMFQS_HIP_OP_PHASE_FLIP) return false;
+	return true;
+}
+
+// A segment of phase flips only (Grover's oracle between two diffusions)
+// negates single amplitudes: done where they are stored, exactly as a
+// window would, without a sweep over the register.
+void flip_at_home(tmfqs_hip_chunked *C, const tmfqs_hip_op *ops, size_t lo, size_t hi) {
+	sync_windows(C);
+	for(size_t i = lo; i < hi; ++i) {
+		const u64 s = ops[i].state;
+		if(C->esz == 16) {
+			double *a = (double *)C->home + 2 * s;
+			a[0] = -a[0];
+			a[1] = -a[1];
+		} else {
+			float *a = (float *)C->home + 2 * s;
+			a[0] = -a[0];
+			a[1] = -a[1];
+		}
+	}
+}
+
 void destroy(tmfqs_hip_chunked *C) {
 	if(!C) return;
 	for(tmfqs_hip_state *&h : C->win)
@@ -455,8 +480,14 @@ int tmfqs_hip_chunked_run(tmfqs_hip_chunked *c, const tmfqs_hip_op *ops, size_t 
 		}
 		if((k == TMFQS_HIP_OP_PHASE || k == TMFQS_HIP_OP_H || k == TMFQS_HIP_OP_X) && ops[i].q0 >= c->n)
 			throw std::out_of_range("chunked state: qubit index out of range");
-		if(k == TMFQS_HIP_OP_MATRIX && (ops[i].k < 1 || ops[i].k > 5 || !mat_pool))
-			throw std::invalid_argument("chunked state: MATRIX records take 1..5 qubits and a matrix pool");
+		if(k == TMFQS_HIP_OP_MATRIX) {
+			if(ops[i].k < 1 || ops[i].k > 5 || !mat_pool || !qubit_pool)
+				throw std::invalid_argument("chunked state: MATRIX records take 1..5 qubits and a matrix pool");
+			for(unsigned a = 0; a < ops[i].k; ++a)
+				for(unsigned b = a + 1; b < ops[i].k; ++b)
+					if(qubit_pool[ops[i].q0 + a] == qubit_pool[ops[i].q0 + b])
+						throw std::invalid_argument("chunked state: MATRIX record with repeated qubits");
+		}
 		mixing_chunk_bits(c, ops[i], qubit_pool, bits);
 		if(bits.size() > c->g)
 			throw std::invalid_argument("chunked state: a record mixes " + std::to_string(bits.size()) + " chunk bits, the window holds " +
@@ -477,6 +508,8 @@ int tmfqs_hip_chunked_run(tmfqs_hip_chunked *c, const tmfqs_hip_op *ops, size_t 
 	for(const Seg &s : segs) {
 		if(s.diffusion)
 			diffuse(c);
+		else if(only_flips(ops, s.lo, s.hi))
+			flip_at_home(c, ops, s.lo, s.hi);
 		else
 			run_segment(c, ops, s.lo, s.hi, s.need, mat_pool, qubit_pool);
 	}
