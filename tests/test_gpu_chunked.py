@@ -140,6 +140,33 @@ def test_qft_vs_oracle():
     assert segs >= 2  # the targets on chunk bits do not fit one window
 
 
+def test_qft_exact_mode_bitexact():
+    """QFT-14 through windows of 2^9 amplitudes in the exact mode: every
+    controlled phase its own product, as in the reference, bit for bit."""
+    n = 14
+    init = O.random_phase(n, 8)
+    ops, cnt = S.qft_ops(n)
+    want = oracle_run(n, ops, cnt, init)
+    with ChunkedState(n, 7, 2, exact_phases=1) as s:
+        s.load(cplx(init))
+        s.run(ops, cnt)
+        assert np.array_equal(s.export().view(np.float64), want)
+
+
+def test_grover_complex64():
+    n = 12
+    marked = [7, 2900, 4000]
+    ops, cnt = S.grover_ops(n, marked, True)
+    want = cplx(oracle_run(n, ops, cnt, None))
+    with ChunkedState(n, 6, 3, precision=T.C64) as s:
+        s.run(ops, cnt)
+        got = s.export()
+    fid = abs(np.vdot(want, got)) ** 2 / (np.vdot(want, want).real * np.vdot(got, got).real)
+    assert 1.0 - fid <= 1e-6
+    p_marked = float(np.sum(np.abs(got[marked]) ** 2))
+    assert abs(p_marked - float(np.sum(np.abs(want[marked]) ** 2))) <= 1e-5
+
+
 @pytest.mark.parametrize("materialized", [True, False], ids=["materialized", "lazy"])
 def test_grover_vs_oracle(materialized):
     n = 10
