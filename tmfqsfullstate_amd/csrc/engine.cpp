@@ -864,7 +864,8 @@ void run_permutation(tmfqs_hip_state *h, std::vector<unsigned> map, void *buf) {
 	bool invol = true;
 	for(unsigned b = 0; b < n; ++b)
 		if(pi[pi[b]] != b) invol = false;
-	const unsigned C = h->prec == 0 ? 4u : 5u;
+	// runs of 2^C amplitudes (512 B) and tiles of up to 2^(2C) per pass
+	const unsigned C = h->prec == 0 ? 5u : 6u;
 	if(invol && n >= 2 * C + 1) {
 		ProfScope ps(h, "k_involution", pass_bytes(h));
 		HIPC(launch_involution(h->L(), buf, n, pi.data(), C));

@@ -151,7 +151,11 @@ __global__ void __launch_bounds__(256) k_bitswap(typename V2T<R>::t *__restrict_
 // In-place bit permutation by an involution perm (perm[perm[b]] = b).  The
 // local set S = {0..C-1} U perm({0..C-1}) is closed under perm, so tile A
 // (fixed bits outside S) and tile perm(A) swap contents; each workgroup owns
-// one such pair and stages both tiles in LDS.
+// one such pair at a time and stages both tiles in LDS.  The low C bits give
+// runs of 2^C contiguous amplitudes; the index bits of the other local bits
+// and the permuted local index come from small LDS tables (S has at most C
+// bits above the low C), and every thread issues all of its loads of a pair
+// (up to U per tile) before it stores.
 struct InvParams {
 	u64 npairs_tiles;  // 2^(n - s)
 	uint32_t n, C, s, nh;
@@ -160,14 +164,33 @@ struct InvParams {
 	uint32_t lperm[16];  // perm restricted to S in local-bit numbering
 };
 
-template <typename R>
+template <typename R, unsigned U>
 __global__ void __launch_bounds__(256) k_involution(typename V2T<R>::t *__restrict__ a, InvParams P) {
 	using V = typename V2T<R>::t;
 	extern __shared__ char smem[];
-	const u64 nloc = 1ull << P.s;
+	const unsigned nloc = 1u << P.s, nhi = 1u << P.nh, nlo = 1u << P.C, lowmask = nlo - 1u;
 	V *ta = (V *)smem;
 	V *tb = ta + nloc;
-	const u64 lowmask = (1ull << P.C) - 1ull;
+	u64 *hitab = (u64 *)(tb + nloc);            // high local bits -> their index bits
+	unsigned *slo = (unsigned *)(hitab + nhi);  // low bits -> bits of the source's local index
+	unsigned *shi = slo + nlo;                  // high local bits -> bits of the source's local index
+	for(unsigned j = threadIdx.x; j < nhi; j += blockDim.x) {
+		u64 h = 0;
+		unsigned sh = 0;
+		for(unsigned b = 0; b < P.nh; ++b)
+			if((j >> b) & 1u) {
+				h |= 1ull << P.hbits[b];
+				sh |= 1u << P.lperm[P.C + b];
+			}
+		hitab[j] = h;
+		shi[j] = sh;
+	}
+	for(unsigned j = threadIdx.x; j < nlo; j += blockDim.x) {
+		unsigned sl = 0;
+		for(unsigned b = 0; b < P.C; ++b)
+			if((j >> b) & 1u) sl |= 1u << P.lperm[b];
+		slo[j] = sl;
+	}
 	for(u64 tid = blockIdx.x; tid < P.npairs_tiles; tid += gridDim.x) {
 		u64 base = tid << P.C;
 		for(unsigned b = 0; b < P.nh; ++b) base = ins0(base, P.hbits[b]);
@@ -176,28 +199,42 @@ __global__ void __launch_bounds__(256) k_involution(typename V2T<R>::t *__restri
 			if((base >> b) & 1ull) pb |= 1ull << P.perm[b];
 		if(pb < base) continue;  // uniform: the partner tile's workgroup handles the pair
 		const bool self = (pb == base);
-		__syncthreads();
-		for(u64 l = threadIdx.x; l < nloc; l += blockDim.x) {
-			u64 hi = 0;
-			for(unsigned b = 0; b < P.nh; ++b)
-				if((l >> (P.C + b)) & 1ull) hi |= 1ull << P.hbits[b];
-			const u64 off = (l & lowmask) | hi;
-			ta[l] = a[base | off];
-			if(!self) tb[l] = a[pb | off];
+		__syncthreads();  // the tables are built; the previous pair's LDS reads are done
+		// complex128 values held as native vectors (with the HIP vector type the
+		// compiler keeps these arrays in scratch)
+		using NV = typename std::conditional<sizeof(R) == 8, R __attribute__((ext_vector_type(2))), V>::type;
+		const NV *an = (const NV *)a;
+		NV xa[U], xb[U];
+#pragma unroll
+		for(unsigned u = 0; u < U; ++u) {
+			const unsigned l = threadIdx.x + u * blockDim.x;
+			if(l < nloc) {
+				const u64 off = (l & lowmask) | hitab[l >> P.C];
+				xa[u] = an[base | off];
+				if(!self) xb[u] = an[pb | off];
+			}
+		}
+#pragma unroll
+		for(unsigned u = 0; u < U; ++u) {
+			const unsigned l = threadIdx.x + u * blockDim.x;
+			if(l < nloc) {
+				ta[l] = V{xa[u].x, xa[u].y};
+				if(!self) tb[l] = V{xb[u].x, xb[u].y};
+			}
 		}
 		__syncthreads();
-		for(u64 l = threadIdx.x; l < nloc; l += blockDim.x) {
-			u64 hi = 0, src = 0;
-			for(unsigned b = 0; b < P.nh; ++b)
-				if((l >> (P.C + b)) & 1ull) hi |= 1ull << P.hbits[b];
-			for(unsigned b = 0; b < P.s; ++b)
-				if((l >> b) & 1ull) src |= 1ull << P.lperm[b];
-			const u64 off = (l & lowmask) | hi;
-			if(self) {
-				a[base | off] = ta[src];
-			} else {
-				a[pb | off] = ta[src];
-				a[base | off] = tb[src];
+#pragma unroll
+		for(unsigned u = 0; u < U; ++u) {
+			const unsigned l = threadIdx.x + u * blockDim.x;
+			if(l < nloc) {
+				const u64 off = (l & lowmask) | hitab[l >> P.C];
+				const unsigned src = slo[l & lowmask] | shi[l >> P.C];
+				if(self) {
+					a[base | off] = ta[src];
+				} else {
+					a[pb | off] = ta[src];
+					a[base | off] = tb[src];
+				}
 			}
 		}
 	}
@@ -746,12 +783,20 @@ hipError_t launch_involution(const Launch &L, void *a, unsigned n, const unsigne
 	}
 	P.npairs_tiles = 1ull << (n - s);
 	const size_t esz = L.prec == 0 ? 16 : 8;
-	const size_t shmem = 2 * esz * (1ull << s);
+	const unsigned umax = L.prec == 0 ? 4u : 16u;  // loads per thread and tile the kernel holds
+	if(P.nh > C || (1ull << s) > 256ull * umax) return hipErrorInvalidValue;
+	const size_t shmem = 2 * esz * (1ull << s) + (8 + 4) * (1ull << P.nh) + 4 * (1ull << C);
 	const unsigned grid = (unsigned)(P.npairs_tiles < 16384 ? P.npairs_tiles : 16384);
+	static bool attr = false;  // complex64 tiles of 2^12 amplitudes take 64 KiB plus the tables
+	if(!attr) {
+		(void)hipFuncSetAttribute((const void *)k_involution<double, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+		(void)hipFuncSetAttribute((const void *)k_involution<float, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+		attr = true;
+	}
 	if(L.prec == 0)
-		TMFQS_LAUNCH(k_involution<double>, grid, 256, shmem, L.stream, (double2 *)a, P);
+		TMFQS_LAUNCH((k_involution<double, 4>), grid, 256, shmem, L.stream, (double2 *)a, P);
 	else
-		TMFQS_LAUNCH(k_involution<float>, grid, 256, shmem, L.stream, (float2 *)a, P);
+		TMFQS_LAUNCH((k_involution<float, 16>), grid, 256, shmem, L.stream, (float2 *)a, P);
 	return hipGetLastError();
 }
 
