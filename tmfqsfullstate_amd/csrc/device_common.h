@@ -56,6 +56,44 @@ __device__ __forceinline__ void nt_store(float2 v, float2 *p) {
 	__builtin_nontemporal_store(w, (nt_f2 *)p);
 }
 
+// s_waitcnt vmcnt(N): at most N of the wave's vector-memory accesses stay
+// outstanding (gfx9 encoding: vmcnt bits 3:0 and 15:14; expcnt 6:4 and
+// lgkmcnt 11:8 at their maxima, not waited for).
+template <unsigned N> __device__ __forceinline__ void wait_vmcnt() {
+	static_assert(N == 0 || N == 16 || N == 32, "wait_vmcnt");
+	__builtin_amdgcn_s_waitcnt((int)((N & 15u) | ((N >> 4) << 14) | (7u << 4) | (15u << 8)));
+}
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+// A 16-byte LDS-direct load per lane (gfx950 global_load_lds_dwordx4,
+// streaming): lane l writes its 16 bytes at the LDS base (M0) + 16 l, and no
+// registers receive data.  Written as inline asm with no output registers:
+// the compiler does not track it, so it adds no vmcnt(0) of its own before
+// later LDS reads or barriers (for the builtin it must, as its wait model
+// lets loads and stores on the vector-memory counter complete out of order,
+// which would make every tile wait for the previous tile's stores).  The
+// hardware counts it with every other vector-memory access of the wave in
+// issue order; the caller waits for it with wait_vmcnt before its data is
+// read, and its address registers are read when it issues.  M0 is saved and
+// restored around it (the compiler reserves M0), with a wait state after each
+// write of M0.
+__device__ __forceinline__ void lds_direct_load16(const void *g, void *lds) {
+	const unsigned base = (unsigned)(size_t)(lds_void_t *)lds;
+	unsigned saved;
+	asm volatile(
+	    "s_mov_b32 %0, m0\n\t"
+	    "s_mov_b32 m0, %2\n\t"
+	    "s_nop 0\n\t"
+	    "global_load_lds_dwordx4 %1, off nt\n\t"
+	    "s_nop 0\n\t"
+	    "s_mov_b32 m0, %0"
+	    : "=&s"(saved)
+	    : "v"(g), "s"(base)
+	    : "memory");
+}
+
+
 // Streaming passes (copies, elementwise chains, per-operation sweeps) run on
 // up to 2^18 workgroups: a 2^30-amplitude copy measured 5.69 TB/s at that
 // grid against 5.10 TB/s at 2^14; per-operation Hadamard and X sweeps were

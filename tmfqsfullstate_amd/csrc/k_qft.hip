@@ -662,20 +662,6 @@ __device__ __forceinline__ u64 xcd_tile(u64 v, u64 ntiles, unsigned xcd) {
 	return xcd ? ((v & 7ull) * (ntiles >> 3) + (v >> 3)) : v;
 }
 
-// LDS-direct loads (gfx950 global_load_lds_dwordx4): lane l of a wavefront's
-// instruction writes its 16 bytes at the instruction's LDS base + 16 l; no
-// registers are involved, and the vector-memory counter covers them in issue
-// order with every other vector memory access of the wave.
-typedef __attribute__((address_space(3))) void lds_void_t;
-typedef __attribute__((address_space(1))) void glb_void_t;
-
-template <unsigned N> __device__ __forceinline__ void wait_vmcnt() {
-	static_assert(N == 0 || N == 16 || N == 32, "wait_vmcnt");
-	if constexpr(N == 0) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-	else if constexpr(N == 16) asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
-	else asm volatile("s_waitcnt vmcnt(32)" ::: "memory");
-}
-
 // In-place pass over the block bits pbot+9 .. pbot, whose top K bits are
 // targets (K = 10, or fewer in a leading pass); C passenger bits lo at the
 // bottom of the index (pbot >= C).  A 128 KiB LDS tile per workgroup (one
@@ -735,8 +721,7 @@ k_qft_wide_block(WV<R> *__restrict__ a, WideBlockParams P, const double2 *__rest
 		const V *src = a + ((((x >> pb) << (pb + 10)) | (x & lowmask)) + lane_off);
 #pragma unroll
 		for(unsigned i = 0; i < NDMA; ++i)
-			__builtin_amdgcn_global_load_lds((glb_void_t *)(src + ((u64)i << (pb + 6))), (lds_void_t *)(tile + (i * 8u + wv) * 64u * EPL),
-			                                 16, 0, 2);
+			lds_direct_load16(src + ((u64)i << (pb + 6)), tile + (i * 8u + wv) * 64u * EPL);
 	};
 	V v[NA];
 	u64 vt = blockIdx.x;
@@ -851,7 +836,7 @@ k_qft_wide_tail(const WV<R> *__restrict__ a, WV<R> *__restrict__ o, WideTailPara
 			unsigned y, h;
 			wide_tail_lane<R>(tt, y, h);
 			const V *src = a + ((((u64)h << (n - C)) | (F << 10) | y) + ((u64)ra << (10u - RB)));
-			__builtin_amdgcn_global_load_lds((glb_void_t *)src, (lds_void_t *)(tile + s0), 16, 0, 2);
+			lds_direct_load16(src, tile + s0);
 		}
 	};
 	V v[NA];
