@@ -196,13 +196,27 @@ __device__ __forceinline__ void tile_hitab(u64 *hitab, const TilePass &P) {
 // gates costs one LDS round trip and one barrier instead of one per gate.
 template <typename V> __device__ __forceinline__ bool is_zero(V x) { return x.x == 0 && x.y == 0; }
 
-template <typename V, typename R, int W, int J>
-__device__ __forceinline__ void reg_h(V *v, R hc) {
+// The pair ops skip all-zero pairs as the reference does
+// (dense_state_backend.cpp:263, gate_block_apply.h:117-143).  When no active
+// lane of the wavefront holds such a pair, the unchecked form runs: the same
+// arithmetic for every pair, in one block the compiler schedules across pairs
+// (the checked form's per-pair branches serialise them).
+template <typename V, int W, int J>
+__device__ __forceinline__ bool any_zero_pair(const V *v) {
+	bool z = false;
+#pragma unroll
+	for(int r = 0; r < (1 << W); ++r)
+		if(!(r & (1 << J))) z = z | (is_zero(v[r]) & is_zero(v[r | (1 << J)]));
+	return z;
+}
+
+template <typename V, typename R, int W, int J, bool CHECKED>
+__device__ __forceinline__ void reg_h_pairs(V *v, R hc) {
 #pragma unroll
 	for(int r = 0; r < (1 << W); ++r) {
 		if(r & (1 << J)) continue;
 		const V x = v[r], y = v[r | (1 << J)];
-		if(is_zero(x) && is_zero(y)) continue;  // dense_state_backend.cpp:263 skips all-zero pairs
+		if(CHECKED && is_zero(x) && is_zero(y)) continue;
 		V o0, o1;
 		o0.x = (x.x + y.x) * hc;
 		o0.y = (x.y + y.y) * hc;
@@ -211,6 +225,12 @@ __device__ __forceinline__ void reg_h(V *v, R hc) {
 		v[r] = o0;
 		v[r | (1 << J)] = o1;
 	}
+}
+
+template <typename V, typename R, int W, int J>
+__device__ __forceinline__ void reg_h(V *v, R hc) {
+	if(__any(any_zero_pair<V, W, J>(v))) reg_h_pairs<V, R, W, J, true>(v, hc);
+	else reg_h_pairs<V, R, W, J, false>(v, hc);
 }
 
 template <typename V, int W, int J>
@@ -226,15 +246,13 @@ __device__ __forceinline__ void reg_x(V *v, const unsigned *lidx, u64 lcond) {
 	}
 }
 
-template <typename V, typename R, int W, int J>
-__device__ __forceinline__ void reg_u1(V *v, const double *g) {
-	const R g00r = (R)g[0], g00i = (R)g[1], g01r = (R)g[2], g01i = (R)g[3];
-	const R g10r = (R)g[4], g10i = (R)g[5], g11r = (R)g[6], g11i = (R)g[7];
+template <typename V, typename R, int W, int J, bool CHECKED>
+__device__ __forceinline__ void reg_u1_pairs(V *v, R g00r, R g00i, R g01r, R g01i, R g10r, R g10i, R g11r, R g11i) {
 #pragma unroll
 	for(int r = 0; r < (1 << W); ++r) {
 		if(r & (1 << J)) continue;
 		const V x = v[r], y = v[r | (1 << J)];
-		if(is_zero(x) && is_zero(y)) continue;
+		if(CHECKED && is_zero(x) && is_zero(y)) continue;
 		V o0, o1;
 		o0.x = g00r * x.x - g00i * x.y + g01r * y.x - g01i * y.y;
 		o0.y = g00r * x.y + g00i * x.x + g01r * y.y + g01i * y.x;
@@ -243,6 +261,14 @@ __device__ __forceinline__ void reg_u1(V *v, const double *g) {
 		v[r] = o0;
 		v[r | (1 << J)] = o1;
 	}
+}
+
+template <typename V, typename R, int W, int J>
+__device__ __forceinline__ void reg_u1(V *v, const double *g) {
+	const R g00r = (R)g[0], g00i = (R)g[1], g01r = (R)g[2], g01i = (R)g[3];
+	const R g10r = (R)g[4], g10i = (R)g[5], g11r = (R)g[6], g11i = (R)g[7];
+	if(__any(any_zero_pair<V, W, J>(v))) reg_u1_pairs<V, R, W, J, true>(v, g00r, g00i, g01r, g01i, g10r, g10i, g11r, g11i);
+	else reg_u1_pairs<V, R, W, J, false>(v, g00r, g00i, g01r, g01i, g10r, g10i, g11r, g11i);
 }
 
 // Dispatch on the runtime stage position sj (< W).
@@ -280,16 +306,17 @@ __device__ __forceinline__ void reg_diag(V *v, const TileOp &op, const unsigned 
                                          u64 lowmask, unsigned C, const u64 *hitab, const TwiddleTerm *__restrict__ terms,
                                          const double2 *__restrict__ ptab) {
 	if(op.kind == TOP_CP) {
+		// states with every condition bit set, zero amplitudes untouched
+		// (selects rather than per-amplitude branches)
 		const R cr = (R)op.cr, ci = (R)op.ci;
 #pragma unroll
 		for(int r = 0; r < (1 << W); ++r) {
-			if((lidx[r] & op.lcond) != op.lcond) continue;
 			const V x = v[r];
-			if(is_zero(x)) continue;
-			V o;
-			o.x = cr * x.x - ci * x.y;
-			o.y = cr * x.y + ci * x.x;
-			v[r] = o;
+			const bool act = ((lidx[r] & op.lcond) == op.lcond) & !is_zero(x);
+			const R ox = cr * x.x - ci * x.y;
+			const R oy = cr * x.y + ci * x.x;
+			v[r].x = act ? ox : x.x;
+			v[r].y = act ? oy : x.y;
 		}
 	} else if(op.kind == TOP_LADDER) {
 		// F is linear in the stage bits (disjoint bits of g & amask), so the
@@ -587,12 +614,15 @@ __device__ __forceinline__ void staged_tile(typename V2T<R>::t *__restrict__ a, 
 				if((tbase & op.gcond) != op.gcond) continue;  // uniform across the block
 				if(op.kind == TOP_GRAD) {
 					if(DUAL) {
+						// acc never holds -0, so adding +0 for the states
+						// outside the condition leaves it unchanged
 						double acc = 0.0;
 						if(act) {
 #pragma unroll
-							for(int r = 0; r < NR; ++r)
-								if((lidx[r] & op.lcond) == op.lcond)
-									acc += (double)vb[r].x * (double)va[r].y - (double)vb[r].y * (double)va[r].x;
+							for(int r = 0; r < NR; ++r) {
+								const double t = (double)vb[r].x * (double)va[r].y - (double)vb[r].y * (double)va[r].x;
+								acc += ((lidx[r] & op.lcond) == op.lcond) ? t : 0.0;
+							}
 						}
 						acc = wave_sum(acc);
 						if(lane == 0) gacc[wave * kMaxGradSlots + op.mat] += acc;
