@@ -306,17 +306,19 @@ __device__ __forceinline__ void reg_diag(V *v, const TileOp &op, const unsigned 
                                          u64 lowmask, unsigned C, const u64 *hitab, const TwiddleTerm *__restrict__ terms,
                                          const double2 *__restrict__ ptab) {
 	if(op.kind == TOP_CP) {
-		// states with every condition bit set, zero amplitudes untouched
-		// (selects rather than per-amplitude branches)
+		// states with every condition bit set, zero amplitudes untouched (a
+		// branch per amplitude: where the condition bits are stage bits the
+		// whole wavefront skips the states outside it)
 		const R cr = (R)op.cr, ci = (R)op.ci;
 #pragma unroll
 		for(int r = 0; r < (1 << W); ++r) {
+			if((lidx[r] & op.lcond) != op.lcond) continue;
 			const V x = v[r];
-			const bool act = ((lidx[r] & op.lcond) == op.lcond) & !is_zero(x);
-			const R ox = cr * x.x - ci * x.y;
-			const R oy = cr * x.y + ci * x.x;
-			v[r].x = act ? ox : x.x;
-			v[r].y = act ? oy : x.y;
+			if(is_zero(x)) continue;
+			V o;
+			o.x = cr * x.x - ci * x.y;
+			o.y = cr * x.y + ci * x.x;
+			v[r] = o;
 		}
 	} else if(op.kind == TOP_LADDER) {
 		// F is linear in the stage bits (disjoint bits of g & amask), so the
@@ -614,15 +616,12 @@ __device__ __forceinline__ void staged_tile(typename V2T<R>::t *__restrict__ a, 
 				if((tbase & op.gcond) != op.gcond) continue;  // uniform across the block
 				if(op.kind == TOP_GRAD) {
 					if(DUAL) {
-						// acc never holds -0, so adding +0 for the states
-						// outside the condition leaves it unchanged
 						double acc = 0.0;
 						if(act) {
 #pragma unroll
-							for(int r = 0; r < NR; ++r) {
-								const double t = (double)vb[r].x * (double)va[r].y - (double)vb[r].y * (double)va[r].x;
-								acc += ((lidx[r] & op.lcond) == op.lcond) ? t : 0.0;
-							}
+							for(int r = 0; r < NR; ++r)
+								if((lidx[r] & op.lcond) == op.lcond)
+									acc += (double)vb[r].x * (double)va[r].y - (double)vb[r].y * (double)va[r].x;
 						}
 						acc = wave_sum(acc);
 						if(lane == 0) gacc[wave * kMaxGradSlots + op.mat] += acc;
