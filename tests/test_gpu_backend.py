@@ -86,6 +86,43 @@ def test_random_circuits(mode):
         assert np.array_equal(got, want), (mode, n)
 
 
+@pytest.mark.parametrize("precision", [T.C128, T.C64], ids=["c128", "c64"])
+@pytest.mark.parametrize("n", [11, 13, 16])
+def test_relabeling_passes(precision, n):
+    # A gate run ending in swaps leaves its data under a bit relabeling that
+    # one pass puts back: k_involution when the relabeling is its own inverse
+    # (disjoint swaps; complex128 from 11 qubits, complex64 from 13) and
+    # k_bitswap passes otherwise (a 3-cycle).  Data moves only, so the fused
+    # run equals the oracle bit for bit in complex128 and the unfused
+    # (one sweep per swap) run in complex64.
+    rng = np.random.default_rng(100 + n)
+    init = rand_state(n, n + 1)
+    for trial in range(4):
+        q = [int(x) for x in rng.permutation(n)]
+        seq = [(O.OP_H, a, 0, 0.0) for a in range(n)]
+        if trial < 3:
+            pairs = 1 + 2 * trial if trial < 2 else n // 2
+            seq += [(O.OP_SWAP, q[2 * i], q[2 * i + 1], 0.0) for i in range(pairs)]
+        else:
+            seq += [(O.OP_SWAP, q[0], q[1], 0.0), (O.OP_SWAP, q[1], q[2], 0.0)]
+        s = st(precision, fusion=1).load(n, init)
+        s.profile(True)
+        s.run_gates(seq)
+        got = s.export()  # runs the queued gates
+        prof = s.profile_read()
+        if trial < 3 and n >= (11 if precision == T.C128 else 13):
+            assert prof["k_involution"]["launches"] == 1, (n, trial, prof)
+        if trial == 3:
+            assert prof["k_bitswap"]["launches"] >= 2, (n, prof)
+        if precision == T.C128:
+            want = O.run_circuit(n, seq, init=init).amplitudes()
+        else:
+            u = st(precision, fusion=0).load(n, init)
+            u.run_gates(seq)
+            want = u.export()
+        assert np.array_equal(got, want), (n, trial)
+
+
 QFT_MODES = MODES + [dict(fusion=1, qft_plan=1)]
 
 
