@@ -107,6 +107,28 @@ __device__ __forceinline__ double wave_sum(double v) {
 	return v;
 }
 
+// The same sum through DPP lane moves (no LDS traffic): prefix sums within
+// each row of 16 lanes (row_shr 1, 2, 4, 8), then the row totals carried up
+// by row_bcast 15 and 31; lane 63 holds the total, which every lane gets.  A
+// different fixed order than wave_sum (deterministic, not bit-identical).
+template <int CTRL, int ROWS> __device__ __forceinline__ double dpp_f64(double v) {
+	const long long b = __double_as_longlong(v);
+	const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWS, 0xf, true);
+	const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, true);
+	return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+	v += dpp_f64<0x111, 0xf>(v);  // row_shr:1
+	v += dpp_f64<0x112, 0xf>(v);  // row_shr:2
+	v += dpp_f64<0x114, 0xf>(v);  // row_shr:4
+	v += dpp_f64<0x118, 0xf>(v);  // row_shr:8
+	v += dpp_f64<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+	v += dpp_f64<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+	const long long b = __double_as_longlong(v);
+	const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+	return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 __device__ __forceinline__ long long wave_sum_ll(long long v) {
 	for(int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
 	return v;

@@ -623,7 +623,7 @@ __device__ __forceinline__ void staged_tile(typename V2T<R>::t *__restrict__ a, 
 								if((lidx[r] & op.lcond) == op.lcond)
 									acc += (double)vb[r].x * (double)va[r].y - (double)vb[r].y * (double)va[r].x;
 						}
-						acc = wave_sum(acc);
+						acc = wave_sum_dpp(acc);  // one per gradient op and tile: DPP, not LDS permutes
 						if(lane == 0) gacc[wave * kMaxGradSlots + op.mat] += acc;
 					}
 					continue;
