@@ -970,6 +970,7 @@ struct PlanCfg {
 	unsigned T = 12, C = 4, qft_c = 5;
 	unsigned W = 0;  // stage width of tile passes; 0 picks tile_stage_width(prec, T, dual)
 	bool mfma_blocks = true;  // complex128 five-qubit blocks on FP64 MFMA (not the reference's accumulation order)
+	bool phase_runs = false;  // dual passes: a run of controlled phases as one phase per amplitude (not bit-identical)
 	unsigned wide_qft_min = 0;  // complete QFTs of at least this many qubits take the ten-target plan (0: never)
 };
 
@@ -1203,6 +1204,7 @@ GatePlan plan_gates(const PlanCfg &cfg, unsigned n, const std::vector<QOp> &in, 
 						t.kind = TOP_CP;
 						add_cond(t, op.b0);
 						add_cond(t, op.b1);
+						t.amask = (1ull << op.b0) | (1ull << op.b1);  // its global bits (phase runs)
 						t.cr = op.cr;
 						t.ci = op.ci;
 						tops.push_back(t);
@@ -1213,6 +1215,7 @@ GatePlan plan_gates(const PlanCfg &cfg, unsigned n, const std::vector<QOp> &in, 
 				case Q_PH:
 					t.kind = TOP_CP;
 					add_cond(t, op.b0);
+					t.amask = 1ull << op.b0;
 					t.cr = op.cr;
 					t.ci = op.ci;
 					tops.push_back(t);
@@ -1230,6 +1233,54 @@ GatePlan plan_gates(const PlanCfg &cfg, unsigned n, const std::vector<QOp> &in, 
 					break;
 				default: ++k; break;
 			}
+		}
+		// Phase runs (dual passes): a run of controlled phases and gradient
+		// reductions becomes the reductions followed by one TOP_PHASES op per
+		// kMaxPhaseTerms phases.  Diagonal gates commute with the reductions'
+		// diagonal projectors and are applied to both states, so each
+		// reduction's value does not depend on where in the run it is taken;
+		// the amplitudes get one rounding of e^{i sum theta} instead of one
+		// per gate.
+		std::vector<TwiddleTerm> pterms;
+		if(cfg.phase_runs && dual) {
+			std::vector<TileOp> out;
+			out.reserve(tops.size());
+			for(size_t a = 0; a < tops.size();) {
+				if(tops[a].kind != TOP_CP && tops[a].kind != TOP_GRAD) {
+					out.push_back(tops[a++]);
+					continue;
+				}
+				size_t e = a;
+				unsigned ncp = 0;
+				for(; e < tops.size() && (tops[e].kind == TOP_CP || tops[e].kind == TOP_GRAD); ++e) ncp += tops[e].kind == TOP_CP;
+				if(ncp < 2) {
+					out.insert(out.end(), tops.begin() + a, tops.begin() + e);
+					a = e;
+					continue;
+				}
+				for(size_t q = a; q < e; ++q)
+					if(tops[q].kind == TOP_GRAD) out.push_back(tops[q]);
+				TileOp ph{};
+				for(size_t q = a; q < e; ++q) {
+					if(tops[q].kind != TOP_CP) continue;
+					if(ph.nterms == 0) {
+						ph = TileOp{};
+						ph.kind = TOP_PHASES;
+						ph.twid = (uint32_t)pterms.size();
+					}
+					const double th = std::atan2(tops[q].ci, tops[q].cr);
+					u64 bits;
+					std::memcpy(&bits, &th, sizeof bits);
+					pterms.push_back(TwiddleTerm{tops[q].amask, bits, 0, 0});
+					if(++ph.nterms == kMaxPhaseTerms) {
+						out.push_back(ph);
+						ph.nterms = 0;
+					}
+				}
+				if(ph.nterms) out.push_back(ph);
+				a = e;
+			}
+			tops.swap(out);
 		}
 		// deferred ladder factors: one twiddle per kMaxTwiddleTerms terms, at the end
 		std::vector<TwiddleTerm> allterms;
@@ -1251,6 +1302,12 @@ GatePlan plan_gates(const PlanCfg &cfg, unsigned n, const std::vector<QOp> &in, 
 			const unsigned hi = (fmax >> 64) ? 63u : 63u - (unsigned)__builtin_clzll((u64)fmax);
 			phase_chunks(lo, hi, t.cj0, t.cj1);
 			tops.push_back(t);
+		}
+		if(!pterms.empty()) {
+			const uint32_t off = (uint32_t)allterms.size();
+			allterms.insert(allterms.end(), pterms.begin(), pterms.end());
+			for(TileOp &o : tops)
+				if(o.kind == TOP_PHASES) o.twid += off;
 		}
 		if(tops.empty()) continue;
 		TilePass tp{};
@@ -1546,6 +1603,9 @@ void exec_planned(tmfqs_hip_state *h, const std::vector<QOp> &in, size_t lo, siz
 	cfg.dual = dual;
 	cfg.ladders = !h->cfg.exact_phases;
 	cfg.mfma_blocks = !h->cfg.exact_phases;
+	// complex128 only: for complex64 the FP64 angle sum and sincos cost more
+	// than the FP32 phase products they replace (measured 10.2 -> 11.7 ms)
+	cfg.phase_runs = dual && !h->cfg.exact_phases && h->prec == 0;
 	cfg.T = tile_bits(h);
 	if(tgt && tgt->tile_delta) cfg.T = (unsigned)std::max<int>(1, std::min<int>((int)n, (int)cfg.T + tgt->tile_delta));
 	cfg.C = coalesce_bits(h, cfg.T);

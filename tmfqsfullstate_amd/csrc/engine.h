@@ -30,7 +30,12 @@ enum TileOpKind : uint32_t {
 	TOP_LADDER = 5,   // multiply by e^{2 pi i F / 2^64}, F = (g & amask) << shift, where the condition holds
 	TOP_TWIDDLE = 6,  // multiply by e^{2 pi i F / 2^64}, F = sum over terms whose bit is set of (g & mask) << shift
 	TOP_GRAD = 7,     // dual passes only: accumulate Im sum conj(b) a where the condition holds into slot `mat`
+	TOP_PHASES = 8,   // multiply by e^{i theta}, theta = sum of the terms [twid, twid + nterms) whose global
+	                  // bits `cond` are all set, each term's angle the double stored in `mask`
 };
+
+// Most angles one TOP_PHASES op sums.
+constexpr unsigned kMaxPhaseTerms = 64;
 
 // Gradient slots one dual-pass launch can accumulate.
 constexpr unsigned kMaxGradSlots = 64;

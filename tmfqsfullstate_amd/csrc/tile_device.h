@@ -629,6 +629,24 @@ __device__ __forceinline__ void staged_tile(typename V2T<R>::t *__restrict__ a, 
 					continue;
 				}
 				if(!act) continue;
+				if constexpr(DUAL)  // the planner emits phase runs for dual passes only
+					if(op.kind == TOP_PHASES) {
+					// one angle sum and one sincos per amplitude, shared by both states
+#pragma unroll
+					for(int r = 0; r < NR; ++r) {
+						const u64 g = tbase | (lidx[r] & lowmask) | hitab[lidx[r] >> C];
+						double th = 0.0;  // never -0, so adding +0 leaves it unchanged
+						for(unsigned t = 0; t < op.nterms; ++t) {
+							const TwiddleTerm w = terms[op.twid + t];
+							th += ((g & w.cond) == w.cond) ? __longlong_as_double((long long)w.mask) : 0.0;
+						}
+						double2 f;
+						sincos(th, &f.y, &f.x);
+						va[r] = rot<V>(va[r], f);
+						vb[r] = rot<V>(vb[r], f);
+					}
+					continue;
+					}
 				if(op.kind == TOP_H || op.kind == TOP_X || op.kind == TOP_U1) {
 					reg_nondiag<V, R, W>(va, op, lidx, mats, hc);
 					if(DUAL) reg_nondiag<V, R, W>(vb, op, lidx, mats, hc);
