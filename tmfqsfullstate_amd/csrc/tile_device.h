@@ -629,7 +629,7 @@ __device__ __forceinline__ void staged_tile(typename V2T<R>::t *__restrict__ a, 
 					continue;
 				}
 				if(!act) continue;
-				if constexpr(DUAL)  // the planner emits phase runs for dual passes only
+				if constexpr(DUAL && sizeof(R) == 8)  // the planner emits phase runs for complex128 dual passes only
 					if(op.kind == TOP_PHASES) {
 					// one angle sum and one sincos per amplitude, shared by both states
 #pragma unroll
