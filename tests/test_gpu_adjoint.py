@@ -41,6 +41,34 @@ def test_ring_vs_oracle_parameter_shift(n, layers):
     assert np.max(np.abs(g - ps)) <= 1e-10
 
 
+def test_long_phase_runs_vs_oracle():
+    # 150 controlled phases and phase gates between two Hadamard layers: a
+    # complex128 dual pass takes such a run as its gradient reductions
+    # followed by summed-angle ops of at most 64 angles (here three, and the
+    # reductions span three launches of at most 64 slots).  Gradients against
+    # the oracle's parameter shift, the final state against the oracle.
+    n = 12
+    rng = np.random.default_rng(5)
+    seq = [(O.OP_H, q) for q in range(n)]
+    for i in range(150):
+        if i % 5 == 4:
+            seq.append((O.OP_PHASE, int(rng.integers(n)), 0, float(rng.uniform(0, 2 * np.pi))))
+        else:
+            a, b = rng.choice(n, 2, replace=False)
+            seq.append((O.OP_CPHASE, int(a), int(b), float(rng.uniform(0, 2 * np.pi))))
+    seq += [(O.OP_H, q) for q in range(n)]
+    ops, cnt = O.make_ops(seq)
+    print("phase runs n=12", flush=True)
+    e, g, final = gpu_adjoint(n, ops, cnt, Z0_PLUS_P0)
+    ref_state = O.run_circuit(n, (ops, cnt)).amplitudes()
+    assert np.max(np.abs(final - ref_state)) <= 1e-12
+    assert abs(e - O.expectation(ref_state, n, Z0_PLUS_P0)) <= 1e-12
+    ps = O.param_shift_grad(n, ops, cnt, Z0_PLUS_P0)
+    assert g.shape == ps.shape == (150,)
+    assert np.max(np.abs(ps)) > 1e-3
+    assert np.max(np.abs(g - ps)) <= 1e-10
+
+
 def test_mixed_circuit_vs_oracle():
     n = 7
     ops, cnt, mats, qubits = mixed_circuit(n, 21)
