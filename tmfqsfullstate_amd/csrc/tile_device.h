@@ -611,17 +611,36 @@ __device__ __forceinline__ void staged_tile(typename V2T<R>::t *__restrict__ a, 
 					if(DUAL) vb[r] = tb[pl];
 				}
 			}
+			// complex128: Im(conj(b) a) of each register amplitude, shared by
+			// consecutive gradient ops (a run's reductions come together) until
+			// an op changes the amplitudes (complex64 recomputes it per op: the
+			// cache would cost its three-bit dual build an occupancy step)
+			constexpr bool GCACHE = DUAL && sizeof(R) == 8;
+			double gt[GCACHE ? NR : 1];
+			bool gvalid = false;
 			for(unsigned o = 0; o < st.nops; ++o) {
 				const TileOp op = ops[st.op0 + o];
 				if((tbase & op.gcond) != op.gcond) continue;  // uniform across the block
 				if(op.kind == TOP_GRAD) {
-					if(DUAL) {
+					if constexpr(DUAL) {
 						double acc = 0.0;
 						if(act) {
+							if constexpr(GCACHE) {
+								if(!gvalid) {
 #pragma unroll
-							for(int r = 0; r < NR; ++r)
-								if((lidx[r] & op.lcond) == op.lcond)
-									acc += (double)vb[r].x * (double)va[r].y - (double)vb[r].y * (double)va[r].x;
+									for(int r = 0; r < NR; ++r)
+										gt[r] = (double)vb[r].x * (double)va[r].y - (double)vb[r].y * (double)va[r].x;
+									gvalid = true;
+								}
+#pragma unroll
+								for(int r = 0; r < NR; ++r)
+									if((lidx[r] & op.lcond) == op.lcond) acc += gt[r];
+							} else {
+#pragma unroll
+								for(int r = 0; r < NR; ++r)
+									if((lidx[r] & op.lcond) == op.lcond)
+										acc += (double)vb[r].x * (double)va[r].y - (double)vb[r].y * (double)va[r].x;
+							}
 						}
 						acc = wave_sum_dpp(acc);  // one per gradient op and tile: DPP, not LDS permutes
 						if(lane == 0) gacc[wave * kMaxGradSlots + op.mat] += acc;
@@ -629,6 +648,7 @@ __device__ __forceinline__ void staged_tile(typename V2T<R>::t *__restrict__ a, 
 					continue;
 				}
 				if(!act) continue;
+				gvalid = false;
 				if constexpr(DUAL && sizeof(R) == 8)  // the planner emits phase runs for complex128 dual passes only
 					if(op.kind == TOP_PHASES) {
 					// one angle sum and one sincos per amplitude, shared by both states
